@@ -1,0 +1,295 @@
+#!/usr/bin/env python3
+"""Benchmark: time-to-1e-6 FP64 residual of the refined hipPCG solve on the
+3D lognormal pressure Poisson 128^3, plus the SpMV / fused-Chebyshev kernel
+bandwidth against the MI355X HBM roofline, the plain-FP64 GPU solve beside it,
+and the same-host CPU PCG(DIC) baseline.
+
+One step = one complete solve from psi = 0 to the FP64 L1 residual 1e-6
+(refined: FP32 Chebyshev-Jacobi PCG inner iterations, constant-mode deflation,
+FP64 refinement on the GPU).  Inputs are resident in HBM before timing starts.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload poisson128]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("time-to-1e-6 FP64 residual & SpMV HBM GB/s (%roofline), "
+          "refined vs plain FP64")
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (generator, solver, description)
+    "poisson128": ("config1", "pcg",
+                   "3D pressure Poisson 128^3 (2,097,152 cells, 6,242,304 faces), "
+                   "lognormal face coefficients, cell 0 pinned; hipPCG refined"),
+    "convdiff96": ("config2", "bicg",
+                   "3D upwind convection-diffusion 96^3 (884,736 cells), asymmetric; "
+                   "hipPBiCGStab refined"),
+    "poisson2d100": ("config0", "pcg",
+                     "2D pressure Poisson 100x100x1 (10,000 cells), unit faces; "
+                     "hipPCG refined"),
+}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def numpy_residual(m, x):
+    """sum|b - A x| / normFactor with plain numpy (independent of the GPU)."""
+    lower = m.upper if m.lower is None else m.lower
+    ax = m.diag * x
+    ax += np.bincount(m.lower_addr, weights=m.upper * x[m.upper_addr], minlength=m.n)
+    ax += np.bincount(m.upper_addr, weights=lower * x[m.lower_addr], minlength=m.n)
+    u = m.diag + np.bincount(m.lower_addr, weights=m.upper, minlength=m.n) + np.bincount(
+        m.upper_addr, weights=lower, minlength=m.n)
+    return np.abs(m.source - ax).sum(), u
+
+
+def spmv_bytes(n, nf, symmetric, extra_vectors=0):
+    """Algorithmic bytes of one FP32 row-gather LDU SpMV: diag, x, y, the two
+    start arrays, and per face the coefficient(s) and the two address arrays
+    the row gather reads (every array counted once)."""
+    face = 16 if symmetric else 20
+    return 20 * n + 8 + face * nf + 4 * extra_vectors * n
+
+
+def cheb_step_bytes(n, nf, symmetric):
+    """Interior fused Chebyshev step: SpMV of z_in (diag, z_in, starts, faces)
+    plus reads r, rD, w and writes w, z_out (y itself is never stored)."""
+    face = 16 if symmetric else 20
+    return 36 * n + 8 + face * nf
+
+
+def cpu_baseline(m, solver, sample_iters):
+    """Oracle PCG(DIC) / PBiCGStab(DILU) in FP64 on one host core, timed on a
+    bounded number of iterations and scaled to the iteration count of the full
+    solve (committed fixture written by the oracle)."""
+    import oracle
+
+    full = json.load(open(os.path.join(ROOT, "tests", "golden", "cpu_baseline_iterations.json")))
+    key = m.name
+    t0 = time.perf_counter()
+    if solver == "pcg":
+        _, perf = oracle.pcg(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=sample_iters)
+    else:
+        _, perf = oracle.pbicgstab(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=sample_iters)
+    dt = time.perf_counter() - t0
+    iters_full = full.get(key, {}).get("iterations")
+    done = perf.nIterations
+    if perf.converged or iters_full is None:
+        est = dt
+        iters_full = done
+    else:
+        est = dt / done * iters_full
+    return {
+        "value": round(1e3 * est, 3),
+        "unit": "ms",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{'PCG(DIC)' if solver == 'pcg' else 'PBiCGStab(DILU)'} FP64, 1 core, "
+                   f"{done} iterations timed ({dt:.2f} s) and scaled to the {iters_full} "
+                   f"iterations the full solve to 1e-6 takes"),
+        "cpu": platform.processor() or platform.machine(),
+        "nproc": os.cpu_count(),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="poisson128", choices=sorted(WORKLOADS))
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--cpu-iters", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build runs on one GPU: the LDU solve is not "
+                          "sharded across GPUs (see DESIGN.md for how it would shard); "
+                          "run with --gpus 1"}))
+        sys.exit(2)
+
+    from metal4foam_amd import hipfoam as H
+    from metal4foam_amd import meshgen as mg
+
+    gen, solver, desc = WORKLOADS[args.workload]
+    m = getattr(mg, gen)()
+    sym = m.lower is None
+    kind = H.PCG if solver == "pcg" else H.PBICGSTAB
+    log(f"workload {m.name}: n={m.n} faces={m.nfaces}")
+
+    ctx = H.Context(0)
+    log(ctx.description)
+    ctx.set_topology(m)
+    D = H.DeviceArray
+    d_diag = D.from_host(ctx, m.diag)
+    d_upper = D.from_host(ctx, m.upper)
+    d_lower = None if sym else D.from_host(ctx, m.lower)
+    d_src = D.from_host(ctx, m.source)
+    d_zero = D.from_host(ctx, np.zeros(m.n))
+    d_psi = D.from_host(ctx, np.zeros(m.n))
+    ctl = H.default_controls(maxRestarts=8, maxIter=20000)
+
+    def step(fp64=False):
+        d_psi.copy_from(d_zero)
+        return ctx.solve_device(kind, d_diag, d_upper, d_lower, d_psi, d_src, ctl, fp64=fp64)
+
+    for _ in range(args.warmup):
+        perf = step()
+    ctx.synchronize()
+    log("warmup done")
+
+    # ---- timed: refined solve -----------------------------------------------------
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    perfs = [step() for _ in range(args.steps)]
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms = 1e3 * (t1 - t0) / max(args.steps, 1)
+    perf = perfs[-1]
+    psi = d_psi.download()
+    rsum, _ = numpy_residual(m, psi)
+    true_res = rsum / perf.normFactor
+    log(f"refined: {ms:.3f} ms/solve, iters={perf.nIterations} restarts={perf.restarts} "
+        f"res={perf.finalResidual:.3e} (numpy {true_res:.3e}) converged={perf.converged}")
+    if not (perf.converged and true_res < 1e-6):
+        log("ERROR: refined solve did not reach 1e-6")
+
+    # ---- plain FP64 comparator ----------------------------------------------------
+    fp64_ms = None
+    fp64_perf = None
+    if args.fp64_steps > 0:
+        step(fp64=True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.fp64_steps):
+            fp64_perf = step(fp64=True)
+        ctx.synchronize()
+        fp64_ms = 1e3 * (time.perf_counter() - t0) / args.fp64_steps
+        log(f"plain fp64: {fp64_ms:.3f} ms/solve, iters={fp64_perf.nIterations} "
+            f"res={fp64_perf.finalResidual:.3e}")
+
+    # ---- kernel roofline: per-launch HIP events on the library stream -------------
+    dom_class = 0 if solver == "pcg" else 2
+    ctx.set_kernel_timing(True, dom_class)
+    step()
+    nl, secs = ctx.kernel_timing()
+    ctx.set_kernel_timing(False, 0)
+    if solver == "pcg":
+        kbytes = cheb_step_bytes(m.n, m.nfaces, sym)
+        kname = "k_cheb_step<float, false, false> (interior Chebyshev step fused with its SpMV)"
+    else:
+        kbytes = spmv_bytes(m.n, m.nfaces, sym, extra_vectors=2)
+        kname = "k_bicg_spmv_alpha / k_bicg_z_spmv (BiCGStab SpMV + dot)"
+    kavg = secs / max(nl, 1)
+    kgbs = kbytes / kavg / 1e9 if nl else 0.0
+    log(f"dominant kernel {kname}: {nl} launches, avg {1e6 * kavg:.2f} us, "
+        f"{kgbs:.0f} GB/s ({100 * kgbs / HBM_PEAK_GBS:.1f}% of HBM peak)")
+
+    # plain FP32 SpMV of the inner operator (the metric's SpMV GB/s)
+    x = D.from_host(ctx, np.random.default_rng(0).standard_normal(m.n).astype(np.float32))
+    y = D(ctx, (m.n,), np.float32)
+    ctx.set_kernel_timing(True, 5)
+    for _ in range(args.spmv_reps):
+        ctx.amul_device(x, y, H.FP32)
+    ns, ssecs = ctx.kernel_timing()
+    ctx.set_kernel_timing(False, 0)
+    sbytes = spmv_bytes(m.n, m.nfaces, sym)
+    savg = ssecs / max(ns, 1)
+    sgbs = sbytes / savg / 1e9 if ns else 0.0
+    log(f"plain FP32 SpMV: avg {1e6 * savg:.2f} us, {sgbs:.0f} GB/s")
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(m, solver, args.cpu_iters)
+        log(f"cpu baseline: {cpu['value']} ms ({cpu['sample']})")
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": desc,
+            "cells": m.n,
+            "faces": m.nfaces,
+            "tolerance": 1e-6,
+            "inner": "FP32 Chebyshev(8)-Jacobi PCG" if solver == "pcg" else
+                     "FP32 Jacobi BiCGStab",
+            "refinement": "FP64 on GPU",
+            "schedule": ctx.get_schedule().as_dict(),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(kgbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(kgbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": kname,
+            "bytes_per_launch": kbytes,
+            "avg_launch_us": round(1e6 * kavg, 3),
+            "launches": nl,
+        },
+        "spmv": {
+            "kernel": "k_spmv<float> (plain FP32 LDU SpMV)",
+            "bytes_per_launch": sbytes,
+            "avg_launch_us": round(1e6 * savg, 3),
+            "GB_s": round(sgbs, 1),
+            "frac": round(sgbs / HBM_PEAK_GBS, 4),
+        },
+        "solve": {
+            "iterations": perf.nIterations,
+            "restarts": perf.restarts,
+            "final_residual": perf.finalResidual,
+            "numpy_residual": true_res,
+            "converged": bool(perf.converged and true_res < 1e-6),
+            "deflated": perf.deflated,
+            "polished": perf.polished,
+        },
+        "fp64_plain": None if fp64_perf is None else {
+            "ms": round(fp64_ms, 4),
+            "iterations": fp64_perf.nIterations,
+            "final_residual": fp64_perf.finalResidual,
+            "converged": bool(fp64_perf.converged),
+            "refined_speedup": round(fp64_ms / ms, 3),
+        },
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(cpu["value"] / ms, 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

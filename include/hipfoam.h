@@ -1,0 +1,251 @@
+/*
+ * hipfoam.h -- C-ABI of the MI355X (gfx950) LDU Krylov library behind the
+ * hipPCG / hipPBiCGStab OpenFOAM solvers.
+ *
+ * Plain C: opaque context handle, plain pointers and sizes, int status codes
+ * (0 = ok).  Host pointers are caller-owned and copied; device state is
+ * library-owned.  A context is not thread-safe; every export restores the
+ * caller's floating-point environment on return (FP traps are masked while
+ * the HIP runtime runs).  Entry points whose names end in _device take HIP
+ * device pointers and leave data resident in HBM.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (scotCW/metal4Foam; B = src/metal4Foam/backend/metalFoamBackend,
+ *  S = src/metal4Foam/solvers/metalFoamSolver).
+ */
+#ifndef HIPFOAM_H
+#define HIPFOAM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HIPFOAM_ABI_VERSION 1
+
+/* status codes */
+#define HIPFOAM_OK 0
+#define HIPFOAM_ERR_ARG 1      /* bad argument (NULL, size, order)      */
+#define HIPFOAM_ERR_DEVICE 2   /* HIP runtime error / no usable device  */
+#define HIPFOAM_ERR_STATE 3    /* call out of order (no topology, ...)  */
+#define HIPFOAM_ERR_SIZE 4     /* size beyond a documented maximum      */
+
+/* solver selectors */
+#define HIPFOAM_PCG 0          /* metalPCG -> hipPCG                    */
+#define HIPFOAM_PBICGSTAB 1    /* metalPBiCGStab -> hipPBiCGStab        */
+
+/* precision selectors */
+#define HIPFOAM_FP32 0
+#define HIPFOAM_FP64 1
+
+typedef struct hipfoam_context hipfoam_context;
+
+/* Inner-solve statistics: metalFoamBackend::Stats (B.H:41-50). */
+typedef struct
+{
+    int32_t nIterations;   /* inner FP32 iterations performed          */
+    double finalResSum;    /* FP32 sum|r| at exit (unnormalised)       */
+    int32_t converged;     /* reached targetResSum                     */
+    int32_t singular;      /* NaN/Inf in the scalars (breakdown)       */
+    int32_t stagnated;     /* 50 iterations without improvement        */
+    double tGpuSeconds;    /* GPU time of the iteration loop           */
+    double tWallSeconds;   /* wall time inside the iteration loop      */
+} hipfoam_stats;
+
+/* fvSolution controls of one solve: the lduMatrix::solver controls plus the
+ * "hip" sub-dictionary (mirror of the reference's "metal" sub-dictionary,
+ * S.C:32-52, 170-171, 224-227, 259-263). */
+typedef struct
+{
+    double tolerance;          /* default 1e-6                            */
+    double relTol;             /* default 0                               */
+    int32_t maxIter;           /* default 1000 (caps total inner iters)   */
+    int32_t minIter;           /* default 0                               */
+    int32_t deflate;           /* 0 off, 1 on, 2 auto (default)           */
+    int32_t chebyshevOrder;    /* default 8 (PCG); 1 = Jacobi             */
+    double chebyshevRatio;     /* default 30                              */
+    int32_t maxRestarts;       /* default 8                               */
+    int32_t iterationsPerSync; /* default 1                               */
+    int32_t verbose;           /* per-restart trace on stdout             */
+} hipfoam_controls;
+
+/* Result of one solve: OpenFOAM solverPerformance + profile (S.C:127-129,
+ * 363-365, 417-426). */
+typedef struct
+{
+    double initialResidual;
+    double finalResidual;
+    double normFactor;
+    int32_t nIterations;  /* total inner iterations                     */
+    int32_t restarts;     /* index of the last outer restart            */
+    int32_t converged;
+    int32_t singular;
+    int32_t deflated;     /* constant-mode deflation was used           */
+    int32_t polished;     /* switched to the exact operator             */
+    double chebBeta;      /* Gershgorin bound used by Chebyshev         */
+    double tConvert;      /* seconds: coefficient staging               */
+    double tInner;        /* seconds: inner Krylov loops (wall)         */
+    double tGpu;          /* seconds: inner Krylov loops (GPU events)   */
+    double tTotal;        /* seconds: whole call                        */
+} hipfoam_performance;
+
+/* Fills the reference's defaults (S.C:170-263). */
+void hipfoam_default_controls(hipfoam_controls* c);
+
+/* ---- context -------------------------------------------------------- */
+
+/* metalFoamBackend::metalFoamBackend (B.C:422-430).  device = HIP ordinal.
+ * Always returns a context (check hipfoam_good); NULL only on OOM. */
+int32_t hipfoam_create(int32_t device, hipfoam_context** out);
+/* metalFoamBackend::~metalFoamBackend (B.C:433). */
+void hipfoam_destroy(hipfoam_context* ctx);
+/* metalFoamBackend::good (B.H:105-106). */
+int32_t hipfoam_good(const hipfoam_context* ctx);
+/* metalFoamBackend::description (B.H:108-109). */
+const char* hipfoam_description(const hipfoam_context* ctx);
+/* Message of the last failing call on this context ("" if none). */
+const char* hipfoam_last_error(const hipfoam_context* ctx);
+
+/* ---- backend (inner) interface: metalFoamBackend -------------------- */
+
+/* metalFoamBackend::setTopology (B.C:454-519).  Uploads int32 addressing
+ * once; a no-op when (nCells, nFaces, lowAddr, uppAddr) are unchanged. */
+int32_t hipfoam_set_topology(hipfoam_context* ctx, int32_t nCells,
+                             int32_t nFaces, const int32_t* lowAddr,
+                             const int32_t* uppAddr, const int32_t* ownStart,
+                             const int32_t* losort,
+                             const int32_t* losortStart);
+
+/* metalFoamBackend::setCoeffs (B.C:522-565): FP64 -> FP32 operator of the
+ * inner solve, rD = float(1/diag).  lower == NULL selects the symmetric
+ * path.  *seconds (may be NULL) receives the staging time. */
+int32_t hipfoam_set_coeffs(hipfoam_context* ctx, const double* diag,
+                           const double* upper, const double* lower,
+                           double* seconds);
+
+/* metalFoamBackend::pcg (B.C:568-732): FP32 PCG on A delta = r, delta0 = 0,
+ * Chebyshev-Jacobi (chebOrder > 1) or Jacobi. */
+int32_t hipfoam_pcg(hipfoam_context* ctx, const double* r,
+                    double targetResSum, int32_t maxIter,
+                    int32_t iterationsPerSync, int32_t chebOrder,
+                    double chebBeta, double chebRatio, hipfoam_stats* stats);
+
+/* metalFoamBackend::bicgstab (B.C:735-959): FP32 Jacobi BiCGStab. */
+int32_t hipfoam_bicgstab(hipfoam_context* ctx, const double* r,
+                         double targetResSum, int32_t maxIter,
+                         hipfoam_stats* stats);
+
+/* metalFoamBackend::delta (B.C:448-451): copies the FP32 correction of the
+ * last pcg/bicgstab call into out[nCells]. */
+int32_t hipfoam_delta(hipfoam_context* ctx, float* out);
+
+/* ---- solver (outer) interface: metalFoamSolver::solveImpl ------------ */
+
+/* metalFoamSolver::solveImpl (S.C:77-437) with every FP64 step on the GPU:
+ * initial residual and normFactor, sumA, deflation, Gershgorin bound, FP32
+ * inner solves, psi += delta, true FP64 residual, constant-mode correction,
+ * hybrid polish.  solver = HIPFOAM_PCG | HIPFOAM_PBICGSTAB.  lower == NULL
+ * for symmetric matrices.  psi is updated in place.  residual (may be NULL)
+ * receives the final FP64 residual b - A psi (setResidualField, S.C:429). */
+int32_t hipfoam_solve(hipfoam_context* ctx, int32_t solver,
+                      const double* diag, const double* upper,
+                      const double* lower, double* psi, const double* source,
+                      const hipfoam_controls* controls,
+                      hipfoam_performance* perf, double* residual);
+
+/* Same, with every array a device pointer (data stays in HBM). */
+int32_t hipfoam_solve_device(hipfoam_context* ctx, int32_t solver,
+                             const double* diag, const double* upper,
+                             const double* lower, double* psi,
+                             const double* source,
+                             const hipfoam_controls* controls,
+                             hipfoam_performance* perf, double* residual);
+
+/* Plain-FP64 GPU comparator (no reference counterpart; the "plain FP64"
+ * side of the metric): OpenFOAM's PCG / PBiCGStab loop in FP64 on the exact
+ * operator, with the same preconditioner (Chebyshev-Jacobi of
+ * controls->chebyshevOrder for PCG, Jacobi for PBiCGStab), the same
+ * normFactor and checkConvergence, no refinement. */
+int32_t hipfoam_solve_fp64(hipfoam_context* ctx, int32_t solver,
+                           const double* diag, const double* upper,
+                           const double* lower, double* psi,
+                           const double* source,
+                           const hipfoam_controls* controls,
+                           hipfoam_performance* perf, double* residual);
+int32_t hipfoam_solve_fp64_device(hipfoam_context* ctx, int32_t solver,
+                                  const double* diag, const double* upper,
+                                  const double* lower, double* psi,
+                                  const double* source,
+                                  const hipfoam_controls* controls,
+                                  hipfoam_performance* perf,
+                                  double* residual);
+
+/* ---- operator primitives --------------------------------------------- */
+
+/* y = A x with the operator last staged: precision HIPFOAM_FP32 uses the
+ * inner FP32 operator (set_coeffs / solve), HIPFOAM_FP64 the FP64 matrix of
+ * the last solve or hipfoam_set_matrix.  The GPU form of the row-gather LDU
+ * SpMV (metalFoamKernels.metal:62-150) and of lduMatrix::Amul (S.C:109).
+ * x, y: float or double [nCells], host pointers. */
+int32_t hipfoam_amul(hipfoam_context* ctx, int32_t precision, const void* x,
+                     void* y);
+int32_t hipfoam_amul_device(hipfoam_context* ctx, int32_t precision,
+                            const void* x, void* y);
+
+/* Stage an FP64 matrix (and its FP32 inner operator, undeflated) without
+ * solving: the per-solve coefficient upload of S.C:211-220 on its own. */
+int32_t hipfoam_set_matrix(hipfoam_context* ctx, const double* diag,
+                           const double* upper, const double* lower);
+int32_t hipfoam_set_matrix_device(hipfoam_context* ctx, const double* diag,
+                                  const double* upper, const double* lower);
+
+/* ---- device memory helpers for callers that keep fields in HBM ------- */
+int32_t hipfoam_device_alloc(hipfoam_context* ctx, uint64_t bytes,
+                             void** ptr);
+int32_t hipfoam_device_free(hipfoam_context* ctx, void* ptr);
+int32_t hipfoam_copy_to_device(hipfoam_context* ctx, void* dst,
+                               const void* src, uint64_t bytes);
+int32_t hipfoam_copy_to_host(hipfoam_context* ctx, void* dst,
+                             const void* src, uint64_t bytes);
+int32_t hipfoam_copy_on_device(hipfoam_context* ctx, void* dst,
+                               const void* src, uint64_t bytes);
+int32_t hipfoam_synchronize(hipfoam_context* ctx);
+
+/* ---- schedule / measurement ------------------------------------------ */
+
+/* SpMV schedules of the inner operator:
+ *   0 = losort-ordered lower coefficients (one indirection),
+ *   1 = direct lower[losort[k]] / lowAddr[losort[k]] (two indirections),
+ * and the iteration driver:
+ *   graphChunk = iterations captured per hipGraph launch (0 = eager). */
+typedef struct
+{
+    int32_t spmvSchedule;
+    int32_t blockSize;    /* 64 .. 1024, multiple of 64 */
+    int32_t xcdRemap;     /* 1 = contiguous row range per XCD */
+    int32_t graphChunk;
+} hipfoam_schedule;
+
+int32_t hipfoam_get_schedule(const hipfoam_context* ctx, hipfoam_schedule* s);
+int32_t hipfoam_set_schedule(hipfoam_context* ctx, const hipfoam_schedule* s);
+/* Times the candidate SpMV schedules on the staged operator and keeps the
+ * fastest; *best (may be NULL) receives it. */
+int32_t hipfoam_autotune(hipfoam_context* ctx, hipfoam_schedule* best);
+
+/* Kernel timing: when enabled, the inner loops launch without graphs and
+ * bracket every launch of kernel class `kclass` with HIP events on the
+ * context's stream.  hipfoam_kernel_timing returns the launch count and the
+ * summed seconds since the last reset.  kclass: 0 = interior Chebyshev
+ * SpMV step, 1 = p SpMV (A p), 2 = BiCGStab SpMV, 3 = first Chebyshev step,
+ * 4 = last Chebyshev step, 5 = plain SpMV of hipfoam_amul*. */
+int32_t hipfoam_set_kernel_timing(hipfoam_context* ctx, int32_t enable,
+                                  int32_t kclass);
+int32_t hipfoam_kernel_timing(hipfoam_context* ctx, int64_t* launches,
+                              double* seconds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPFOAM_H */

@@ -1,0 +1,1336 @@
+// hipfoam_api.hip -- host side of the MI355X LDU Krylov library: context,
+// device-resident inner loops (hipGraph chunks), the FP64-on-GPU refined
+// solve, and the extern "C" exports declared in include/hipfoam.h.
+//
+// Reference interfaces replaced (scotCW/metal4Foam):
+//   metalFoamBackend   src/metal4Foam/backend/metalFoamBackend.{H,C}
+//   metalFoamSolver    src/metal4Foam/solvers/metalFoamSolver.C:77-437
+#include <hip/hip_runtime.h>
+
+#include <fenv.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/hipfoam.h"
+#include "hipfoam_device.hpp"
+
+using namespace hipfoam;
+
+namespace {
+
+double now_seconds() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// OpenFOAM traps SIGFPE; run library code with the default FP environment
+// and restore the caller's on exit (metalFoamBackend.C:48-57).
+struct FpTrapGuard {
+  fenv_t saved;
+  FpTrapGuard() {
+    fegetenv(&saved);
+    fesetenv(FE_DFL_ENV);
+  }
+  ~FpTrapGuard() { fesetenv(&saved); }
+};
+
+template <typename T>
+struct Buf {
+  T* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t n) {
+    if (n == 0) n = 1;
+    if (p && n <= cap) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      cap = 0;
+      return e;
+    }
+    cap = n;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Krylov workspace of one precision.
+template <typename T>
+struct Work {
+  Buf<T> r, p0, p1, wA, Ap, delta, deltaBest, z0, z1, w, rA0, yA, AyA, sA, zA, tA;
+  size_t n = 0;
+  hipError_t ensure(size_t nn) {
+    if (n == nn && r.p) return hipSuccess;
+    Buf<T>* all[] = {&r, &p0, &p1, &wA, &Ap, &delta, &deltaBest, &z0,
+                     &z1, &w, &rA0, &yA, &AyA, &sA, &zA, &tA};
+    for (auto* b : all) {
+      b->release();
+      hipError_t e = b->ensure(nn);
+      if (e != hipSuccess) return e;
+    }
+    n = nn;
+    return hipSuccess;
+  }
+  void release() {
+    Buf<T>* all[] = {&r, &p0, &p1, &wA, &Ap, &delta, &deltaBest, &z0,
+                     &z1, &w, &rA0, &yA, &AyA, &sA, &zA, &tA};
+    for (auto* b : all) b->release();
+    n = 0;
+  }
+};
+
+enum Kind { KIND_PCG = 0, KIND_BICG = 1 };
+
+struct GraphKey {
+  int kind, prec, order, chunk, direct, remap, block, gen;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(kind, prec, order, chunk, direct, remap, block, gen) <
+           std::tie(o.kind, o.prec, o.order, o.chunk, o.direct, o.remap, o.block, o.gen);
+  }
+};
+
+// Per-launch HIP-event timing of one kernel class on the context stream.
+struct KernelTimer {
+  bool on = false;
+  int kclass = 0;
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  int64_t launches = 0;
+  double seconds = 0;
+};
+
+}  // namespace
+
+struct hipfoam_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool good = false;
+  std::string desc = "hipfoam: no device";
+  std::string err;
+  int numCUs = 0;
+
+  // topology
+  int n = 0, nf = 0;
+  const int32_t* lastLow = nullptr;
+  const int32_t* lastUpp = nullptr;
+  int gen = 0;  // bumps whenever device buffers are reallocated
+  Buf<int32_t> ownStart, losortStart, uppAddr, lowAddr, losort, losortLow;
+
+  // FP64 matrix (staged per solve) and its FP32 inner operator
+  Buf<double> diag64, upper64, lower64, lowerL64, rD64;
+  bool sym64 = true;
+  bool have64 = false;
+  Buf<float> diag32, upper32, lowerL32, lower32, rD32;
+  bool sym32 = true;
+  bool have32 = false;
+
+  // FP64 outer vectors
+  Buf<double> psi0, psi1, b, r64, u64, scratch64;
+  Work<float> w32;
+  Work<double> w64;
+
+  Buf<double> partials;
+  int pstride = 0;
+  Buf<KState> kst;
+  Buf<OState> ost;
+  KState* hk = nullptr;  // pinned
+  OState* ho = nullptr;  // pinned
+  int32_t* hstatus = nullptr;  // pinned, 2 slots
+  hipEvent_t evPoll[2] = {nullptr, nullptr};
+  hipEvent_t evT0 = nullptr, evT1 = nullptr;
+
+  hipfoam_schedule sched{0, 256, 1, 8};
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  KernelTimer timer;
+};
+
+namespace {
+
+#define CK(expr)                                                              \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      c->err = std::string(#expr) + ": " + hipGetErrorString(e_);             \
+      return HIPFOAM_ERR_DEVICE;                                              \
+    }                                                                         \
+  } while (0)
+
+#define CKL()                                                                 \
+  do {                                                                        \
+    hipError_t e_ = hipGetLastError();                                        \
+    if (e_ != hipSuccess) {                                                   \
+      c->err = std::string("kernel launch: ") + hipGetErrorString(e_);        \
+      return HIPFOAM_ERR_DEVICE;                                              \
+    }                                                                         \
+  } while (0)
+
+int fail(hipfoam_context* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+inline int blocks_for(int n, int bs) { return (n + bs - 1) / bs; }
+
+void clear_graphs(hipfoam_context* c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+}
+
+template <typename T>
+LduOp<T> make_op(hipfoam_context* c, const T* diag, const T* upper, const T* lowerL,
+                 const T* lowerF) {
+  LduOp<T> op;
+  op.ownStart = c->ownStart.p;
+  op.losortStart = c->losortStart.p;
+  op.uppAddr = c->uppAddr.p;
+  op.losort = c->losort.p;
+  op.diag = diag;
+  op.upper = upper;
+  op.n = c->n;
+  op.xcdRemap = c->sched.xcdRemap;
+  op.direct = c->sched.spmvSchedule == 1;
+  if (op.direct) {
+    op.nbrAddr = c->lowAddr.p;
+    op.lowerL = lowerF;
+  } else {
+    op.nbrAddr = c->losortLow.p;
+    op.lowerL = lowerL;
+  }
+  return op;
+}
+
+LduOp<float> op32(hipfoam_context* c) {
+  return make_op<float>(c, c->diag32.p, c->upper32.p, c->lowerL32.p,
+                        c->sym32 ? c->upper32.p : c->lower32.p);
+}
+
+LduOp<double> op64(hipfoam_context* c) {
+  return make_op<double>(c, c->diag64.p, c->upper64.p, c->lowerL64.p,
+                         c->sym64 ? c->upper64.p : c->lower64.p);
+}
+
+// Host copy of the Chebyshev constants (encCheb, metalFoamBackend.C:349-372).
+void cheb_constants(KState& s, int order, double beta, double ratio, bool round32) {
+  const double alpha = beta / ratio;
+  const double theta = 0.5 * (beta + alpha);
+  const double delta = 0.5 * (beta - alpha);
+  const double sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  auto rd = [&](double x) { return round32 ? (double)(float)x : x; };
+  s.chebC1 = rd(1.0 / theta);
+  for (int i = 1; i < order && i <= kMaxCheb; ++i) {
+    const double rhoNew = 1.0 / (2.0 * sigma - rho);
+    s.chebC2[i - 1] = rd(rhoNew * rho);
+    s.chebC3[i - 1] = rd(2.0 * rhoNew / delta);
+    rho = rhoNew;
+  }
+}
+
+template <typename T>
+Work<T>& work(hipfoam_context* c);
+template <>
+Work<float>& work<float>(hipfoam_context* c) { return c->w32; }
+template <>
+Work<double>& work<double>(hipfoam_context* c) { return c->w64; }
+
+// -------------------------------------------------------------------------
+// kernel timing hooks
+// -------------------------------------------------------------------------
+
+int timer_flush(hipfoam_context* c) {
+  KernelTimer& t = c->timer;
+  if (t.used == 0) return HIPFOAM_OK;
+  CK(hipEventSynchronize(t.ev[t.used - 1]));
+  for (size_t k = 0; k + 1 < t.used; k += 2) {
+    float ms = 0.f;
+    CK(hipEventElapsedTime(&ms, t.ev[k], t.ev[k + 1]));
+    t.seconds += 1e-3 * ms;
+    t.launches += 1;
+  }
+  t.used = 0;
+  return HIPFOAM_OK;
+}
+
+struct TimedLaunch {
+  hipfoam_context* c;
+  bool active;
+  TimedLaunch(hipfoam_context* cc, int kclass, bool capturing) : c(cc) {
+    KernelTimer& t = c->timer;
+    active = t.on && !capturing && t.kclass == kclass;
+    if (!active) return;
+    if (t.used + 2 > t.ev.size()) timer_flush(c);
+    (void)hipEventRecord(t.ev[t.used], c->stream);
+  }
+  ~TimedLaunch() {
+    if (!active) return;
+    KernelTimer& t = c->timer;
+    (void)hipEventRecord(t.ev[t.used + 1], c->stream);
+    t.used += 2;
+  }
+};
+
+// -------------------------------------------------------------------------
+// one Krylov iteration, enqueued on the context stream
+// -------------------------------------------------------------------------
+
+template <typename T>
+void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order,
+                      int parity, bool capturing) {
+  Work<T>& w = work<T>(c);
+  const int bs = c->sched.blockSize;
+  const dim3 grid(blocks_for(c->n, bs)), block(bs);
+  KState* st = c->kst.p;
+  double* P = c->partials.p;
+  const int ps = c->pstride;
+  if (order > 1) {
+    const T* zin = nullptr;
+    for (int s = 1; s < order; ++s) {
+      const bool first = (s == 1), last = (s == order - 1);
+      T* zout = last ? w.wA.p : ((s & 1) ? w.z0.p : w.z1.p);
+      TimedLaunch tl(c, first ? 3 : (last ? 4 : 0), capturing);
+      if (first && last)
+        hipLaunchKernelGGL((k_cheb_step<T, true, true>), grid, block, 0, c->stream, A, w.r.p,
+                           rD, zin, w.w.p, zout, st, P, ps, s);
+      else if (first)
+        hipLaunchKernelGGL((k_cheb_step<T, true, false>), grid, block, 0, c->stream, A, w.r.p,
+                           rD, zin, w.w.p, zout, st, P, ps, s);
+      else if (last)
+        hipLaunchKernelGGL((k_cheb_step<T, false, true>), grid, block, 0, c->stream, A, w.r.p,
+                           rD, zin, w.w.p, zout, st, P, ps, s);
+      else
+        hipLaunchKernelGGL((k_cheb_step<T, false, false>), grid, block, 0, c->stream, A,
+                           w.r.p, rD, zin, w.w.p, zout, st, P, ps, s);
+      zin = zout;
+    }
+  } else {
+    hipLaunchKernelGGL((k_jacobi_dot<T>), grid, block, 0, c->stream, c->n, A.xcdRemap, w.r.p,
+                       rD, w.wA.p, st, P, ps);
+  }
+  T* pold = parity ? w.p1.p : w.p0.p;
+  T* pnew = parity ? w.p0.p : w.p1.p;
+  {
+    TimedLaunch tl(c, 1, capturing);
+    hipLaunchKernelGGL((k_pcg_p_spmv<T>), grid, block, 0, c->stream, A, w.wA.p, pold, pnew,
+                       w.Ap.p, st, P, ps);
+  }
+  hipLaunchKernelGGL((k_pcg_update<T>), grid, block, 0, c->stream, c->n, A.xcdRemap,
+                     w.delta.p, w.deltaBest.p, w.r.p, pnew, w.Ap.p, st, P, ps);
+}
+
+template <typename T>
+void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, bool capturing) {
+  Work<T>& w = work<T>(c);
+  const int bs = c->sched.blockSize;
+  const dim3 grid(blocks_for(c->n, bs)), block(bs);
+  KState* st = c->kst.p;
+  double* P = c->partials.p;
+  const int ps = c->pstride;
+  const int n = c->n, rm = A.xcdRemap;
+  hipLaunchKernelGGL((k_bicg_rho<T>), grid, block, 0, c->stream, n, rm, w.rA0.p, w.r.p, st, P,
+                     ps);
+  hipLaunchKernelGGL((k_bicg_p_y<T>), grid, block, 0, c->stream, n, rm, w.p0.p, w.r.p, w.AyA.p,
+                     rD, w.yA.p, st);
+  {
+    TimedLaunch tl(c, 2, capturing);
+    hipLaunchKernelGGL((k_bicg_spmv_alpha<T>), grid, block, 0, c->stream, A, w.yA.p, w.rA0.p,
+                       w.AyA.p, st, P, ps);
+  }
+  hipLaunchKernelGGL((k_bicg_s<T>), grid, block, 0, c->stream, n, rm, w.sA.p, w.r.p, w.AyA.p,
+                     st, P, ps);
+  {
+    TimedLaunch tl(c, 2, capturing);
+    hipLaunchKernelGGL((k_bicg_z_spmv<T>), grid, block, 0, c->stream, A, rD, w.sA.p, w.zA.p,
+                       w.tA.p, st, P, ps);
+  }
+  hipLaunchKernelGGL((k_bicg_update<T>), grid, block, 0, c->stream, n, rm, w.delta.p,
+                     w.deltaBest.p, w.yA.p, w.zA.p, w.r.p, w.sA.p, w.tA.p, st, P, ps);
+}
+
+template <typename T>
+void enqueue_chunk(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
+                   int chunk, bool capturing) {
+  for (int k = 0; k < chunk; ++k) {
+    if (kind == KIND_PCG)
+      enqueue_pcg_iter<T>(c, A, rD, order, k & 1, capturing);
+    else
+      enqueue_bicg_iter<T>(c, A, rD, capturing);
+  }
+}
+
+template <typename T>
+int get_graph(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
+              int chunk, hipGraphExec_t* out) {
+  GraphKey key{(int)kind, (int)sizeof(T), order, chunk, A.direct, A.xcdRemap,
+               c->sched.blockSize, c->gen};
+  auto it = c->graphs.find(key);
+  if (it != c->graphs.end()) {
+    *out = it->second;
+    return HIPFOAM_OK;
+  }
+  hipGraph_t g = nullptr;
+  CK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_chunk<T>(c, kind, A, rD, order, chunk, true);
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess) return fail(c, HIPFOAM_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(e));
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(c, HIPFOAM_ERR_DEVICE, std::string("instantiate: ") + hipGetErrorString(e));
+  c->graphs[key] = ex;
+  *out = ex;
+  return HIPFOAM_OK;
+}
+
+// Runs the device-resident iteration loop until the device state leaves
+// RUNNING.  Chunks of `chunk` iterations are launched back to back; the host
+// polls the status word of the chunk before the one just launched, so one
+// chunk stays queued behind the GPU.  In kernel-timing mode every iteration is
+// launched eagerly and polled synchronously, so no timed launch is a no-op.
+// Vectors and *kst must be initialised.
+template <typename T>
+int run_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
+             int maxIter, double* gpuSeconds) {
+  const bool timing = c->timer.on;
+  const bool eager = c->sched.graphChunk <= 0 || timing;
+  const int chunk = timing ? 1 : (c->sched.graphChunk > 0 ? c->sched.graphChunk : 8);
+  hipGraphExec_t ex = nullptr;
+  if (!eager) {
+    int rc = get_graph<T>(c, kind, A, rD, order, chunk, &ex);
+    if (rc) return rc;
+  }
+  CK(hipEventRecord(c->evT0, c->stream));
+  const long maxChunks = (long)std::max(maxIter, 1) / chunk + 3;
+  int slot = 0;
+  bool havePrev = false;
+  for (long k = 0; k < maxChunks + 2; ++k) {
+    if (eager) {
+      enqueue_chunk<T>(c, kind, A, rD, order, chunk, false);
+      CKL();
+    } else {
+      CK(hipGraphLaunch(ex, c->stream));
+    }
+    CK(hipMemcpyAsync(&c->hstatus[slot], &c->kst.p->status, sizeof(int32_t),
+                      hipMemcpyDeviceToHost, c->stream));
+    CK(hipEventRecord(c->evPoll[slot], c->stream));
+    if (timing) {
+      CK(hipEventSynchronize(c->evPoll[slot]));
+      if (c->timer.used + 64 > c->timer.ev.size()) {
+        int rc = timer_flush(c);
+        if (rc) return rc;
+      }
+      if (c->hstatus[slot] != ST_RUNNING) break;
+      continue;
+    }
+    if (havePrev) {
+      CK(hipEventSynchronize(c->evPoll[slot ^ 1]));
+      if (c->hstatus[slot ^ 1] != ST_RUNNING) break;
+    }
+    havePrev = true;
+    slot ^= 1;
+  }
+  CK(hipEventRecord(c->evT1, c->stream));
+  CK(hipEventSynchronize(c->evT1));
+  float ms = 0.f;
+  CK(hipEventElapsedTime(&ms, c->evT0, c->evT1));
+  if (gpuSeconds) *gpuSeconds += 1e-3 * ms;
+  if (timing) {
+    int rc = timer_flush(c);
+    if (rc) return rc;
+  }
+  CK(hipMemcpy(c->hk, c->kst.p, sizeof(KState), hipMemcpyDeviceToHost));
+  if (c->hk->status == ST_RUNNING)
+    return fail(c, HIPFOAM_ERR_STATE, "inner loop did not terminate within maxIter chunks");
+  return HIPFOAM_OK;
+}
+
+// Inner correction solve in reference-backend semantics
+// (metalFoamBackend::pcg / ::bicgstab): r64 -> FP32 residual (minus *mean
+// when given), delta0 = 0, device loop, best-iterate restore.
+int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* mean,
+                double target, int maxIter, int ips, int order, double beta, double ratio,
+                hipfoam_stats* stats) {
+  const double tw0 = now_seconds();
+  hipfoam_stats st{};
+  if (!c->good || c->n == 0) {
+    st.singular = 1;
+    if (stats) *stats = st;
+    return HIPFOAM_OK;
+  }
+  if (order > kMaxCheb + 1) return fail(c, HIPFOAM_ERR_ARG, "chebyshevOrder > 33");
+  Work<float>& w = c->w32;
+  const int n = c->n;
+  const dim3 g1(blocks_for(n, 256)), b1(256);
+  hipLaunchKernelGGL((k_inner_init<float>), g1, b1, 0, c->stream, n, r64, mean, w.r.p,
+                     kind == KIND_BICG ? w.rA0.p : nullptr, w.delta.p, w.p0.p, w.p1.p,
+                     kind == KIND_BICG ? w.AyA.p : nullptr);
+  CKL();
+  KState s{};
+  s.mode = MODE_INNER;
+  s.target = target;
+  s.maxIter = maxIter;
+  s.ips = ips < 1 ? 1 : ips;
+  s.batchSize = std::min(s.ips, std::max(maxIter, 0));
+  s.batchLeft = s.batchSize;
+  s.bestRes = 1.0e300;
+  s.status = maxIter > 0 ? ST_RUNNING : ST_MAXITER;
+  if (kind == KIND_PCG && order > 1) cheb_constants(s, order, beta, ratio, true);
+  *c->hk = s;
+  CK(hipMemcpyAsync(c->kst.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+  double gpu = 0;
+  if (s.status == ST_RUNNING) {
+    const LduOp<float> A = op32(c);
+    int rc = run_loop<float>(c, kind, A, c->rD32.p, kind == KIND_PCG ? order : 1, maxIter, &gpu);
+    if (rc) return rc;
+  } else {
+    CK(hipStreamSynchronize(c->stream));
+  }
+  const KState& h = *c->hk;
+  st.nIterations = h.it;
+  st.finalResSum = (s.status == ST_RUNNING) ? h.res : 0.0;
+  st.converged = h.status == ST_CONVERGED;
+  st.singular = h.status == ST_SINGULAR;
+  st.stagnated = h.status == ST_STAGNATED;
+  // best iterate (metalFoamBackend.C:718-727); the |sA| exit keeps delta
+  if (s.status == ST_RUNNING && h.bestRes < st.finalResSum) {
+    CK(hipMemcpyAsync(w.delta.p, w.deltaBest.p, sizeof(float) * n, hipMemcpyDeviceToDevice,
+                      c->stream));
+    st.finalResSum = h.bestRes;
+  }
+  st.tGpuSeconds = gpu;
+  st.tWallSeconds = now_seconds() - tw0;
+  if (stats) *stats = st;
+  return HIPFOAM_OK;
+}
+
+// -------------------------------------------------------------------------
+// staging
+// -------------------------------------------------------------------------
+
+int ensure_vectors(hipfoam_context* c) {
+  const int n = c->n;
+  CK(c->w32.ensure(n));
+  CK(c->psi0.ensure(n));
+  CK(c->psi1.ensure(n));
+  CK(c->b.ensure(n));
+  CK(c->r64.ensure(n));
+  CK(c->u64.ensure(n));
+  CK(c->scratch64.ensure(n));
+  CK(c->diag32.ensure(n));
+  CK(c->rD32.ensure(n));
+  CK(c->diag64.ensure(n));
+  CK(c->rD64.ensure(n));
+  return HIPFOAM_OK;
+}
+
+// Stage an FP64 matrix (copy into library buffers, losort-ordered lower,
+// FP32 faces).  kind: hipMemcpyHostToDevice or hipMemcpyDeviceToDevice.
+int stage_matrix(hipfoam_context* c, const double* diag, const double* upper,
+                 const double* lower, hipMemcpyKind kind) {
+  const int n = c->n, nf = c->nf;
+  if (!diag || !upper) return fail(c, HIPFOAM_ERR_ARG, "diag/upper must not be NULL");
+  CK(c->diag64.ensure(n));
+  CK(c->upper64.ensure(nf));
+  CK(c->lowerL64.ensure(nf));
+  CK(c->upper32.ensure(nf));
+  CK(c->lowerL32.ensure(nf));
+  CK(hipMemcpyAsync(c->diag64.p, diag, sizeof(double) * n, kind, c->stream));
+  CK(hipMemcpyAsync(c->upper64.p, upper, sizeof(double) * nf, kind, c->stream));
+  c->sym64 = (lower == nullptr);
+  c->sym32 = c->sym64;
+  if (lower) {
+    CK(c->lower64.ensure(nf));
+    CK(c->lower32.ensure(nf));
+    CK(hipMemcpyAsync(c->lower64.p, lower, sizeof(double) * nf, kind, c->stream));
+  }
+  if (nf > 0) {
+    hipLaunchKernelGGL(k_stage_faces, dim3(blocks_for(nf, 256)), dim3(256), 0, c->stream, nf,
+                       c->losort.p, c->upper64.p, lower ? c->lower64.p : nullptr,
+                       c->lowerL64.p, c->upper32.p, c->lowerL32.p,
+                       lower ? c->lower32.p : nullptr);
+    CKL();
+  }
+  c->have64 = true;
+  return HIPFOAM_OK;
+}
+
+// FP32 inner operator diagonal: diag* = deflate ? diag - u : diag (+ rD, and
+// the Gershgorin bound when wantBeta).  Writes os->bail / os->beta*.
+int prepare_inner(hipfoam_context* c, int deflate, int wantBeta) {
+  const int n = c->n;
+  const int bs = c->sched.blockSize;
+  LduOp<double> A = op64(c);
+  hipLaunchKernelGGL((k_prepare_inner<float>), dim3(blocks_for(n, bs)), dim3(bs), 0, c->stream,
+                     A, c->upper64.p, c->losort.p, c->u64.p, deflate, wantBeta, c->diag32.p,
+                     c->rD32.p, c->ost.p, c->partials.p, c->pstride);
+  CKL();
+  c->have32 = true;
+  return HIPFOAM_OK;
+}
+
+int read_ostate(hipfoam_context* c) {
+  CK(hipMemcpyAsync(c->ho, c->ost.p, sizeof(OState), hipMemcpyDeviceToHost, c->stream));
+  CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+// initial FP64 residual + normFactor (metalFoamSolver.C:101-129): psi in psiCur.
+int initial_residual(hipfoam_context* c, const double* psiCur, const double* src) {
+  const int n = c->n, bs = c->sched.blockSize;
+  LduOp<double> A = op64(c);
+  const dim3 g(blocks_for(n, bs)), bl(bs);
+  hipLaunchKernelGGL(k_rowsum, g, bl, 0, c->stream, A, psiCur, c->u64.p, c->ost.p, c->partials.p,
+                     c->pstride);
+  hipLaunchKernelGGL(k_initial_residual, g, bl, 0, c->stream, A, psiCur, src, c->u64.p,
+                     c->r64.p, c->ost.p, c->partials.p, c->pstride);
+  CKL();
+  return read_ostate(c);
+}
+
+bool of_converged_h(double res, double res0, double tol, double relTol) {
+  return res < tol || (relTol > kSmall && res < relTol * res0);
+}
+
+int check_ready(hipfoam_context* c) {
+  if (!c) return HIPFOAM_ERR_ARG;
+  if (!c->good) return fail(c, HIPFOAM_ERR_DEVICE, "no usable device: " + c->desc);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, HIPFOAM_ERR_DEVICE, "hipSetDevice");
+  if (!c->ownStart.p) return fail(c, HIPFOAM_ERR_STATE, "set_topology not called");
+  return HIPFOAM_OK;
+}
+
+// -------------------------------------------------------------------------
+// metalFoamSolver::solveImpl with the FP64 steps on the device
+// -------------------------------------------------------------------------
+
+int refined_solve(hipfoam_context* c, int solver, const double* diag, const double* upper,
+                  const double* lower, double* psi, const double* source,
+                  const hipfoam_controls* ctl, hipfoam_performance* perf, double* residual,
+                  hipMemcpyKind kind) {
+  const double t0 = now_seconds();
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!psi || !source || !ctl) return fail(c, HIPFOAM_ERR_ARG, "NULL psi/source/controls");
+  if (solver != HIPFOAM_PCG && solver != HIPFOAM_PBICGSTAB)
+    return fail(c, HIPFOAM_ERR_ARG, "unknown solver");
+  const bool useBiCG = solver == HIPFOAM_PBICGSTAB;
+  const int n = c->n;
+  hipfoam_performance pf{};
+  rc = ensure_vectors(c);
+  if (rc) return rc;
+  // lower is used only by PBiCGStab on asymmetric matrices (metalFoamSolver.C:214-219)
+  rc = stage_matrix(c, diag, upper, lower, kind);
+  if (rc) return rc;
+  const double tStage = now_seconds() - t0;
+  CK(hipMemcpyAsync(c->psi0.p, psi, sizeof(double) * n, kind, c->stream));
+  CK(hipMemcpyAsync(c->b.p, source, sizeof(double) * n, kind, c->stream));
+  double* psiCur = c->psi0.p;
+  double* psiNxt = c->psi1.p;
+  rc = initial_residual(c, psiCur, c->b.p);
+  if (rc) return rc;
+  const OState& o = *c->ho;
+  const double normFactor = o.normFactor;
+  pf.normFactor = normFactor;
+  pf.initialResidual = o.initRes;
+  pf.finalResidual = o.initRes;
+  if (ctl->minIter > 0 ||
+      !of_converged_h(pf.finalResidual, pf.initialResidual, ctl->tolerance, ctl->relTol)) {
+    // deflation (metalFoamSolver.C:170-190)
+    bool deflate = false;
+    if (!useBiCG && ctl->deflate != 0) {
+      if (ctl->deflate == 1)
+        deflate = true;
+      else
+        deflate = (10.0 * o.nPinned <= (double)n);
+    }
+    const int chebOrder = useBiCG ? 1 : ctl->chebyshevOrder;
+    if (chebOrder > kMaxCheb + 1 || chebOrder < 1)
+      return fail(c, HIPFOAM_ERR_ARG, "chebyshevOrder must be in [1, 33]");
+    const bool wantBeta = chebOrder > 1;
+    const double tc0 = now_seconds();
+    // metalFoamSolver.C:192-249: diag0 = diag - u, sign-flip bail, setCoeffs, Gershgorin
+    rc = prepare_inner(c, deflate ? 1 : 0, wantBeta);
+    if (rc) return rc;
+    rc = read_ostate(c);
+    if (rc) return rc;
+    if (deflate && o.bail > 0.0) {
+      deflate = false;
+      rc = prepare_inner(c, 0, wantBeta);
+      if (rc) return rc;
+      rc = read_ostate(c);
+      if (rc) return rc;
+    }
+    pf.tConvert = tStage + (now_seconds() - tc0);
+    pf.deflated = deflate;
+    const double chebRatio = ctl->chebyshevRatio;
+    const double chebBeta = wantBeta ? (deflate ? o.betaDefl : o.betaExact) : 2.0;
+    pf.chebBeta = chebBeta;
+    // metalFoamSolver.C:251-257
+    const double target = std::max(ctl->tolerance, ctl->relTol * pf.initialResidual) * normFactor;
+    const double uu = o.uu;
+    int totalIter = 0, outer = 0;
+    bool singular = false;
+    double prevRes = pf.initialResidual;
+    const int bs = c->sched.blockSize;
+    const dim3 g(blocks_for(n, bs)), bl(bs);
+    for (outer = 0; outer <= ctl->maxRestarts; ++outer) {
+      hipfoam_stats st{};
+      // zero-mean projection of the inner RHS (metalFoamSolver.C:278-287)
+      rc = inner_solve(c, useBiCG ? KIND_BICG : KIND_PCG, c->r64.p,
+                       deflate ? &c->ost.p->rmean : nullptr, target, ctl->maxIter - totalIter,
+                       ctl->iterationsPerSync, chebOrder, chebBeta, chebRatio, &st);
+      if (rc) return rc;
+      totalIter += st.nIterations;
+      pf.tGpu += st.tGpuSeconds;
+      pf.tInner += st.tWallSeconds;
+      singular = st.singular;
+      // psi += delta - mean(delta); r = b - A psi (metalFoamSolver.C:318-340)
+      LduOp<double> A = op64(c);
+      if (deflate) {
+        hipLaunchKernelGGL((k_delta_mean<float>), g, bl, 0, c->stream, n, A.xcdRemap,
+                           c->w32.delta.p, c->ost.p, c->partials.p, c->pstride);
+      }
+      hipLaunchKernelGGL((k_refine_residual<float>), g, bl, 0, c->stream, A, psiCur,
+                         c->w32.delta.p, psiNxt, c->b.p, c->u64.p, c->r64.p, deflate ? 1 : 0,
+                         c->ost.p, c->partials.p, c->pstride);
+      std::swap(psiCur, psiNxt);
+      // constant-mode correction (metalFoamSolver.C:342-361)
+      if (deflate && uu > kVSmall) {
+        hipLaunchKernelGGL(k_const_shift, g, bl, 0, c->stream, n, A.xcdRemap, psiCur, c->r64.p,
+                           c->u64.p, c->ost.p, c->partials.p, c->pstride);
+      }
+      CKL();
+      rc = read_ostate(c);
+      if (rc) return rc;
+      pf.finalResidual = o.finalRes;
+      pf.nIterations = totalIter;
+      // hybrid polish (metalFoamSolver.C:371-391): exact operator, beta kept
+      if (deflate && pf.finalResidual > prevRes / 1.5 && !st.singular) {
+        deflate = false;
+        pf.polished = 1;
+        rc = prepare_inner(c, 0, 0);
+        if (rc) return rc;
+        if (ctl->verbose)
+          printf("    %s switching to exact operator (polish)\n",
+                 useBiCG ? "hipPBiCGStab" : "hipPCG");
+      }
+      prevRes = pf.finalResidual;
+      if (ctl->verbose)
+        printf("    %s restart %d: inner=%d fp32ResSum=%g fp64Res=%g%s%s\n",
+               useBiCG ? "hipPBiCGStab" : "hipPCG", outer, st.nIterations, st.finalResSum,
+               pf.finalResidual, st.stagnated ? " (stagnated)" : "",
+               st.singular ? " (breakdown)" : "");
+      // metalFoamSolver.C:404-414
+      if (singular) break;
+      if (totalIter >= ctl->maxIter) break;
+      if (pf.nIterations >= ctl->minIter &&
+          of_converged_h(pf.finalResidual, pf.initialResidual, ctl->tolerance, ctl->relTol))
+        break;
+    }
+    pf.restarts = outer;
+    pf.singular = singular;
+  }
+  pf.converged =
+      of_converged_h(pf.finalResidual, pf.initialResidual, ctl->tolerance, ctl->relTol);
+  const hipMemcpyKind back =
+      kind == hipMemcpyHostToDevice ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  CK(hipMemcpyAsync(psi, psiCur, sizeof(double) * n, back, c->stream));
+  if (residual) CK(hipMemcpyAsync(residual, c->r64.p, sizeof(double) * n, back, c->stream));
+  CK(hipStreamSynchronize(c->stream));
+  pf.tTotal = now_seconds() - t0;
+  if (perf) *perf = pf;
+  return HIPFOAM_OK;
+}
+
+// Plain FP64: OpenFOAM PCG / PBiCGStab loops on the device.
+int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double* upper,
+               const double* lower, double* psi, const double* source,
+               const hipfoam_controls* ctl, hipfoam_performance* perf, double* residual,
+               hipMemcpyKind kind) {
+  const double t0 = now_seconds();
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!psi || !source || !ctl) return fail(c, HIPFOAM_ERR_ARG, "NULL psi/source/controls");
+  if (solver != HIPFOAM_PCG && solver != HIPFOAM_PBICGSTAB)
+    return fail(c, HIPFOAM_ERR_ARG, "unknown solver");
+  const bool useBiCG = solver == HIPFOAM_PBICGSTAB;
+  const int n = c->n;
+  hipfoam_performance pf{};
+  rc = ensure_vectors(c);
+  if (rc) return rc;
+  CK(c->w64.ensure(n));
+  rc = stage_matrix(c, diag, upper, useBiCG ? lower : nullptr, kind);
+  if (rc) return rc;
+  CK(hipMemcpyAsync(c->psi0.p, psi, sizeof(double) * n, kind, c->stream));
+  CK(hipMemcpyAsync(c->b.p, source, sizeof(double) * n, kind, c->stream));
+  rc = initial_residual(c, c->psi0.p, c->b.p);
+  if (rc) return rc;
+  const OState& o = *c->ho;
+  pf.normFactor = o.normFactor;
+  pf.initialResidual = o.initRes;
+  pf.finalResidual = o.initRes;
+  Work<double>& w = c->w64;
+  const int order = useBiCG ? 1 : ctl->chebyshevOrder;
+  if (order > kMaxCheb + 1 || order < 1)
+    return fail(c, HIPFOAM_ERR_ARG, "chebyshevOrder must be in [1, 33]");
+  double* result = c->psi0.p;
+  if (ctl->minIter > 0 ||
+      !of_converged_h(pf.finalResidual, pf.initialResidual, ctl->tolerance, ctl->relTol)) {
+    const int bs = c->sched.blockSize;
+    const dim3 g1(blocks_for(n, 256)), b1(256);
+    // Gershgorin bound of the exact operator for Chebyshev
+    double beta = 2.0;
+    if (order > 1) {
+      rc = prepare_inner(c, 0, 1);
+      if (rc) return rc;
+      rc = read_ostate(c);
+      if (rc) return rc;
+      beta = o.betaExact;
+    }
+    pf.chebBeta = beta;
+    hipLaunchKernelGGL(k_recip, g1, b1, 0, c->stream, n, c->diag64.p, c->rD64.p);
+    hipLaunchKernelGGL((k_inner_init<double>), g1, b1, 0, c->stream, n, c->r64.p,
+                       (const double*)nullptr, w.r.p, useBiCG ? w.rA0.p : nullptr,
+                       (double*)nullptr, w.p0.p, w.p1.p, useBiCG ? w.AyA.p : nullptr);
+    hipLaunchKernelGGL((k_copy<double>), g1, b1, 0, c->stream, n, c->psi0.p, w.delta.p);
+    CKL();
+    KState s{};
+    s.mode = MODE_OPENFOAM;
+    s.normFactor = pf.normFactor;
+    s.initRes = pf.initialResidual;
+    s.tol = ctl->tolerance;
+    s.relTol = ctl->relTol;
+    s.maxIter = ctl->maxIter;
+    s.minIter = ctl->minIter;
+    s.status = ST_RUNNING;
+    if (order > 1) cheb_constants(s, order, beta, ctl->chebyshevRatio, false);
+    *c->hk = s;
+    CK(hipMemcpyAsync(c->kst.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+    const LduOp<double> A = op64(c);
+    double gpu = 0;
+    const double tw = now_seconds();
+    rc = run_loop<double>(c, useBiCG ? KIND_BICG : KIND_PCG, A, c->rD64.p, order,
+                          std::max(ctl->maxIter, ctl->minIter), &gpu);
+    if (rc) return rc;
+    pf.tInner = now_seconds() - tw;
+    pf.tGpu = gpu;
+    pf.nIterations = c->hk->it;
+    pf.finalResidual = c->hk->res;
+    pf.singular = c->hk->status == ST_SINGULAR;
+    if (pf.singular && c->hk->it == 0) pf.finalResidual = pf.initialResidual;
+    result = w.delta.p;
+    (void)bs;
+  }
+  pf.converged =
+      of_converged_h(pf.finalResidual, pf.initialResidual, ctl->tolerance, ctl->relTol);
+  const hipMemcpyKind back =
+      kind == hipMemcpyHostToDevice ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  CK(hipMemcpyAsync(psi, result, sizeof(double) * n, back, c->stream));
+  if (residual) {
+    const double* rr = (result == c->psi0.p) ? c->r64.p : w.r.p;
+    CK(hipMemcpyAsync(residual, rr, sizeof(double) * n, back, c->stream));
+  }
+  CK(hipStreamSynchronize(c->stream));
+  pf.tTotal = now_seconds() - t0;
+  if (perf) *perf = pf;
+  return HIPFOAM_OK;
+}
+
+int amul_impl(hipfoam_context* c, int precision, const void* x, void* y, bool device) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!x || !y) return fail(c, HIPFOAM_ERR_ARG, "NULL x/y");
+  const int n = c->n, bs = c->sched.blockSize;
+  const dim3 g(blocks_for(n, bs)), bl(bs);
+  if (precision == HIPFOAM_FP32) {
+    if (!c->have32) return fail(c, HIPFOAM_ERR_STATE, "no FP32 operator staged");
+    const float* xd = static_cast<const float*>(x);
+    float* yd = static_cast<float*>(y);
+    if (!device) {
+      CK(c->w32.ensure(n));
+      CK(hipMemcpyAsync(c->w32.z0.p, x, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+      xd = c->w32.z0.p;
+      yd = c->w32.z1.p;
+    }
+    {
+      TimedLaunch tl(c, 5, false);
+      hipLaunchKernelGGL((k_spmv<float>), g, bl, 0, c->stream, op32(c), xd, yd);
+    }
+    CKL();
+    if (!device)
+      CK(hipMemcpyAsync(y, yd, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  } else if (precision == HIPFOAM_FP64) {
+    if (!c->have64) return fail(c, HIPFOAM_ERR_STATE, "no FP64 matrix staged");
+    const double* xd = static_cast<const double*>(x);
+    double* yd = static_cast<double*>(y);
+    if (!device) {
+      CK(c->psi1.ensure(n));
+      CK(c->scratch64.ensure(n));
+      CK(hipMemcpyAsync(c->psi1.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+      xd = c->psi1.p;
+      yd = c->scratch64.p;
+    }
+    {
+      TimedLaunch tl(c, 5, false);
+      hipLaunchKernelGGL((k_spmv<double>), g, bl, 0, c->stream, op64(c), xd, yd);
+    }
+    CKL();
+    if (!device)
+      CK(hipMemcpyAsync(y, yd, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    return fail(c, HIPFOAM_ERR_ARG, "precision must be HIPFOAM_FP32 or HIPFOAM_FP64");
+  }
+  if (!device || !c->timer.on) CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+int set_matrix_impl(hipfoam_context* c, const double* diag, const double* upper,
+                    const double* lower, hipMemcpyKind kind) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  rc = ensure_vectors(c);
+  if (rc) return rc;
+  rc = stage_matrix(c, diag, upper, lower, kind);
+  if (rc) return rc;
+  rc = prepare_inner(c, 0, 0);
+  if (rc) return rc;
+  CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// extern "C" exports
+// ===========================================================================
+
+extern "C" {
+
+void hipfoam_default_controls(hipfoam_controls* c) {
+  if (!c) return;
+  c->tolerance = 1e-6;
+  c->relTol = 0.0;
+  c->maxIter = 1000;
+  c->minIter = 0;
+  c->deflate = 2;
+  c->chebyshevOrder = 8;
+  c->chebyshevRatio = 30.0;
+  c->maxRestarts = 8;
+  c->iterationsPerSync = 1;
+  c->verbose = 0;
+}
+
+int32_t hipfoam_create(int32_t device, hipfoam_context** out) {
+  FpTrapGuard fpe;
+  if (!out) return HIPFOAM_ERR_ARG;
+  hipfoam_context* c = new (std::nothrow) hipfoam_context();
+  *out = c;
+  if (!c) return HIPFOAM_ERR_DEVICE;
+  c->device = device;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) {
+    c->desc = "hipfoam: no HIP device " + std::to_string(device);
+    c->err = c->desc;
+    return HIPFOAM_ERR_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    c->desc = "hipfoam: cannot initialise HIP device " + std::to_string(device);
+    c->err = c->desc;
+    return HIPFOAM_ERR_DEVICE;
+  }
+  c->numCUs = prop.multiProcessorCount;
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->hk), sizeof(KState)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->ho), sizeof(OState)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->hstatus), 4 * sizeof(int32_t)) != hipSuccess ||
+      c->kst.ensure(1) != hipSuccess || c->ost.ensure(1) != hipSuccess ||
+      hipEventCreate(&c->evPoll[0]) != hipSuccess || hipEventCreate(&c->evPoll[1]) != hipSuccess ||
+      hipEventCreate(&c->evT0) != hipSuccess || hipEventCreate(&c->evT1) != hipSuccess) {
+    c->desc = "hipfoam: device allocation failed";
+    c->err = c->desc;
+    return HIPFOAM_ERR_DEVICE;
+  }
+  (void)hipMemset(c->kst.p, 0, sizeof(KState));
+  (void)hipMemset(c->ost.p, 0, sizeof(OState));
+  (void)hipDeviceSynchronize();
+  char buf[512];
+  snprintf(buf, sizeof(buf), "hipfoam: %s (%s, %d CUs, %.0f GiB HBM)", prop.name,
+           prop.gcnArchName, prop.multiProcessorCount,
+           (double)prop.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
+  c->desc = buf;
+  c->good = true;
+  return HIPFOAM_OK;
+}
+
+void hipfoam_destroy(hipfoam_context* c) {
+  FpTrapGuard fpe;
+  if (!c) return;
+  if (c->good) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+  }
+  clear_graphs(c);
+  for (auto& e : c->timer.ev) (void)hipEventDestroy(e);
+  Buf<int32_t>* ib[] = {&c->ownStart, &c->losortStart, &c->uppAddr,
+                        &c->lowAddr, &c->losort, &c->losortLow};
+  for (auto* b : ib) b->release();
+  Buf<double>* db[] = {&c->diag64, &c->upper64, &c->lower64, &c->lowerL64, &c->rD64,
+                       &c->psi0, &c->psi1, &c->b, &c->r64, &c->u64, &c->scratch64,
+                       &c->partials};
+  for (auto* b : db) b->release();
+  Buf<float>* fb[] = {&c->diag32, &c->upper32, &c->lowerL32, &c->lower32, &c->rD32};
+  for (auto* b : fb) b->release();
+  c->w32.release();
+  c->w64.release();
+  c->kst.release();
+  c->ost.release();
+  if (c->hk) (void)hipHostFree(c->hk);
+  if (c->ho) (void)hipHostFree(c->ho);
+  if (c->hstatus) (void)hipHostFree(c->hstatus);
+  for (hipEvent_t e : {c->evPoll[0], c->evPoll[1], c->evT0, c->evT1})
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int32_t hipfoam_good(const hipfoam_context* c) { return c && c->good ? 1 : 0; }
+
+const char* hipfoam_description(const hipfoam_context* c) {
+  return c ? c->desc.c_str() : "hipfoam: NULL context";
+}
+
+const char* hipfoam_last_error(const hipfoam_context* c) {
+  return c ? c->err.c_str() : "NULL context";
+}
+
+int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
+                             const int32_t* lowAddr, const int32_t* uppAddr,
+                             const int32_t* ownStart, const int32_t* losort,
+                             const int32_t* losortStart) {
+  FpTrapGuard fpe;
+  if (!c) return HIPFOAM_ERR_ARG;
+  if (!c->good) return fail(c, HIPFOAM_ERR_DEVICE, "no usable device: " + c->desc);
+  if (nCells < 0 || nFaces < 0) return fail(c, HIPFOAM_ERR_ARG, "negative size");
+  if (nFaces > 0 && (!lowAddr || !uppAddr || !losort))
+    return fail(c, HIPFOAM_ERR_ARG, "NULL addressing");
+  if (!ownStart || !losortStart) return fail(c, HIPFOAM_ERR_ARG, "NULL start arrays");
+  CK(hipSetDevice(c->device));
+  // no-op when unchanged (metalFoamBackend.C:469-477)
+  if (c->ownStart.p && c->n == nCells && c->nf == nFaces && c->lastLow == lowAddr &&
+      c->lastUpp == uppAddr)
+    return HIPFOAM_OK;
+  CK(hipStreamSynchronize(c->stream));
+  clear_graphs(c);
+  c->gen += 1;
+  c->n = nCells;
+  c->nf = nFaces;
+  c->lastLow = lowAddr;
+  c->lastUpp = uppAddr;
+  c->have32 = c->have64 = false;
+  const size_t nf = (size_t)nFaces, n1 = (size_t)nCells + 1;
+  CK(c->ownStart.ensure(n1));
+  CK(c->losortStart.ensure(n1));
+  CK(c->uppAddr.ensure(nf));
+  CK(c->lowAddr.ensure(nf));
+  CK(c->losort.ensure(nf));
+  CK(c->losortLow.ensure(nf));
+  CK(hipMemcpyAsync(c->ownStart.p, ownStart, sizeof(int32_t) * n1, hipMemcpyHostToDevice,
+                    c->stream));
+  CK(hipMemcpyAsync(c->losortStart.p, losortStart, sizeof(int32_t) * n1, hipMemcpyHostToDevice,
+                    c->stream));
+  if (nFaces > 0) {
+    CK(hipMemcpyAsync(c->uppAddr.p, uppAddr, sizeof(int32_t) * nf, hipMemcpyHostToDevice,
+                      c->stream));
+    CK(hipMemcpyAsync(c->lowAddr.p, lowAddr, sizeof(int32_t) * nf, hipMemcpyHostToDevice,
+                      c->stream));
+    CK(hipMemcpyAsync(c->losort.p, losort, sizeof(int32_t) * nf, hipMemcpyHostToDevice,
+                      c->stream));
+    hipLaunchKernelGGL(k_gather_addr, dim3(blocks_for(nFaces, 256)), dim3(256), 0, c->stream,
+                       nFaces, c->losort.p, c->lowAddr.p, c->losortLow.p);
+    CKL();
+  }
+  c->pstride = std::max(1, blocks_for(nCells, 64));
+  CK(c->partials.ensure((size_t)c->pstride * 3));
+  c->w32.release();
+  c->w64.release();
+  int rc = ensure_vectors(c);
+  if (rc) return rc;
+  CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_set_coeffs(hipfoam_context* c, const double* diag, const double* upper,
+                           const double* lower, double* seconds) {
+  FpTrapGuard fpe;
+  const double t0 = now_seconds();
+  int rc = set_matrix_impl(c, diag, upper, lower, hipMemcpyHostToDevice);
+  if (seconds) *seconds = now_seconds() - t0;
+  return rc;
+}
+
+int32_t hipfoam_set_matrix(hipfoam_context* c, const double* diag, const double* upper,
+                           const double* lower) {
+  FpTrapGuard fpe;
+  return set_matrix_impl(c, diag, upper, lower, hipMemcpyHostToDevice);
+}
+
+int32_t hipfoam_set_matrix_device(hipfoam_context* c, const double* diag, const double* upper,
+                                  const double* lower) {
+  FpTrapGuard fpe;
+  return set_matrix_impl(c, diag, upper, lower, hipMemcpyDeviceToDevice);
+}
+
+static int32_t inner_entry(hipfoam_context* c, Kind kind, const double* r, double target,
+                           int32_t maxIter, int32_t ips, int32_t order, double beta,
+                           double ratio, hipfoam_stats* stats) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!r) return fail(c, HIPFOAM_ERR_ARG, "NULL r");
+  if (!c->have32) return fail(c, HIPFOAM_ERR_STATE, "set_coeffs not called");
+  if (order < 1 || order > kMaxCheb + 1)
+    return fail(c, HIPFOAM_ERR_ARG, "chebOrder must be in [1, 33]");
+  CK(hipMemcpyAsync(c->r64.p, r, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+  rc = inner_solve(c, kind, c->r64.p, nullptr, target, maxIter, ips, order, beta, ratio, stats);
+  if (rc) return rc;
+  CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_pcg(hipfoam_context* c, const double* r, double targetResSum, int32_t maxIter,
+                    int32_t iterationsPerSync, int32_t chebOrder, double chebBeta,
+                    double chebRatio, hipfoam_stats* stats) {
+  FpTrapGuard fpe;
+  return inner_entry(c, KIND_PCG, r, targetResSum, maxIter, iterationsPerSync, chebOrder,
+                     chebBeta, chebRatio, stats);
+}
+
+int32_t hipfoam_bicgstab(hipfoam_context* c, const double* r, double targetResSum,
+                         int32_t maxIter, hipfoam_stats* stats) {
+  FpTrapGuard fpe;
+  return inner_entry(c, KIND_BICG, r, targetResSum, maxIter, 1, 1, 2.0, 30.0, stats);
+}
+
+int32_t hipfoam_delta(hipfoam_context* c, float* out) {
+  FpTrapGuard fpe;
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!out) return fail(c, HIPFOAM_ERR_ARG, "NULL out");
+  if (!c->w32.delta.p) return fail(c, HIPFOAM_ERR_STATE, "no inner solve yet");
+  CK(hipMemcpyAsync(out, c->w32.delta.p, sizeof(float) * c->n, hipMemcpyDeviceToHost,
+                    c->stream));
+  CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_solve(hipfoam_context* c, int32_t solver, const double* diag,
+                      const double* upper, const double* lower, double* psi,
+                      const double* source, const hipfoam_controls* controls,
+                      hipfoam_performance* perf, double* residual) {
+  FpTrapGuard fpe;
+  return refined_solve(c, solver, diag, upper, lower, psi, source, controls, perf, residual,
+                       hipMemcpyHostToDevice);
+}
+
+int32_t hipfoam_solve_device(hipfoam_context* c, int32_t solver, const double* diag,
+                             const double* upper, const double* lower, double* psi,
+                             const double* source, const hipfoam_controls* controls,
+                             hipfoam_performance* perf, double* residual) {
+  FpTrapGuard fpe;
+  return refined_solve(c, solver, diag, upper, lower, psi, source, controls, perf, residual,
+                       hipMemcpyDeviceToDevice);
+}
+
+int32_t hipfoam_solve_fp64(hipfoam_context* c, int32_t solver, const double* diag,
+                           const double* upper, const double* lower, double* psi,
+                           const double* source, const hipfoam_controls* controls,
+                           hipfoam_performance* perf, double* residual) {
+  FpTrapGuard fpe;
+  return fp64_solve(c, solver, diag, upper, lower, psi, source, controls, perf, residual,
+                    hipMemcpyHostToDevice);
+}
+
+int32_t hipfoam_solve_fp64_device(hipfoam_context* c, int32_t solver, const double* diag,
+                                  const double* upper, const double* lower, double* psi,
+                                  const double* source, const hipfoam_controls* controls,
+                                  hipfoam_performance* perf, double* residual) {
+  FpTrapGuard fpe;
+  return fp64_solve(c, solver, diag, upper, lower, psi, source, controls, perf, residual,
+                    hipMemcpyDeviceToDevice);
+}
+
+int32_t hipfoam_amul(hipfoam_context* c, int32_t precision, const void* x, void* y) {
+  FpTrapGuard fpe;
+  return amul_impl(c, precision, x, y, false);
+}
+
+int32_t hipfoam_amul_device(hipfoam_context* c, int32_t precision, const void* x, void* y) {
+  FpTrapGuard fpe;
+  return amul_impl(c, precision, x, y, true);
+}
+
+int32_t hipfoam_device_alloc(hipfoam_context* c, uint64_t bytes, void** ptr) {
+  FpTrapGuard fpe;
+  if (!c || !ptr) return HIPFOAM_ERR_ARG;
+  if (!c->good) return fail(c, HIPFOAM_ERR_DEVICE, "no usable device");
+  CK(hipSetDevice(c->device));
+  CK(hipMalloc(ptr, bytes ? bytes : 1));
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_device_free(hipfoam_context* c, void* ptr) {
+  FpTrapGuard fpe;
+  if (!c) return HIPFOAM_ERR_ARG;
+  if (!ptr) return HIPFOAM_OK;
+  CK(hipSetDevice(c->device));
+  CK(hipStreamSynchronize(c->stream));
+  CK(hipFree(ptr));
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_copy_to_device(hipfoam_context* c, void* dst, const void* src, uint64_t bytes) {
+  FpTrapGuard fpe;
+  if (!c || (!dst && bytes) || (!src && bytes)) return HIPFOAM_ERR_ARG;
+  CK(hipSetDevice(c->device));
+  CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_copy_to_host(hipfoam_context* c, void* dst, const void* src, uint64_t bytes) {
+  FpTrapGuard fpe;
+  if (!c || (!dst && bytes) || (!src && bytes)) return HIPFOAM_ERR_ARG;
+  CK(hipSetDevice(c->device));
+  CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_copy_on_device(hipfoam_context* c, void* dst, const void* src,
+                               uint64_t bytes) {
+  FpTrapGuard fpe;
+  if (!c || (!dst && bytes) || (!src && bytes)) return HIPFOAM_ERR_ARG;
+  CK(hipSetDevice(c->device));
+  CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_synchronize(hipfoam_context* c) {
+  FpTrapGuard fpe;
+  if (!c) return HIPFOAM_ERR_ARG;
+  if (!c->good) return HIPFOAM_ERR_DEVICE;
+  CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_get_schedule(const hipfoam_context* c, hipfoam_schedule* s) {
+  if (!c || !s) return HIPFOAM_ERR_ARG;
+  *s = c->sched;
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_set_schedule(hipfoam_context* c, const hipfoam_schedule* s) {
+  FpTrapGuard fpe;
+  if (!c || !s) return HIPFOAM_ERR_ARG;
+  if (s->spmvSchedule < 0 || s->spmvSchedule > 1)
+    return fail(c, HIPFOAM_ERR_ARG, "spmvSchedule must be 0 or 1");
+  if (s->blockSize < 64 || s->blockSize > 1024 || s->blockSize % 64)
+    return fail(c, HIPFOAM_ERR_ARG, "blockSize must be a multiple of 64 in [64, 1024]");
+  if (s->graphChunk < 0 || s->graphChunk > 4096 || (s->graphChunk & 1))
+    return fail(c, HIPFOAM_ERR_ARG, "graphChunk must be even in [0, 4096]");
+  if (c->good) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+  }
+  c->sched = *s;
+  c->sched.xcdRemap = s->xcdRemap ? 1 : 0;
+  clear_graphs(c);
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_autotune(hipfoam_context* c, hipfoam_schedule* best) {
+  FpTrapGuard fpe;
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!c->have32) return fail(c, HIPFOAM_ERR_STATE, "stage a matrix first");
+  const int n = c->n;
+  CK(c->w32.ensure(n));
+  CK(hipMemsetAsync(c->w32.z0.p, 0, sizeof(float) * n, c->stream));
+  const hipfoam_schedule keep = c->sched;
+  hipfoam_schedule bestS = keep;
+  double bestT = 1e300;
+  hipEvent_t e0, e1;
+  CK(hipEventCreate(&e0));
+  CK(hipEventCreate(&e1));
+  const int reps = 20;
+  for (int sch = 0; sch <= 1; ++sch)
+    for (int bs : {128, 256, 512, 1024})
+      for (int rm = 0; rm <= 1; ++rm) {
+        c->sched.spmvSchedule = sch;
+        c->sched.blockSize = bs;
+        c->sched.xcdRemap = rm;
+        const LduOp<float> A = op32(c);
+        const dim3 g(blocks_for(n, bs)), bl(bs);
+        hipLaunchKernelGGL((k_spmv<float>), g, bl, 0, c->stream, A, c->w32.z0.p, c->w32.z1.p);
+        CK(hipEventRecord(e0, c->stream));
+        for (int k = 0; k < reps; ++k)
+          hipLaunchKernelGGL((k_spmv<float>), g, bl, 0, c->stream, A, c->w32.z0.p,
+                             c->w32.z1.p);
+        CK(hipEventRecord(e1, c->stream));
+        CK(hipEventSynchronize(e1));
+        float ms = 0;
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < bestT) {
+          bestT = ms;
+          bestS = c->sched;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  bestS.graphChunk = keep.graphChunk;
+  c->sched = bestS;
+  clear_graphs(c);
+  if (best) *best = bestS;
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_set_kernel_timing(hipfoam_context* c, int32_t enable, int32_t kclass) {
+  FpTrapGuard fpe;
+  if (!c) return HIPFOAM_ERR_ARG;
+  if (!c->good) return fail(c, HIPFOAM_ERR_DEVICE, "no usable device");
+  CK(hipSetDevice(c->device));
+  KernelTimer& t = c->timer;
+  if (t.ev.empty()) {
+    t.ev.resize(4096);
+    for (auto& e : t.ev) CK(hipEventCreate(&e));
+  }
+  t.on = enable != 0;
+  t.kclass = kclass;
+  t.used = 0;
+  t.launches = 0;
+  t.seconds = 0;
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_kernel_timing(hipfoam_context* c, int64_t* launches, double* seconds) {
+  FpTrapGuard fpe;
+  if (!c) return HIPFOAM_ERR_ARG;
+  int rc = timer_flush(c);
+  if (rc) return rc;
+  if (launches) *launches = c->timer.launches;
+  if (seconds) *seconds = c->timer.seconds;
+  return HIPFOAM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,951 @@
+// hipfoam_device.hpp -- gfx950 device code of the LDU Krylov path.
+//
+// Matrix layout in HBM (rows = cells, faces = internal faces):
+//   ownStart[n+1], uppAddr[F]            owner faces of row i:  upper[f] * x[uppAddr[f]]
+//   losortStart[n+1], losortLow[F]       neighbour faces of row i, in losort order:
+//                                        lowerL[k] * x[losortLow[k]], with
+//                                        losortLow[k] = lowAddr[losort[k]] (built once per mesh)
+//                                        lowerL[k]    = lower[losort[k]]   (staged per solve)
+//   diag[n]
+// Row i of A*x is summed in the reference kernel's order: diag first, owner
+// faces ascending, then losort faces ascending (metalFoamKernels.metal:87-104).
+// The "direct" schedule reads lower[losort[k]] and lowAddr[losort[k]]
+// through losort instead of the staged copies.
+//
+// One thread per row, 64-wide wavefronts, 256-thread workgroups by default.
+// Reductions: per-thread values widened to double, wavefront butterfly
+// (ds_swizzle/DPP via __shfl_xor), LDS across the block, one double per block
+// into `partials`; the last block to arrive (device-scope counter) folds the
+// partials in a fixed order and finalises the Krylov scalars on the device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hipfoam {
+
+constexpr int kMaxCheb = 32;
+constexpr double kVSmall = 1.0e-300;  // OpenFOAM VSMALL (double)
+constexpr double kSmall = 1.0e-20;    // solverPerformance small_
+
+enum : int32_t {
+  ST_RUNNING = 0,
+  ST_CONVERGED = 1,
+  ST_SINGULAR = 2,
+  ST_STAGNATED = 3,
+  ST_MAXITER = 4,
+  ST_EARLY = 5  // BiCGStab |sA| exit pending its delta update
+};
+
+enum : int32_t {
+  MODE_INNER = 0,    // reference backend semantics (FP32 correction solve)
+  MODE_OPENFOAM = 1  // OpenFOAM PCG/PBiCGStab semantics (plain solve)
+};
+
+// Device-resident Krylov state: the reference's scalars buffer
+// (metalFoamKernels.metal:20-32) widened with the host loop's bookkeeping
+// (metalFoamBackend.C:599-715) so the whole loop runs without host syncs.
+struct alignas(128) KState {
+  double rho, rhoOld, wApA, alpha, beta, res;     // PCG (rho = wA.rA)
+  double rA0rA, rA0rAold, rA0AyA, omega, tAtA, tAsA;  // BiCGStab
+  double bestRes, target;                          // inner mode
+  double normFactor, initRes, tol, relTol;         // OpenFOAM mode
+  double chebC1, chebC2[kMaxCheb], chebC3[kMaxCheb];
+  int32_t it, maxIter, minIter, ips, batchSize, batchLeft, noProgress, status;
+  int32_t copyBest, mode, pad0, pad1;
+  uint32_t counter[8];
+};
+
+// Outer (FP64 refinement) scalars.
+struct alignas(128) OState {
+  double sumPsi, xbar, nPinned, uu;        // rowsum pass
+  double normFactor, sumMagR, sumR, initRes;
+  double rmean;                             // mean of the current residual
+  double dmean;                             // mean of the inner correction
+  double ur, c, finalRes;
+  double bail, betaDefl, betaExact;
+  int32_t n, deflate, pad0, pad1;
+  uint32_t counter[8];
+};
+
+template <typename T>
+struct LduOp {
+  const int32_t* __restrict__ ownStart;
+  const int32_t* __restrict__ losortStart;
+  const int32_t* __restrict__ uppAddr;
+  const int32_t* __restrict__ nbrAddr;   // losortLow (staged) or lowAddr (direct)
+  const int32_t* __restrict__ losort;    // direct schedule only
+  const T* __restrict__ diag;
+  const T* __restrict__ upper;
+  const T* __restrict__ lowerL;          // staged: lower[losort[k]]; direct: lower[f]
+  int32_t n;
+  int32_t direct;
+  int32_t xcdRemap;
+};
+
+// ---------------------------------------------------------------------------
+// launch geometry
+// ---------------------------------------------------------------------------
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remapping gives each
+// XCD one contiguous range of rows so neighbour gathers stay in its L2.
+__device__ __forceinline__ int logical_block(int xcdRemap) {
+  const int b = blockIdx.x;
+  if (!xcdRemap) return b;
+  const int G = gridDim.x;
+  const int xcd = b & 7, local = b >> 3;
+  const int q = G >> 3, r = G & 7;
+  return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block-reduce NV sums (and optionally NV-th as a max) and write one value per
+// block into partials[j*pstride + blk].
+template <int NV, bool LASTMAX = false>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* partials,
+                                               int pstride, int blk) {
+  __shared__ double sh[NV][16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const bool mx = LASTMAX && j == NV - 1;
+    const double t = mx ? wave_max(v[j]) : wave_sum(v[j]);
+    if (lane == 0) sh[j][wid] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const bool mx = LASTMAX && j == NV - 1;
+      double t = sh[j][0];
+      for (int w = 1; w < nw; ++w) t = mx ? fmax(t, sh[j][w]) : t + sh[j][w];
+      partials[j * pstride + blk] = t;
+    }
+  }
+}
+
+// True in every thread of the last block to finish; resets the counter.
+__device__ __forceinline__ bool last_block(uint32_t* counter) {
+  __shared__ int amLast;
+  __threadfence();  // release this block's partials (agent scope)
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t prev = atomicAdd(counter, 1u);
+    amLast = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (amLast) {
+    __threadfence();  // acquire the other blocks' partials
+    if (threadIdx.x == 0) atomicExch(counter, 0u);
+  }
+  return amLast != 0;
+}
+
+// Fold partials[j*pstride + 0 .. nblk) in a fixed order (deterministic).
+template <int NV, bool LASTMAX = false>
+__device__ __forceinline__ void fold_partials(const double* partials, int pstride,
+                                              int nblk, double (&out)[NV]) {
+  __shared__ double sh[NV][16];
+  double v[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] = (LASTMAX && j == NV - 1) ? -1.0e308 : 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const double p = partials[j * pstride + b];
+      v[j] = (LASTMAX && j == NV - 1) ? fmax(v[j], p) : v[j] + p;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const bool mx = LASTMAX && j == NV - 1;
+    const double t = mx ? wave_max(v[j]) : wave_sum(v[j]);
+    if (lane == 0) sh[j][wid] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const bool mx = LASTMAX && j == NV - 1;
+    double t = sh[j][0];
+    for (int w = 1; w < nw; ++w) t = mx ? fmax(t, sh[j][w]) : t + sh[j][w];
+    out[j] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the LDU row
+// ---------------------------------------------------------------------------
+
+// Row i of A*x: diag, owner faces ascending, neighbour faces in losort order.
+template <typename T, typename XF>
+__device__ __forceinline__ T ldu_row(const LduOp<T>& A, int i, T xi, XF xf) {
+  T s = A.diag[i] * xi;
+  const int f0 = A.ownStart[i], f1 = A.ownStart[i + 1];
+  for (int f = f0; f < f1; ++f) s += A.upper[f] * xf(A.uppAddr[f]);
+  const int k0 = A.losortStart[i], k1 = A.losortStart[i + 1];
+  if (!A.direct) {
+    for (int k = k0; k < k1; ++k) s += A.lowerL[k] * xf(A.nbrAddr[k]);
+  } else {
+    for (int k = k0; k < k1; ++k) {
+      const int f = A.losort[k];
+      s += A.lowerL[f] * xf(A.nbrAddr[f]);
+    }
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// scalar finalisation (runs in thread 0 of the last block)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool of_converged(const KState* s, double res) {
+  return res < s->tol || (s->relTol > kSmall && res < s->relTol * s->initRes);
+}
+
+// End of one PCG/BiCGStab iteration in inner mode: the host-loop checks of
+// metalFoamBackend.C:676-715 / 908-943, evaluated at sync-batch boundaries.
+__device__ __forceinline__ void inner_iteration_end(KState* s, double checkB) {
+  s->copyBest = 0;
+  s->it += 1;
+  s->batchLeft -= 1;
+  if (s->batchLeft > 0) return;
+  const double res = s->res;
+  if (!isfinite(res) || !isfinite(checkB)) {
+    s->status = ST_SINGULAR;
+    return;
+  }
+  if (res < s->bestRes) {
+    s->bestRes = res;
+    s->noProgress = 0;
+    s->copyBest = 1;
+  } else {
+    s->noProgress += s->batchSize;
+  }
+  if (res <= s->target) {
+    s->status = ST_CONVERGED;
+    return;
+  }
+  if (s->noProgress >= 50) {
+    s->status = ST_STAGNATED;
+    return;
+  }
+  if (s->it >= s->maxIter) {
+    s->status = ST_MAXITER;
+    return;
+  }
+  const int left = s->maxIter - s->it;
+  s->batchSize = s->ips < left ? s->ips : left;
+  s->batchLeft = s->batchSize;
+}
+
+// OpenFOAM loop condition: while ((++nIter < maxIter && !converged) || nIter < minIter)
+__device__ __forceinline__ void openfoam_iteration_end(KState* s) {
+  s->it += 1;
+  const bool conv = of_converged(s, s->res);
+  const bool cont = (s->it < s->maxIter && !conv) || s->it < s->minIter;
+  if (!cont) s->status = conv ? ST_CONVERGED : ST_MAXITER;
+}
+
+// ---------------------------------------------------------------------------
+// PCG kernels
+// ---------------------------------------------------------------------------
+
+// One Chebyshev step fused with its SpMV (cheb_init + ldu_spmv + cheb_step of
+// metalFoamKernels.metal:112-194):
+//   v = A z_{s-1};  w_s = c2 w_{s-1} + c3 rD (r - v);  z_s = z_{s-1} + w_s
+// FIRST: z_0 = w_0 = c1 rD r is formed on the fly from gathered rD and r.
+// LAST:  z_s is the preconditioned residual wA; fuses dot(wA, rA) and the
+//        beta finalisation (reduce_scalars mode 0, metalFoamKernels.metal:336-340).
+template <typename T, bool FIRST, bool LAST>
+__global__ __launch_bounds__(1024) void k_cheb_step(LduOp<T> A, const T* __restrict__ r,
+                                                    const T* __restrict__ rD,
+                                                    const T* __restrict__ zin,
+                                                    T* __restrict__ w, T* __restrict__ zout,
+                                                    KState* st, double* partials, int pstride,
+                                                    int step) {
+  if (st->status != ST_RUNNING) return;
+  const int blk = logical_block(A.xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  const T c2 = (T)st->chebC2[step - 1];
+  const T c3 = (T)st->chebC3[step - 1];
+  double dot = 0.0;
+  if (i < A.n) {
+    T zi_new, wi;
+    if (FIRST) {
+      const T c1 = (T)st->chebC1;
+      const T z0 = c1 * rD[i] * r[i];
+      const T v = ldu_row(A, i, z0, [&](int j) { return c1 * rD[j] * r[j]; });
+      wi = c2 * z0 + c3 * rD[i] * (r[i] - v);
+      zi_new = z0 + wi;
+    } else {
+      const T zi = zin[i];
+      const T v = ldu_row(A, i, zi, [&](int j) { return zin[j]; });
+      wi = c2 * w[i] + c3 * rD[i] * (r[i] - v);
+      zi_new = zi + wi;
+    }
+    if (!LAST) w[i] = wi;
+    zout[i] = zi_new;
+    if (LAST) dot = (double)(zi_new * r[i]);
+  }
+  if (LAST) {
+    double v[1] = {dot};
+    block_partials<1>(v, partials, pstride, blk);
+    if (last_block(&st->counter[0])) {
+      double t[1];
+      fold_partials<1>(partials, pstride, gridDim.x, t);
+      if (threadIdx.x == 0) {
+        const bool first = (st->it == 0);
+        st->rhoOld = st->rho;
+        if (st->mode == MODE_INNER) {
+          const T rho = (T)t[0];
+          st->rho = rho;
+          st->beta = first ? 0.0 : (double)(rho / (T)st->rhoOld);
+        } else {
+          st->rho = t[0];
+          st->beta = first ? 0.0 : t[0] / st->rhoOld;
+        }
+      }
+    }
+  }
+}
+
+// Jacobi apply fused with dot(wA, rA) (pcg_precond_dot, metalFoamKernels.metal:211-233)
+template <typename T>
+__global__ __launch_bounds__(1024) void k_jacobi_dot(int n, int xcdRemap, const T* __restrict__ r,
+                                                     const T* __restrict__ rD,
+                                                     T* __restrict__ wA, KState* st,
+                                                     double* partials, int pstride) {
+  if (st->status != ST_RUNNING) return;
+  const int blk = logical_block(xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  double dot = 0.0;
+  if (i < n) {
+    const T wv = r[i] * rD[i];
+    wA[i] = wv;
+    dot = (double)(wv * r[i]);
+  }
+  double v[1] = {dot};
+  block_partials<1>(v, partials, pstride, blk);
+  if (last_block(&st->counter[0])) {
+    double t[1];
+    fold_partials<1>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      const bool first = (st->it == 0);
+      st->rhoOld = st->rho;
+      if (st->mode == MODE_INNER) {
+        const T rho = (T)t[0];
+        st->rho = rho;
+        st->beta = first ? 0.0 : (double)(rho / (T)st->rhoOld);
+      } else {
+        st->rho = t[0];
+        st->beta = first ? 0.0 : t[0] / st->rhoOld;
+      }
+    }
+  }
+}
+
+// p = wA + beta p fused into A p (pcg_update_p + ldu_spmv_dotw with w = p,
+// metalFoamKernels.metal:62-109, 236-247); p ping-pongs (pold -> pnew) so
+// neighbour gathers read the previous p.  Finalises wApA and alpha
+// (reduce_scalars mode 1) or, in OpenFOAM mode, the singularity check.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_pcg_p_spmv(LduOp<T> A, const T* __restrict__ wA,
+                                                     const T* __restrict__ pold,
+                                                     T* __restrict__ pnew, T* __restrict__ Ap,
+                                                     KState* st, double* partials,
+                                                     int pstride) {
+  if (st->status != ST_RUNNING) return;
+  const int blk = logical_block(A.xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  const T beta = (T)st->beta;
+  double dot = 0.0;
+  if (i < A.n) {
+    const T pi = wA[i] + beta * pold[i];
+    const T y = ldu_row(A, i, pi, [&](int j) { return wA[j] + beta * pold[j]; });
+    pnew[i] = pi;
+    Ap[i] = y;
+    dot = (double)(pi * y);
+  }
+  double v[1] = {dot};
+  block_partials<1>(v, partials, pstride, blk);
+  if (last_block(&st->counter[1])) {
+    double t[1];
+    fold_partials<1>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      if (st->mode == MODE_INNER) {
+        const T wApA = (T)t[0];
+        st->wApA = wApA;
+        st->alpha = (double)((T)st->rho / wApA);
+      } else {
+        st->wApA = t[0];
+        if (fabs(t[0]) / st->normFactor < kVSmall)
+          st->status = ST_SINGULAR;
+        else
+          st->alpha = st->rho / t[0];
+      }
+    }
+  }
+}
+
+// delta += alpha p; r -= alpha Ap; partial sum|r| (pcg_update_sol_res,
+// metalFoamKernels.metal:250-276).  Before the update, copies delta to
+// deltaBest when the previous sync batch improved the residual.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_pcg_update(int n, int xcdRemap, T* __restrict__ delta,
+                                                     T* __restrict__ deltaBest,
+                                                     T* __restrict__ r, const T* __restrict__ p,
+                                                     const T* __restrict__ Ap, KState* st,
+                                                     double* partials, int pstride) {
+  if (st->status != ST_RUNNING) return;
+  const int blk = logical_block(xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  const T alpha = (T)st->alpha;
+  const bool copy = st->copyBest != 0;
+  double m = 0.0;
+  if (i < n) {
+    const T d = delta[i];
+    if (copy) deltaBest[i] = d;
+    delta[i] = d + alpha * p[i];
+    const T ri = r[i] - alpha * Ap[i];
+    r[i] = ri;
+    m = (double)fabs(ri);
+  }
+  double v[1] = {m};
+  block_partials<1>(v, partials, pstride, blk);
+  if (last_block(&st->counter[2])) {
+    double t[1];
+    fold_partials<1>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      if (st->mode == MODE_INNER) {
+        st->res = (double)(T)t[0];
+        inner_iteration_end(st, st->wApA);
+      } else {
+        st->res = t[0] / st->normFactor;
+        openfoam_iteration_end(st);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGStab kernels (metalFoamKernels.metal:279-458; metalFoamBackend.C:784-906)
+// ---------------------------------------------------------------------------
+
+// rho = rA0.rA, beta (reduce_scalars mode 3).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_bicg_rho(int n, int xcdRemap, const T* __restrict__ rA0,
+                                                   const T* __restrict__ r, KState* st,
+                                                   double* partials, int pstride) {
+  if (st->status != ST_RUNNING) return;
+  const int blk = logical_block(xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  double d = 0.0;
+  if (i < n) d = (double)(rA0[i] * r[i]);
+  double v[1] = {d};
+  block_partials<1>(v, partials, pstride, blk);
+  if (last_block(&st->counter[0])) {
+    double t[1];
+    fold_partials<1>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      const bool first = (st->it == 0);
+      st->rA0rAold = st->rA0rA;
+      if (st->mode == MODE_INNER) {
+        const T rr = (T)t[0];
+        st->rA0rA = rr;
+        st->beta = first ? 0.0
+                         : (double)((rr / (T)st->rA0rAold) * ((T)st->alpha / (T)st->omega));
+      } else {
+        st->rA0rA = t[0];
+        if (fabs(t[0]) < kVSmall) {
+          st->status = ST_SINGULAR;
+        } else if (first) {
+          st->beta = 0.0;
+        } else if (fabs(st->omega) < kVSmall) {
+          st->status = ST_SINGULAR;
+        } else {
+          st->beta = (t[0] / st->rA0rAold) * (st->alpha / st->omega);
+        }
+      }
+    }
+  }
+}
+
+// p = rA + beta (p - omega AyA); yA = rD p   (bicg_update_p + emul)
+template <typename T>
+__global__ __launch_bounds__(1024) void k_bicg_p_y(int n, int xcdRemap, T* __restrict__ p,
+                                                   const T* __restrict__ r,
+                                                   const T* __restrict__ AyA,
+                                                   const T* __restrict__ rD, T* __restrict__ yA,
+                                                   KState* st) {
+  if (st->status != ST_RUNNING) return;
+  const int blk = logical_block(xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  const T beta = (T)st->beta, omega = (T)st->omega;
+  if (i < n) {
+    const T pi = r[i] + beta * (p[i] - omega * AyA[i]);
+    p[i] = pi;
+    yA[i] = rD[i] * pi;
+  }
+}
+
+// AyA = A yA, partial rA0.AyA, alpha (ldu_spmv_dotw w = rA0; mode 4)
+template <typename T>
+__global__ __launch_bounds__(1024) void k_bicg_spmv_alpha(LduOp<T> A, const T* __restrict__ yA,
+                                                          const T* __restrict__ rA0,
+                                                          T* __restrict__ AyA, KState* st,
+                                                          double* partials, int pstride) {
+  if (st->status != ST_RUNNING) return;
+  const int blk = logical_block(A.xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  double d = 0.0;
+  if (i < A.n) {
+    const T y = ldu_row(A, i, yA[i], [&](int j) { return yA[j]; });
+    AyA[i] = y;
+    d = (double)(rA0[i] * y);
+  }
+  double v[1] = {d};
+  block_partials<1>(v, partials, pstride, blk);
+  if (last_block(&st->counter[1])) {
+    double t[1];
+    fold_partials<1>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      if (st->mode == MODE_INNER) {
+        const T a = (T)t[0];
+        st->rA0AyA = a;
+        st->alpha = (double)((T)st->rA0rA / a);
+      } else {
+        st->rA0AyA = t[0];
+        st->alpha = st->rA0rA / t[0];
+      }
+    }
+  }
+}
+
+// sA = rA - alpha AyA, partial |sA|; breakdown and early-exit tests
+// (bicg_s_res; metalFoamBackend.C:828-866).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_bicg_s(int n, int xcdRemap, T* __restrict__ sA,
+                                                 const T* __restrict__ r,
+                                                 const T* __restrict__ AyA, KState* st,
+                                                 double* partials, int pstride) {
+  if (st->status != ST_RUNNING) return;
+  const int blk = logical_block(xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  const T alpha = (T)st->alpha;
+  double m = 0.0;
+  if (i < n) {
+    const T s = r[i] - alpha * AyA[i];
+    sA[i] = s;
+    m = (double)fabs(s);
+  }
+  double v[1] = {m};
+  block_partials<1>(v, partials, pstride, blk);
+  if (last_block(&st->counter[2])) {
+    double t[1];
+    fold_partials<1>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      if (st->mode == MODE_INNER) {
+        const double res = (double)(T)t[0];
+        st->res = res;
+        if (!isfinite(st->rA0rA) || !isfinite(st->alpha) || !isfinite(res))
+          st->status = ST_SINGULAR;
+        else if (res <= st->target)
+          st->status = ST_EARLY;
+      } else {
+        st->res = t[0] / st->normFactor;
+        if (of_converged(st, st->res)) st->status = ST_EARLY;
+      }
+    }
+  }
+}
+
+// zA = rD sA (own row) and tA = A zA with zA gathered on the fly; partials
+// tA.sA and tA.tA; omega (emul + ldu_spmv_dotw w = sA + dot_partial; mode 5).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_bicg_z_spmv(LduOp<T> A, const T* __restrict__ rD,
+                                                      const T* __restrict__ sA,
+                                                      T* __restrict__ zA, T* __restrict__ tA,
+                                                      KState* st, double* partials,
+                                                      int pstride) {
+  if (st->status != ST_RUNNING) return;
+  const int blk = logical_block(A.xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  double d[2] = {0.0, 0.0};
+  if (i < A.n) {
+    const T zi = rD[i] * sA[i];
+    const T t = ldu_row(A, i, zi, [&](int j) { return rD[j] * sA[j]; });
+    zA[i] = zi;
+    tA[i] = t;
+    d[0] = (double)(sA[i] * t);
+    d[1] = (double)(t * t);
+  }
+  block_partials<2>(d, partials, pstride, blk);
+  if (last_block(&st->counter[3])) {
+    double t[2];
+    fold_partials<2>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      if (st->mode == MODE_INNER) {
+        const T tAsA = (T)t[0], tAtA = (T)t[1];
+        st->tAsA = tAsA;
+        st->tAtA = tAtA;
+        st->omega = (double)(tAsA / tAtA);
+      } else {
+        st->tAsA = t[0];
+        st->tAtA = t[1];
+        st->omega = t[0] / t[1];
+      }
+    }
+  }
+}
+
+// delta += alpha yA + omega zA; rA = sA - omega tA; partial |rA|
+// (bicg_update_sol_res); or, after an |sA| exit, delta += alpha yA
+// (bicg_early_update, metalFoamKernels.metal:415-426).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_bicg_update(int n, int xcdRemap, T* __restrict__ delta,
+                                                      T* __restrict__ deltaBest,
+                                                      const T* __restrict__ yA,
+                                                      const T* __restrict__ zA,
+                                                      T* __restrict__ r,
+                                                      const T* __restrict__ sA,
+                                                      const T* __restrict__ tA, KState* st,
+                                                      double* partials, int pstride) {
+  const int status = st->status;
+  if (status != ST_RUNNING && status != ST_EARLY) return;
+  const int blk = logical_block(xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  const T alpha = (T)st->alpha;
+  if (status == ST_EARLY) {
+    if (i < n) delta[i] += alpha * yA[i];
+    if (last_block(&st->counter[4])) {
+      if (threadIdx.x == 0) {
+        st->it += 1;
+        st->status = ST_CONVERGED;
+      }
+    }
+    return;
+  }
+  const T omega = (T)st->omega;
+  const bool copy = st->copyBest != 0;
+  double m = 0.0;
+  if (i < n) {
+    const T d = delta[i];
+    if (copy) deltaBest[i] = d;
+    delta[i] = d + (alpha * yA[i] + omega * zA[i]);
+    const T ri = sA[i] - omega * tA[i];
+    r[i] = ri;
+    m = (double)fabs(ri);
+  }
+  double v[1] = {m};
+  block_partials<1>(v, partials, pstride, blk);
+  if (last_block(&st->counter[4])) {
+    double t[1];
+    fold_partials<1>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      if (st->mode == MODE_INNER) {
+        st->res = (double)(T)t[0];
+        inner_iteration_end(st, st->omega);
+      } else {
+        st->res = t[0] / st->normFactor;
+        openfoam_iteration_end(st);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// inner-solve initialisation
+// ---------------------------------------------------------------------------
+
+// rA = T(r64 - mean); delta = p0 = p1 = AyA = 0; rA0 = rA (BiCGStab).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_inner_init(int n, const double* __restrict__ r64,
+                                                     const double* mean, T* __restrict__ rA,
+                                                     T* __restrict__ rA0, T* __restrict__ delta,
+                                                     T* __restrict__ p0, T* __restrict__ p1,
+                                                     T* __restrict__ AyA) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = mean ? *mean : 0.0;
+  const T v = (T)(mean ? r64[i] - m : r64[i]);
+  rA[i] = v;
+  if (rA0) rA0[i] = v;
+  if (delta) delta[i] = (T)0;
+  if (p0) p0[i] = (T)0;
+  if (p1) p1[i] = (T)0;
+  if (AyA) AyA[i] = (T)0;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 refinement kernels (metalFoamSolver.C:101-365 on the device)
+// ---------------------------------------------------------------------------
+
+// u = A 1 (sumA), partials: sum psi, #{|u_i| > 1e-8 |diag_i|}, u.u
+__global__ __launch_bounds__(1024) void k_rowsum(LduOp<double> A, const double* __restrict__ psi,
+                                                 double* __restrict__ u, OState* os,
+                                                 double* partials, int pstride) {
+  const int blk = logical_block(A.xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < A.n) {
+    const double ui = ldu_row(A, i, 1.0, [](int) { return 1.0; });
+    u[i] = ui;
+    v[0] = psi[i];
+    v[1] = (fabs(ui) > 1e-8 * fabs(A.diag[i])) ? 1.0 : 0.0;
+    v[2] = ui * ui;
+  }
+  block_partials<3>(v, partials, pstride, blk);
+  if (last_block(&os->counter[0])) {
+    double t[3];
+    fold_partials<3>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      os->sumPsi = t[0];
+      os->xbar = t[0] / (double)A.n;
+      os->nPinned = t[1];
+      os->uu = t[2];
+    }
+  }
+}
+
+// wA = A psi; r = b - wA; partials: sum|r|, normFactor sum, sum r.
+// normFactor = sum(|A psi - xbar u| + |b - xbar u|) + small (metalFoamSolver.C:120).
+__global__ __launch_bounds__(1024) void k_initial_residual(LduOp<double> A,
+                                                           const double* __restrict__ psi,
+                                                           const double* __restrict__ b,
+                                                           const double* __restrict__ u,
+                                                           double* __restrict__ r, OState* os,
+                                                           double* partials, int pstride) {
+  const int blk = logical_block(A.xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  const double xbar = os->xbar;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < A.n) {
+    const double wA = ldu_row(A, i, psi[i], [&](int j) { return psi[j]; });
+    const double ri = b[i] - wA;
+    r[i] = ri;
+    const double t = u[i] * xbar;
+    v[0] = fabs(ri);
+    v[1] = fabs(wA - t) + fabs(b[i] - t);
+    v[2] = ri;
+  }
+  block_partials<3>(v, partials, pstride, blk);
+  if (last_block(&os->counter[1])) {
+    double t[3];
+    fold_partials<3>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      os->normFactor = t[1] + kSmall;
+      os->sumMagR = t[0];
+      os->initRes = t[0] / os->normFactor;
+      os->finalRes = os->initRes;
+      os->sumR = t[2];
+      os->rmean = t[2] / (double)A.n;
+    }
+  }
+}
+
+// Inner operator of one restart: diag* = deflate ? diag - u : diag;
+// diag32 = float(diag*), rD = float(1/diag*) (setCoeffs, metalFoamBackend.C:557-562);
+// deflation sign-flip test (metalFoamSolver.C:199-207) and the Gershgorin bound
+// max_i(1 + sum_f |upper_f| / |diag*_i|) (metalFoamSolver.C:229-249, |upper| only).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_prepare_inner(LduOp<double> A,
+                                                        const double* __restrict__ upperF,
+                                                        const int32_t* __restrict__ losort,
+                                                        const double* __restrict__ u,
+                                                        int deflate, int wantBeta,
+                                                        T* __restrict__ diagT,
+                                                        T* __restrict__ rDT, OState* os,
+                                                        double* partials, int pstride) {
+  const int blk = logical_block(A.xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  double v[2] = {0.0, -1.0e308};
+  if (i < A.n) {
+    const double d = A.diag[i];
+    double ds = d;
+    if (deflate) {
+      ds = d - u[i];
+      if (ds * d <= kVSmall * fabs(ds)) v[0] = 1.0;
+    }
+    diagT[i] = (T)ds;
+    rDT[i] = (T)(1.0 / ds);
+    if (wantBeta) {
+      double so = 0.0;
+      for (int f = A.ownStart[i]; f < A.ownStart[i + 1]; ++f) so += fabs(upperF[f]);
+      for (int k = A.losortStart[i]; k < A.losortStart[i + 1]; ++k)
+        so += fabs(upperF[losort[k]]);
+      v[1] = 1.0 + so / fabs(ds);
+    }
+  }
+  block_partials<2, true>(v, partials, pstride, blk);
+  if (last_block(&os->counter[2])) {
+    double t[2];
+    fold_partials<2, true>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      os->bail = t[0];
+      if (wantBeta) {
+        if (deflate)
+          os->betaDefl = t[1];
+        else
+          os->betaExact = t[1];
+      }
+    }
+  }
+}
+
+// Face staging, once per matrix: lowerL64[k] = lower[losort[k]] (upper when
+// symmetric), upper32 = float(upper), lowerL32 = float(lowerL64).
+__global__ __launch_bounds__(1024) void k_stage_faces(int nf, const int32_t* __restrict__ losort,
+                                                      const double* __restrict__ upper,
+                                                      const double* __restrict__ lower,
+                                                      double* __restrict__ lowerL64,
+                                                      float* __restrict__ upper32,
+                                                      float* __restrict__ lowerL32,
+                                                      float* __restrict__ lower32) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  const double* lo = lower ? lower : upper;
+  const double l = lo[losort[k]];
+  lowerL64[k] = l;
+  upper32[k] = (float)upper[k];
+  lowerL32[k] = (float)l;
+  if (lower32) lower32[k] = (float)lo[k];
+}
+
+// sum(double(delta)) -> dmean
+template <typename T>
+__global__ __launch_bounds__(1024) void k_delta_mean(int n, int xcdRemap,
+                                                     const T* __restrict__ delta, OState* os,
+                                                     double* partials, int pstride) {
+  const int blk = logical_block(xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  double v[1] = {i < n ? (double)delta[i] : 0.0};
+  block_partials<1>(v, partials, pstride, blk);
+  if (last_block(&os->counter[3])) {
+    double t[1];
+    fold_partials<1>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) os->dmean = t[0] / (double)n;
+  }
+}
+
+// psiNew = psiOld + (double(delta) - dmean); r = b - A psiNew with psiNew
+// gathered on the fly; partials u.r, sum|r|, sum r (metalFoamSolver.C:318-365).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_refine_residual(
+    LduOp<double> A, const double* __restrict__ psiOld, const T* __restrict__ delta,
+    double* __restrict__ psiNew, const double* __restrict__ b, const double* __restrict__ u,
+    double* __restrict__ r, int useMean, OState* os, double* partials, int pstride) {
+  const int blk = logical_block(A.xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  const double dm = useMean ? os->dmean : 0.0;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < A.n) {
+    auto pf = [&](int j) { return psiOld[j] + ((double)delta[j] - dm); };
+    const double pi = pf(i);
+    const double wA = ldu_row(A, i, pi, pf);
+    psiNew[i] = pi;
+    const double ri = b[i] - wA;
+    r[i] = ri;
+    v[0] = u[i] * ri;
+    v[1] = fabs(ri);
+    v[2] = ri;
+  }
+  block_partials<3>(v, partials, pstride, blk);
+  if (last_block(&os->counter[4])) {
+    double t[3];
+    fold_partials<3>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      os->ur = t[0];
+      os->sumMagR = t[1];
+      os->finalRes = t[1] / os->normFactor;
+      os->sumR = t[2];
+      os->rmean = t[2] / (double)A.n;
+      os->c = (os->uu > kVSmall) ? t[0] / os->uu : 0.0;
+    }
+  }
+}
+
+// Constant-mode correction psi += c, r -= c u (metalFoamSolver.C:342-361).
+__global__ __launch_bounds__(1024) void k_const_shift(int n, int xcdRemap, double* __restrict__ psi,
+                                                      double* __restrict__ r,
+                                                      const double* __restrict__ u, OState* os,
+                                                      double* partials, int pstride) {
+  const int blk = logical_block(xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  const double c = os->c;
+  double v[2] = {0.0, 0.0};
+  if (i < n) {
+    psi[i] += c;
+    const double ri = r[i] - c * u[i];
+    r[i] = ri;
+    v[0] = fabs(ri);
+    v[1] = ri;
+  }
+  block_partials<2>(v, partials, pstride, blk);
+  if (last_block(&os->counter[5])) {
+    double t[2];
+    fold_partials<2>(partials, pstride, gridDim.x, t);
+    if (threadIdx.x == 0) {
+      os->sumMagR = t[0];
+      os->finalRes = t[0] / os->normFactor;
+      os->sumR = t[1];
+      os->rmean = t[1] / (double)n;
+    }
+  }
+}
+
+// Plain y = A x.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_spmv(LduOp<T> A, const T* __restrict__ x,
+                                               T* __restrict__ y) {
+  const int blk = logical_block(A.xcdRemap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  if (i < A.n) y[i] = ldu_row(A, i, x[i], [&](int j) { return x[j]; });
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(1024) void k_convert(int n, const S* __restrict__ s,
+                                                  D* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = (D)s[i];
+}
+
+// lowerL64 = lower[losort] for the staged FP64 operator, reciprocal diag.
+__global__ __launch_bounds__(1024) void k_recip(int n, const double* __restrict__ d,
+                                                double* __restrict__ rD) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rD[i] = 1.0 / d[i];
+}
+
+// Build losortLow[k] = lowAddr[losort[k]] (once per mesh).
+__global__ __launch_bounds__(1024) void k_gather_addr(int nf, const int32_t* __restrict__ losort,
+                                                      const int32_t* __restrict__ low,
+                                                      int32_t* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nf) out[k] = low[losort[k]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_copy(int n, const T* __restrict__ s, T* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = s[i];
+}
+
+}  // namespace hipfoam
