@@ -358,12 +358,14 @@ void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, bool 
                      w.deltaBest.p, w.yA.p, w.zA.p, w.r.p, w.sA.p, w.tA.p, st, P, ps);
 }
 
+// Iteration `first + k` uses p-buffer parity (first + k) & 1; graphs are
+// captured with first = 0 and an even chunk, so every replay starts even.
 template <typename T>
 void enqueue_chunk(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
-                   int chunk, bool capturing) {
+                   int chunk, bool capturing, long first = 0) {
   for (int k = 0; k < chunk; ++k) {
     if (kind == KIND_PCG)
-      enqueue_pcg_iter<T>(c, A, rD, order, k & 1, capturing);
+      enqueue_pcg_iter<T>(c, A, rD, order, (int)((first + k) & 1), capturing);
     else
       enqueue_bicg_iter<T>(c, A, rD, capturing);
   }
@@ -416,7 +418,7 @@ int run_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int 
   bool havePrev = false;
   for (long k = 0; k < maxChunks + 2; ++k) {
     if (eager) {
-      enqueue_chunk<T>(c, kind, A, rD, order, chunk, false);
+      enqueue_chunk<T>(c, kind, A, rD, order, chunk, false, k * chunk);
       CKL();
     } else {
       CK(hipGraphLaunch(ex, c->stream));

@@ -135,25 +135,31 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* partials
       const bool mx = LASTMAX && j == NV - 1;
       double t = sh[j][0];
       for (int w = 1; w < nw; ++w) t = mx ? fmax(t, sh[j][w]) : t + sh[j][w];
-      partials[j * pstride + blk] = t;
+      // agent-scope store: written through to the point of coherence shared
+      // by all XCDs, so no L2 writeback is needed before signalling
+      __hip_atomic_store(&partials[j * pstride + blk], t, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
 
 // True in every thread of the last block to finish; resets the counter.
+// Thread 0 wrote this block's partials with agent-scope stores; it waits for
+// them to complete, then bumps the agent-scope counter.  The last block reads
+// the partials with agent-scope loads (fold_partials).  No cache-wide
+// writeback or invalidate is involved (a __threadfence per block would flush
+// the XCD's L2 once per block).
 __device__ __forceinline__ bool last_block(uint32_t* counter) {
   __shared__ int amLast;
-  __threadfence();  // release this block's partials (agent scope)
-  __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t prev = atomicAdd(counter, 1u);
+    __builtin_amdgcn_s_waitcnt(0);  // partial stores acknowledged
+    const uint32_t prev =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     amLast = (prev == gridDim.x - 1);
   }
   __syncthreads();
-  if (amLast) {
-    __threadfence();  // acquire the other blocks' partials
-    if (threadIdx.x == 0) atomicExch(counter, 0u);
-  }
+  if (amLast && threadIdx.x == 0)
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return amLast != 0;
 }
 
@@ -168,7 +174,8 @@ __device__ __forceinline__ void fold_partials(const double* partials, int pstrid
   for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-      const double p = partials[j * pstride + b];
+      const double p =
+          __hip_atomic_load(&partials[j * pstride + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       v[j] = (LASTMAX && j == NV - 1) ? fmax(v[j], p) : v[j] + p;
     }
   }
