@@ -181,6 +181,33 @@ int32_t hipfoam_solve_fp64_device(hipfoam_context* ctx, int32_t solver,
                                   hipfoam_performance* perf,
                                   double* residual);
 
+/* ---- adjoint (no reference counterpart) ------------------------------- */
+
+/* Gradient of a loss L(x) through the solve A x = b: given the forward
+ * solution x and g = dL/dx, solves the transpose system A^T lambda = g with
+ * the same refined solver and controls (upper and lower swap roles on the
+ * same LDU addressing, lduMatrix::Tmul semantics; lambda0 = 0), then forms
+ *   dL/dupper_f = -lambda[l] x[u],  dL/dlower_f = -lambda[u] x[l],
+ *   dL/ddiag    = -lambda o x,      dL/db       = lambda.
+ * For a symmetric matrix (lower == NULL) the single face coefficient's
+ * gradient -(lambda[l] x[u] + lambda[u] x[l]) goes to dUpper and dLower is
+ * not written.  Any gradient pointer may be NULL.  perf describes the
+ * transpose solve. */
+int32_t hipfoam_adjoint(hipfoam_context* ctx, int32_t solver,
+                        const double* diag, const double* upper,
+                        const double* lower, const double* x, const double* g,
+                        double* lambda, double* dUpper, double* dLower,
+                        double* dDiag, double* dSource,
+                        const hipfoam_controls* controls,
+                        hipfoam_performance* perf);
+int32_t hipfoam_adjoint_device(hipfoam_context* ctx, int32_t solver,
+                               const double* diag, const double* upper,
+                               const double* lower, const double* x,
+                               const double* g, double* lambda, double* dUpper,
+                               double* dLower, double* dDiag, double* dSource,
+                               const hipfoam_controls* controls,
+                               hipfoam_performance* perf);
+
 /* ---- operator primitives --------------------------------------------- */
 
 /* y = A x with the operator last staged: precision HIPFOAM_FP32 uses the
@@ -218,13 +245,16 @@ int32_t hipfoam_synchronize(hipfoam_context* ctx);
  *   0 = losort-ordered lower coefficients (one indirection),
  *   1 = direct lower[losort[k]] / lowAddr[losort[k]] (two indirections),
  * and the iteration driver:
- *   graphChunk = iterations captured per hipGraph launch (0 = eager). */
+ *   graphChunk = iterations captured per hipGraph launch (0 = eager).
+ * A row kernel launches min(ceil(n / blockSize), gridLimit) workgroups; each
+ * walks a contiguous run of blockSize-row tiles. */
 typedef struct
 {
     int32_t spmvSchedule;
     int32_t blockSize;    /* 64 .. 1024, multiple of 64 */
     int32_t xcdRemap;     /* 1 = contiguous row range per XCD */
     int32_t graphChunk;
+    int32_t gridLimit;    /* max workgroups per row kernel; 0 = 8 per CU */
 } hipfoam_schedule;
 
 int32_t hipfoam_get_schedule(const hipfoam_context* ctx, hipfoam_schedule* s);

@@ -58,7 +58,8 @@ class Performance(C.Structure):
 
 class Schedule(C.Structure):
     _fields_ = [("spmvSchedule", C.c_int32), ("blockSize", C.c_int32),
-                ("xcdRemap", C.c_int32), ("graphChunk", C.c_int32)]
+                ("xcdRemap", C.c_int32), ("graphChunk", C.c_int32),
+                ("gridLimit", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -93,6 +94,12 @@ SIGNATURES = {
     "hipfoam_solve_fp64_device": (C.c_int32, [_P, C.c_int32, _vp, _vp, _vp, _vp,
                                               _vp, C.POINTER(Controls),
                                               C.POINTER(Performance), _vp]),
+    "hipfoam_adjoint": (C.c_int32, [_P, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    _vp, _vp, _vp, C.POINTER(Controls),
+                                    C.POINTER(Performance)]),
+    "hipfoam_adjoint_device": (C.c_int32, [_P, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp,
+                                           _vp, _vp, _vp, _vp, C.POINTER(Controls),
+                                           C.POINTER(Performance)]),
     "hipfoam_amul": (C.c_int32, [_P, C.c_int32, _vp, _vp]),
     "hipfoam_amul_device": (C.c_int32, [_P, C.c_int32, _vp, _vp]),
     "hipfoam_set_matrix": (C.c_int32, [_P, _vp, _vp, _vp]),

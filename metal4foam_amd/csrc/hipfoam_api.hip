@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <new>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -94,10 +95,10 @@ struct Work {
 enum Kind { KIND_PCG = 0, KIND_BICG = 1 };
 
 struct GraphKey {
-  int kind, prec, order, chunk, direct, remap, block, gen;
+  int kind, prec, order, chunk, direct, remap, block, grid, gen;
   bool operator<(const GraphKey& o) const {
-    return std::tie(kind, prec, order, chunk, direct, remap, block, gen) <
-           std::tie(o.kind, o.prec, o.order, o.chunk, o.direct, o.remap, o.block, o.gen);
+    return std::tie(kind, prec, order, chunk, direct, remap, block, grid, gen) <
+           std::tie(o.kind, o.prec, o.order, o.chunk, o.direct, o.remap, o.block, o.grid, o.gen);
   }
 };
 
@@ -106,6 +107,7 @@ struct KernelTimer {
   bool on = false;
   int kclass = 0;
   std::vector<hipEvent_t> ev;
+  std::vector<int> count;  // launches inside each event pair
   size_t used = 0;
   int64_t launches = 0;
   double seconds = 0;
@@ -138,6 +140,8 @@ struct hipfoam_context {
 
   // FP64 outer vectors
   Buf<double> psi0, psi1, b, r64, u64, scratch64;
+  // adjoint staging (host-pointer entry point)
+  Buf<double> adjX, adjG, adjLam, adjDU, adjDL, adjDD, adjDB;
   Work<float> w32;
   Work<double> w64;
 
@@ -151,12 +155,37 @@ struct hipfoam_context {
   hipEvent_t evPoll[2] = {nullptr, nullptr};
   hipEvent_t evT0 = nullptr, evT1 = nullptr;
 
-  hipfoam_schedule sched{0, 256, 1, 8};
+  hipfoam_schedule sched{0, 256, 1, 8, 0};
   std::map<GraphKey, hipGraphExec_t> graphs;
   KernelTimer timer;
 };
 
 namespace {
+
+// Workgroups of a row-kernel launch: one per tile of blockSize rows, capped at
+// gridLimit (each workgroup then walks a contiguous run of tiles).
+inline int row_grid(const hipfoam_context* c) {
+  const int bs = c->sched.blockSize;
+  const int tiles = std::max(1, (c->n + bs - 1) / bs);
+  const int cap = c->sched.gridLimit > 0 ? c->sched.gridLimit : 8 * std::max(c->numCUs, 1);
+  return std::min(tiles, cap);
+}
+
+template <int NV, bool LASTMAX, typename F>
+void launch_fin(hipfoam_context* c, int nblk, bool checkStatus, int runMask, F f) {
+  hipLaunchKernelGGL((k_finalize<NV, LASTMAX, F>), dim3(1), dim3(256), 0, c->stream,
+                     c->partials.p, c->pstride, nblk,
+                     checkStatus ? &c->kst.p->status : (const int32_t*)nullptr, runMask, f);
+}
+
+// Second stage of a reduction: one workgroup folds the partials of the
+// kernel just enqueued (nblk workgroups) and applies the finaliser f.  With
+// checkStatus, it runs only when the Krylov status bit is set in runMask.
+constexpr int kRunning = 1 << ST_RUNNING;
+constexpr int kRunningOrEarly = (1 << ST_RUNNING) | (1 << ST_EARLY);
+
+template <int NV, bool LASTMAX = false, typename F>
+void launch_fin(hipfoam_context* c, int nblk, bool checkStatus, int runMask, F f);
 
 #define CK(expr)                                                              \
   do {                                                                        \
@@ -257,16 +286,19 @@ int timer_flush(hipfoam_context* c) {
     float ms = 0.f;
     CK(hipEventElapsedTime(&ms, t.ev[k], t.ev[k + 1]));
     t.seconds += 1e-3 * ms;
-    t.launches += 1;
+    t.launches += t.count[k / 2];
   }
   t.used = 0;
   return HIPFOAM_OK;
 }
 
+// Brackets `count` back-to-back launches of one kernel class with a pair of
+// HIP events on the context stream (only outside graph capture).
 struct TimedLaunch {
   hipfoam_context* c;
   bool active;
-  TimedLaunch(hipfoam_context* cc, int kclass, bool capturing) : c(cc) {
+  int count;
+  TimedLaunch(hipfoam_context* cc, int kclass, bool capturing, int n = 1) : c(cc), count(n) {
     KernelTimer& t = c->timer;
     active = t.on && !capturing && t.kclass == kclass;
     if (!active) return;
@@ -277,6 +309,7 @@ struct TimedLaunch {
     if (!active) return;
     KernelTimer& t = c->timer;
     (void)hipEventRecord(t.ev[t.used + 1], c->stream);
+    t.count[t.used / 2] = count;
     t.used += 2;
   }
 };
@@ -290,33 +323,46 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
                       int parity, bool capturing) {
   Work<T>& w = work<T>(c);
   const int bs = c->sched.blockSize;
-  const dim3 grid(blocks_for(c->n, bs)), block(bs);
+  const dim3 grid(row_grid(c)), block(bs);
   KState* st = c->kst.p;
   double* P = c->partials.p;
   const int ps = c->pstride;
   if (order > 1) {
     const T* zin = nullptr;
+    // the order-3 interior steps (2 .. order-2) are consecutive launches,
+    // timed as one region of kernel class 0
+    TimedLaunch* interior = nullptr;
+    alignas(TimedLaunch) unsigned char interiorBuf[sizeof(TimedLaunch)];
     for (int s = 1; s < order; ++s) {
       const bool first = (s == 1), last = (s == order - 1);
       T* zout = last ? w.wA.p : ((s & 1) ? w.z0.p : w.z1.p);
-      TimedLaunch tl(c, first ? 3 : (last ? 4 : 0), capturing);
-      if (first && last)
-        hipLaunchKernelGGL((k_cheb_step<T, true, true>), grid, block, 0, c->stream, A, w.r.p,
-                           rD, zin, w.w.p, zout, st, P, ps, s);
-      else if (first)
-        hipLaunchKernelGGL((k_cheb_step<T, true, false>), grid, block, 0, c->stream, A, w.r.p,
-                           rD, zin, w.w.p, zout, st, P, ps, s);
-      else if (last)
-        hipLaunchKernelGGL((k_cheb_step<T, false, true>), grid, block, 0, c->stream, A, w.r.p,
-                           rD, zin, w.w.p, zout, st, P, ps, s);
-      else
-        hipLaunchKernelGGL((k_cheb_step<T, false, false>), grid, block, 0, c->stream, A,
-                           w.r.p, rD, zin, w.w.p, zout, st, P, ps, s);
+      if (s == 2 && order > 3) interior = new (interiorBuf) TimedLaunch(c, 0, capturing, order - 3);
+      {
+        TimedLaunch tl(c, first ? 3 : (last ? 4 : -1), capturing);
+        if (first && last)
+          hipLaunchKernelGGL((k_cheb_step<T, true, true>), grid, block, 0, c->stream, A, w.r.p,
+                             rD, zin, w.w.p, zout, st, P, ps, s);
+        else if (first)
+          hipLaunchKernelGGL((k_cheb_step<T, true, false>), grid, block, 0, c->stream, A, w.r.p,
+                             rD, zin, w.w.p, zout, st, P, ps, s);
+        else if (last)
+          hipLaunchKernelGGL((k_cheb_step<T, false, true>), grid, block, 0, c->stream, A, w.r.p,
+                             rD, zin, w.w.p, zout, st, P, ps, s);
+        else
+          hipLaunchKernelGGL((k_cheb_step<T, false, false>), grid, block, 0, c->stream, A,
+                             w.r.p, rD, zin, w.w.p, zout, st, P, ps, s);
+      }
+      if (interior && s == order - 2) {
+        interior->~TimedLaunch();
+        interior = nullptr;
+      }
+      if (last) launch_fin<1>(c, grid.x, true, kRunning, FinBeta<T>{st});
       zin = zout;
     }
   } else {
     hipLaunchKernelGGL((k_jacobi_dot<T>), grid, block, 0, c->stream, c->n, A.xcdRemap, w.r.p,
                        rD, w.wA.p, st, P, ps);
+    launch_fin<1>(c, grid.x, true, kRunning, FinBeta<T>{st});
   }
   T* pold = parity ? w.p1.p : w.p0.p;
   T* pnew = parity ? w.p0.p : w.p1.p;
@@ -325,21 +371,25 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
     hipLaunchKernelGGL((k_pcg_p_spmv<T>), grid, block, 0, c->stream, A, w.wA.p, pold, pnew,
                        w.Ap.p, st, P, ps);
   }
+  launch_fin<1>(c, grid.x, true, kRunning, FinAlpha<T>{st});
   hipLaunchKernelGGL((k_pcg_update<T>), grid, block, 0, c->stream, c->n, A.xcdRemap,
                      w.delta.p, w.deltaBest.p, w.r.p, pnew, w.Ap.p, st, P, ps);
+  launch_fin<1>(c, grid.x, true, kRunning, FinPcgRes<T>{st});
 }
 
 template <typename T>
 void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, bool capturing) {
   Work<T>& w = work<T>(c);
   const int bs = c->sched.blockSize;
-  const dim3 grid(blocks_for(c->n, bs)), block(bs);
+  const dim3 grid(row_grid(c)), block(bs);
   KState* st = c->kst.p;
   double* P = c->partials.p;
   const int ps = c->pstride;
   const int n = c->n, rm = A.xcdRemap;
+  const int nb = grid.x;
   hipLaunchKernelGGL((k_bicg_rho<T>), grid, block, 0, c->stream, n, rm, w.rA0.p, w.r.p, st, P,
                      ps);
+  launch_fin<1>(c, nb, true, kRunning, FinBicgRho<T>{st});
   hipLaunchKernelGGL((k_bicg_p_y<T>), grid, block, 0, c->stream, n, rm, w.p0.p, w.r.p, w.AyA.p,
                      rD, w.yA.p, st);
   {
@@ -347,15 +397,19 @@ void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, bool 
     hipLaunchKernelGGL((k_bicg_spmv_alpha<T>), grid, block, 0, c->stream, A, w.yA.p, w.rA0.p,
                        w.AyA.p, st, P, ps);
   }
+  launch_fin<1>(c, nb, true, kRunning, FinBicgAlpha<T>{st});
   hipLaunchKernelGGL((k_bicg_s<T>), grid, block, 0, c->stream, n, rm, w.sA.p, w.r.p, w.AyA.p,
                      st, P, ps);
+  launch_fin<1>(c, nb, true, kRunning, FinBicgS<T>{st});
   {
     TimedLaunch tl(c, 2, capturing);
     hipLaunchKernelGGL((k_bicg_z_spmv<T>), grid, block, 0, c->stream, A, rD, w.sA.p, w.zA.p,
                        w.tA.p, st, P, ps);
   }
+  launch_fin<2>(c, nb, true, kRunning, FinBicgOmega<T>{st});
   hipLaunchKernelGGL((k_bicg_update<T>), grid, block, 0, c->stream, n, rm, w.delta.p,
                      w.deltaBest.p, w.yA.p, w.zA.p, w.r.p, w.sA.p, w.tA.p, st, P, ps);
+  launch_fin<1>(c, nb, true, kRunningOrEarly, FinBicgUpdate<T>{st});
 }
 
 // Iteration `first + k` uses p-buffer parity (first + k) & 1; graphs are
@@ -375,7 +429,7 @@ template <typename T>
 int get_graph(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
               int chunk, hipGraphExec_t* out) {
   GraphKey key{(int)kind, (int)sizeof(T), order, chunk, A.direct, A.xcdRemap,
-               c->sched.blockSize, c->gen};
+               c->sched.blockSize, row_grid(c), c->gen};
   auto it = c->graphs.find(key);
   if (it != c->graphs.end()) {
     *out = it->second;
@@ -570,12 +624,12 @@ int stage_matrix(hipfoam_context* c, const double* diag, const double* upper,
 // FP32 inner operator diagonal: diag* = deflate ? diag - u : diag (+ rD, and
 // the Gershgorin bound when wantBeta).  Writes os->bail / os->beta*.
 int prepare_inner(hipfoam_context* c, int deflate, int wantBeta) {
-  const int n = c->n;
   const int bs = c->sched.blockSize;
   LduOp<double> A = op64(c);
-  hipLaunchKernelGGL((k_prepare_inner<float>), dim3(blocks_for(n, bs)), dim3(bs), 0, c->stream,
+  hipLaunchKernelGGL((k_prepare_inner<float>), dim3(row_grid(c)), dim3(bs), 0, c->stream,
                      A, c->upper64.p, c->losort.p, c->u64.p, deflate, wantBeta, c->diag32.p,
                      c->rD32.p, c->ost.p, c->partials.p, c->pstride);
+  launch_fin<2, true>(c, row_grid(c), false, 0, FinPrepare{c->ost.p, deflate, wantBeta});
   CKL();
   c->have32 = true;
   return HIPFOAM_OK;
@@ -589,13 +643,15 @@ int read_ostate(hipfoam_context* c) {
 
 // initial FP64 residual + normFactor (metalFoamSolver.C:101-129): psi in psiCur.
 int initial_residual(hipfoam_context* c, const double* psiCur, const double* src) {
-  const int n = c->n, bs = c->sched.blockSize;
+  const int bs = c->sched.blockSize;
   LduOp<double> A = op64(c);
-  const dim3 g(blocks_for(n, bs)), bl(bs);
+  const dim3 g(row_grid(c)), bl(bs);
   hipLaunchKernelGGL(k_rowsum, g, bl, 0, c->stream, A, psiCur, c->u64.p, c->ost.p, c->partials.p,
                      c->pstride);
+  launch_fin<3>(c, g.x, false, 0, FinRowsum{c->ost.p, c->n});
   hipLaunchKernelGGL(k_initial_residual, g, bl, 0, c->stream, A, psiCur, src, c->u64.p,
                      c->r64.p, c->ost.p, c->partials.p, c->pstride);
+  launch_fin<3>(c, g.x, false, 0, FinInitialResidual{c->ost.p, c->n});
   CKL();
   return read_ostate(c);
 }
@@ -685,7 +741,7 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
     bool singular = false;
     double prevRes = pf.initialResidual;
     const int bs = c->sched.blockSize;
-    const dim3 g(blocks_for(n, bs)), bl(bs);
+    const dim3 g(row_grid(c)), bl(bs);
     for (outer = 0; outer <= ctl->maxRestarts; ++outer) {
       hipfoam_stats st{};
       // zero-mean projection of the inner RHS (metalFoamSolver.C:278-287)
@@ -702,15 +758,18 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
       if (deflate) {
         hipLaunchKernelGGL((k_delta_mean<float>), g, bl, 0, c->stream, n, A.xcdRemap,
                            c->w32.delta.p, c->ost.p, c->partials.p, c->pstride);
+        launch_fin<1>(c, g.x, false, 0, FinDeltaMean{c->ost.p, n});
       }
       hipLaunchKernelGGL((k_refine_residual<float>), g, bl, 0, c->stream, A, psiCur,
                          c->w32.delta.p, psiNxt, c->b.p, c->u64.p, c->r64.p, deflate ? 1 : 0,
                          c->ost.p, c->partials.p, c->pstride);
+      launch_fin<3>(c, g.x, false, 0, FinRefine{c->ost.p, n});
       std::swap(psiCur, psiNxt);
       // constant-mode correction (metalFoamSolver.C:342-361)
       if (deflate && uu > kVSmall) {
         hipLaunchKernelGGL(k_const_shift, g, bl, 0, c->stream, n, A.xcdRemap, psiCur, c->r64.p,
                            c->u64.p, c->ost.p, c->partials.p, c->pstride);
+        launch_fin<2>(c, g.x, false, 0, FinShift{c->ost.p, n});
       }
       CKL();
       rc = read_ostate(c);
@@ -854,7 +913,7 @@ int amul_impl(hipfoam_context* c, int precision, const void* x, void* y, bool de
   if (rc) return rc;
   if (!x || !y) return fail(c, HIPFOAM_ERR_ARG, "NULL x/y");
   const int n = c->n, bs = c->sched.blockSize;
-  const dim3 g(blocks_for(n, bs)), bl(bs);
+  const dim3 g(row_grid(c)), bl(bs);
   if (precision == HIPFOAM_FP32) {
     if (!c->have32) return fail(c, HIPFOAM_ERR_STATE, "no FP32 operator staged");
     const float* xd = static_cast<const float*>(x);
@@ -909,6 +968,89 @@ int set_matrix_impl(hipfoam_context* c, const double* diag, const double* upper,
   if (rc) return rc;
   CK(hipStreamSynchronize(c->stream));
   return HIPFOAM_OK;
+}
+
+// Adjoint of a solve (all pointers on the device): the transpose solve
+// A^T lambda = g runs through the same refined path with upper and lower
+// swapped on the same addressing (lduMatrix::Tmul semantics), then the face,
+// diagonal and source gradients are formed from x and lambda.
+int adjoint_device(hipfoam_context* c, int solver, const double* diag, const double* upper,
+                   const double* lower, const double* x, const double* g, double* lambda,
+                   double* dUpper, double* dLower, double* dDiag, double* dSource,
+                   const hipfoam_controls* ctl, hipfoam_performance* perf) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!x || !g || !lambda || !ctl) return fail(c, HIPFOAM_ERR_ARG, "NULL x/g/lambda/controls");
+  const int n = c->n, nf = c->nf;
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, 0.0, lambda);
+  CKL();
+  const double* upT = lower ? lower : upper;
+  const double* loT = lower ? upper : nullptr;
+  rc = refined_solve(c, solver, diag, upT, loT, lambda, g, ctl, perf, nullptr,
+                     hipMemcpyDeviceToDevice);
+  if (rc) return rc;
+  if (nf > 0 && (dUpper || dLower)) {
+    hipLaunchKernelGGL(k_adjoint_faces, dim3(blocks_for(nf, 256)), dim3(256), 0, c->stream, nf,
+                       c->lowAddr.p, c->uppAddr.p, x, lambda, lower ? 0 : 1, dUpper,
+                       lower ? dLower : nullptr);
+  }
+  if (n > 0 && (dDiag || dSource)) {
+    hipLaunchKernelGGL(k_adjoint_cells, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x,
+                       lambda, dDiag, dSource);
+  }
+  CKL();
+  CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+int adjoint_host(hipfoam_context* c, int solver, const double* diag, const double* upper,
+                 const double* lower, const double* x, const double* g, double* lambda,
+                 double* dUpper, double* dLower, double* dDiag, double* dSource,
+                 const hipfoam_controls* ctl, hipfoam_performance* perf) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!diag || !upper || !x || !g || !lambda || !ctl)
+    return fail(c, HIPFOAM_ERR_ARG, "NULL argument");
+  const int n = c->n, nf = c->nf;
+  const hipMemcpyKind h2d = hipMemcpyHostToDevice, d2h = hipMemcpyDeviceToHost;
+  CK(c->adjX.ensure(n));
+  CK(c->adjG.ensure(n));
+  CK(c->adjLam.ensure(n));
+  CK(c->adjDD.ensure(n));
+  CK(c->adjDB.ensure(n));
+  CK(c->adjDU.ensure(nf));
+  CK(c->adjDL.ensure(nf));
+  // matrix staged into the library's own buffers, then used as device input
+  rc = ensure_vectors(c);
+  if (rc) return rc;
+  CK(c->diag64.ensure(n));
+  CK(hipMemcpyAsync(c->adjX.p, x, sizeof(double) * n, h2d, c->stream));
+  CK(hipMemcpyAsync(c->adjG.p, g, sizeof(double) * n, h2d, c->stream));
+  // device copies of the coefficients (stage_matrix inside the solve copies again)
+  Buf<double> d, u, l;
+  CK(d.ensure(n));
+  CK(u.ensure(nf));
+  if (lower) CK(l.ensure(nf));
+  CK(hipMemcpyAsync(d.p, diag, sizeof(double) * n, h2d, c->stream));
+  CK(hipMemcpyAsync(u.p, upper, sizeof(double) * nf, h2d, c->stream));
+  if (lower) CK(hipMemcpyAsync(l.p, lower, sizeof(double) * nf, h2d, c->stream));
+  rc = adjoint_device(c, solver, d.p, u.p, lower ? l.p : nullptr, c->adjX.p, c->adjG.p,
+                      c->adjLam.p, dUpper ? c->adjDU.p : nullptr,
+                      (dLower && lower) ? c->adjDL.p : nullptr, dDiag ? c->adjDD.p : nullptr,
+                      dSource ? c->adjDB.p : nullptr, ctl, perf);
+  if (rc == HIPFOAM_OK) {
+    CK(hipMemcpyAsync(lambda, c->adjLam.p, sizeof(double) * n, d2h, c->stream));
+    if (dUpper) CK(hipMemcpyAsync(dUpper, c->adjDU.p, sizeof(double) * nf, d2h, c->stream));
+    if (dLower && lower)
+      CK(hipMemcpyAsync(dLower, c->adjDL.p, sizeof(double) * nf, d2h, c->stream));
+    if (dDiag) CK(hipMemcpyAsync(dDiag, c->adjDD.p, sizeof(double) * n, d2h, c->stream));
+    if (dSource) CK(hipMemcpyAsync(dSource, c->adjDB.p, sizeof(double) * n, d2h, c->stream));
+  }
+  CK(hipStreamSynchronize(c->stream));
+  d.release();
+  u.release();
+  l.release();
+  return rc;
 }
 
 }  // namespace
@@ -990,7 +1132,8 @@ void hipfoam_destroy(hipfoam_context* c) {
   for (auto* b : ib) b->release();
   Buf<double>* db[] = {&c->diag64, &c->upper64, &c->lower64, &c->lowerL64, &c->rD64,
                        &c->psi0, &c->psi1, &c->b, &c->r64, &c->u64, &c->scratch64,
-                       &c->partials};
+                       &c->partials, &c->adjX, &c->adjG, &c->adjLam, &c->adjDU,
+                       &c->adjDL, &c->adjDD, &c->adjDB};
   for (auto* b : db) b->release();
   Buf<float>* fb[] = {&c->diag32, &c->upper32, &c->lowerL32, &c->lower32, &c->rD32};
   for (auto* b : fb) b->release();
@@ -1172,6 +1315,26 @@ int32_t hipfoam_solve_fp64_device(hipfoam_context* c, int32_t solver, const doub
                     hipMemcpyDeviceToDevice);
 }
 
+int32_t hipfoam_adjoint(hipfoam_context* c, int32_t solver, const double* diag,
+                        const double* upper, const double* lower, const double* x,
+                        const double* g, double* lambda, double* dUpper, double* dLower,
+                        double* dDiag, double* dSource, const hipfoam_controls* controls,
+                        hipfoam_performance* perf) {
+  FpTrapGuard fpe;
+  return adjoint_host(c, solver, diag, upper, lower, x, g, lambda, dUpper, dLower, dDiag,
+                      dSource, controls, perf);
+}
+
+int32_t hipfoam_adjoint_device(hipfoam_context* c, int32_t solver, const double* diag,
+                               const double* upper, const double* lower, const double* x,
+                               const double* g, double* lambda, double* dUpper,
+                               double* dLower, double* dDiag, double* dSource,
+                               const hipfoam_controls* controls, hipfoam_performance* perf) {
+  FpTrapGuard fpe;
+  return adjoint_device(c, solver, diag, upper, lower, x, g, lambda, dUpper, dLower, dDiag,
+                        dSource, controls, perf);
+}
+
 int32_t hipfoam_amul(hipfoam_context* c, int32_t precision, const void* x, void* y) {
   FpTrapGuard fpe;
   return amul_impl(c, precision, x, y, false);
@@ -1251,6 +1414,7 @@ int32_t hipfoam_set_schedule(hipfoam_context* c, const hipfoam_schedule* s) {
     return fail(c, HIPFOAM_ERR_ARG, "blockSize must be a multiple of 64 in [64, 1024]");
   if (s->graphChunk < 0 || s->graphChunk > 4096 || (s->graphChunk & 1))
     return fail(c, HIPFOAM_ERR_ARG, "graphChunk must be even in [0, 4096]");
+  if (s->gridLimit < 0) return fail(c, HIPFOAM_ERR_ARG, "gridLimit must be >= 0");
   if (c->good) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
@@ -1276,14 +1440,17 @@ int32_t hipfoam_autotune(hipfoam_context* c, hipfoam_schedule* best) {
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
   const int reps = 20;
+  const int cus = std::max(c->numCUs, 1);
   for (int sch = 0; sch <= 1; ++sch)
     for (int bs : {128, 256, 512, 1024})
-      for (int rm = 0; rm <= 1; ++rm) {
+      for (int rm = 0; rm <= 1; ++rm)
+        for (int gl : {0, 4 * cus, 16 * cus, 1 << 30}) {
         c->sched.spmvSchedule = sch;
         c->sched.blockSize = bs;
         c->sched.xcdRemap = rm;
+        c->sched.gridLimit = gl;
         const LduOp<float> A = op32(c);
-        const dim3 g(blocks_for(n, bs)), bl(bs);
+        const dim3 g(row_grid(c)), bl(bs);
         hipLaunchKernelGGL((k_spmv<float>), g, bl, 0, c->stream, A, c->w32.z0.p, c->w32.z1.p);
         CK(hipEventRecord(e0, c->stream));
         for (int k = 0; k < reps; ++k)
@@ -1315,6 +1482,7 @@ int32_t hipfoam_set_kernel_timing(hipfoam_context* c, int32_t enable, int32_t kc
   KernelTimer& t = c->timer;
   if (t.ev.empty()) {
     t.ev.resize(4096);
+    t.count.assign(2048, 0);
     for (auto& e : t.ev) CK(hipEventCreate(&e));
   }
   t.on = enable != 0;

@@ -12,11 +12,19 @@
 // The "direct" schedule reads lower[losort[k]] and lowAddr[losort[k]]
 // through losort instead of the staged copies.
 //
-// One thread per row, 64-wide wavefronts, 256-thread workgroups by default.
+// Launch shape: one thread per row, 64-wide wavefronts, blockDim rows per
+// tile.  A grid of G workgroups covers the ceil(n/blockDim) tiles; logical
+// workgroup b owns the contiguous tile range [b*T, (b+1)*T), and the logical
+// index is remapped so that each XCD (hardware round-robin over 8) owns one
+// contiguous row range, keeping neighbour gathers inside its L2.
+//
 // Reductions: per-thread values widened to double, wavefront butterfly
-// (ds_swizzle/DPP via __shfl_xor), LDS across the block, one double per block
-// into `partials`; the last block to arrive (device-scope counter) folds the
-// partials in a fixed order and finalises the Krylov scalars on the device.
+// (__shfl_xor), LDS across the block, one double per workgroup into
+// `partials`; a one-workgroup finalise kernel folds them in a fixed order --
+// so results are deterministic -- and updates the Krylov scalars on the
+// device.  (Electing the last workgroup with device-scope atomics instead
+// costs ~13 ns per arriving workgroup, serialised: 27 us at 2048 workgroups,
+// against ~4.5 us for the extra kernel.)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,26 +54,24 @@ enum : int32_t {
 // (metalFoamKernels.metal:20-32) widened with the host loop's bookkeeping
 // (metalFoamBackend.C:599-715) so the whole loop runs without host syncs.
 struct alignas(128) KState {
-  double rho, rhoOld, wApA, alpha, beta, res;     // PCG (rho = wA.rA)
+  double rho, rhoOld, wApA, alpha, beta, res;         // PCG (rho = wA.rA)
   double rA0rA, rA0rAold, rA0AyA, omega, tAtA, tAsA;  // BiCGStab
-  double bestRes, target;                          // inner mode
-  double normFactor, initRes, tol, relTol;         // OpenFOAM mode
+  double bestRes, target;                             // inner mode
+  double normFactor, initRes, tol, relTol;            // OpenFOAM mode
   double chebC1, chebC2[kMaxCheb], chebC3[kMaxCheb];
   int32_t it, maxIter, minIter, ips, batchSize, batchLeft, noProgress, status;
   int32_t copyBest, mode, pad0, pad1;
-  uint32_t counter[8];
 };
 
 // Outer (FP64 refinement) scalars.
 struct alignas(128) OState {
-  double sumPsi, xbar, nPinned, uu;        // rowsum pass
+  double sumPsi, xbar, nPinned, uu;  // rowsum pass
   double normFactor, sumMagR, sumR, initRes;
-  double rmean;                             // mean of the current residual
-  double dmean;                             // mean of the inner correction
+  double rmean;  // mean of the current residual
+  double dmean;  // mean of the inner correction
   double ur, c, finalRes;
   double bail, betaDefl, betaExact;
   int32_t n, deflate, pad0, pad1;
-  uint32_t counter[8];
 };
 
 template <typename T>
@@ -73,11 +79,11 @@ struct LduOp {
   const int32_t* __restrict__ ownStart;
   const int32_t* __restrict__ losortStart;
   const int32_t* __restrict__ uppAddr;
-  const int32_t* __restrict__ nbrAddr;   // losortLow (staged) or lowAddr (direct)
-  const int32_t* __restrict__ losort;    // direct schedule only
+  const int32_t* __restrict__ nbrAddr;  // losortLow (staged) or lowAddr (direct)
+  const int32_t* __restrict__ losort;   // direct schedule only
   const T* __restrict__ diag;
   const T* __restrict__ upper;
-  const T* __restrict__ lowerL;          // staged: lower[losort[k]]; direct: lower[f]
+  const T* __restrict__ lowerL;  // staged: lower[losort[k]]; direct: lower[f]
   int32_t n;
   int32_t direct;
   int32_t xcdRemap;
@@ -88,7 +94,7 @@ struct LduOp {
 // ---------------------------------------------------------------------------
 
 // Workgroups are dispatched round-robin over the 8 XCDs; remapping gives each
-// XCD one contiguous range of rows so neighbour gathers stay in its L2.
+// XCD one contiguous range of logical indices.
 __device__ __forceinline__ int logical_block(int xcdRemap) {
   const int b = blockIdx.x;
   if (!xcdRemap) return b;
@@ -96,6 +102,17 @@ __device__ __forceinline__ int logical_block(int xcdRemap) {
   const int xcd = b & 7, local = b >> 3;
   const int q = G >> 3, r = G & 7;
   return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+// Rows [r0, r1) owned by logical workgroup lb: a contiguous run of tiles.
+__device__ __forceinline__ void block_rows(int n, int lb, int& r0, int& r1) {
+  const int bs = blockDim.x;
+  const int ntiles = (n + bs - 1) / bs;
+  const int per = (ntiles + gridDim.x - 1) / gridDim.x;
+  const long t0 = (long)lb * per;
+  const long a = t0 * bs, e = (t0 + per) * bs;
+  r0 = (int)(a < n ? a : n);
+  r1 = (int)(e < n ? e : n);
 }
 
 // ---------------------------------------------------------------------------
@@ -114,11 +131,12 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Block-reduce NV sums (and optionally NV-th as a max) and write one value per
-// block into partials[j*pstride + blk].
+// Block-reduce NV values (the last one as a max when LASTMAX) and write one
+// value per workgroup into partials[j*pstride + lb] (plain stores: the fold
+// runs in the next kernel, after the kernel boundary has made them visible).
 template <int NV, bool LASTMAX = false>
-__device__ __forceinline__ void block_partials(double (&v)[NV], double* partials,
-                                               int pstride, int blk) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* partials, int pstride,
+                                               int lb) {
   __shared__ double sh[NV][16];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;
@@ -135,49 +153,34 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* partials
       const bool mx = LASTMAX && j == NV - 1;
       double t = sh[j][0];
       for (int w = 1; w < nw; ++w) t = mx ? fmax(t, sh[j][w]) : t + sh[j][w];
-      // agent-scope store: written through to the point of coherence shared
-      // by all XCDs, so no L2 writeback is needed before signalling
-      __hip_atomic_store(&partials[j * pstride + blk], t, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      partials[j * pstride + lb] = t;
     }
   }
-}
-
-// True in every thread of the last block to finish; resets the counter.
-// Thread 0 wrote this block's partials with agent-scope stores; it waits for
-// them to complete, then bumps the agent-scope counter.  The last block reads
-// the partials with agent-scope loads (fold_partials).  No cache-wide
-// writeback or invalidate is involved (a __threadfence per block would flush
-// the XCD's L2 once per block).
-__device__ __forceinline__ bool last_block(uint32_t* counter) {
-  __shared__ int amLast;
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_s_waitcnt(0);  // partial stores acknowledged
-    const uint32_t prev =
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    amLast = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (amLast && threadIdx.x == 0)
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return amLast != 0;
 }
 
 // Fold partials[j*pstride + 0 .. nblk) in a fixed order (deterministic).
 template <int NV, bool LASTMAX = false>
-__device__ __forceinline__ void fold_partials(const double* partials, int pstride,
-                                              int nblk, double (&out)[NV]) {
+__device__ __forceinline__ void fold_partials(const double* partials, int pstride, int nblk,
+                                              double (&out)[NV]) {
   __shared__ double sh[NV][16];
   double v[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) v[j] = (LASTMAX && j == NV - 1) ? -1.0e308 : 0.0;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+  constexpr int U = 8;  // independent loads in flight per thread
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * blockDim.x) {
+    double t[U][NV];
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      const double p =
-          __hip_atomic_load(&partials[j * pstride + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      v[j] = (LASTMAX && j == NV - 1) ? fmax(v[j], p) : v[j] + p;
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * blockDim.x;
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+        t[u][j] = b < nblk ? partials[j * pstride + b] : ((LASTMAX && j == NV - 1) ? -1.0e308 : 0.0);
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+        v[j] = (LASTMAX && j == NV - 1) ? fmax(v[j], t[u][j]) : v[j] + t[u][j];
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;
@@ -195,6 +198,20 @@ __device__ __forceinline__ void fold_partials(const double* partials, int pstrid
     for (int w = 1; w < nw; ++w) t = mx ? fmax(t, sh[j][w]) : t + sh[j][w];
     out[j] = t;
   }
+}
+
+// Second stage of every reduction: one workgroup folds the partials of the
+// previous kernel and thread 0 applies the finaliser f (the device-side
+// scalar update, reduce_scalars of metalFoamKernels.metal:298-367).  It runs
+// only when the Krylov status is in runMask (status == NULL: always).
+template <int NV, bool LASTMAX, typename F>
+__global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ partials, int pstride,
+                                                  int nblk, const int32_t* status, int runMask,
+                                                  F f) {
+  if (status && !((runMask >> *status) & 1)) return;
+  double t[NV];
+  fold_partials<NV, LASTMAX>(partials, pstride, nblk, t);
+  if (threadIdx.x == 0) f(t);
 }
 
 // ---------------------------------------------------------------------------
@@ -220,7 +237,7 @@ __device__ __forceinline__ T ldu_row(const LduOp<T>& A, int i, T xi, XF xf) {
 }
 
 // ---------------------------------------------------------------------------
-// scalar finalisation (runs in thread 0 of the last block)
+// scalar finalisation (runs in thread 0 of the last workgroup)
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ bool of_converged(const KState* s, double res) {
@@ -271,6 +288,21 @@ __device__ __forceinline__ void openfoam_iteration_end(KState* s) {
   if (!cont) s->status = conv ? ST_CONVERGED : ST_MAXITER;
 }
 
+// rho = wA.rA and beta (reduce_scalars mode 0, metalFoamKernels.metal:336-340)
+template <typename T>
+__device__ __forceinline__ void finalize_beta(KState* st, double t) {
+  const bool first = (st->it == 0);
+  st->rhoOld = st->rho;
+  if (st->mode == MODE_INNER) {
+    const T rho = (T)t;
+    st->rho = rho;
+    st->beta = first ? 0.0 : (double)(rho / (T)st->rhoOld);
+  } else {
+    st->rho = t;
+    st->beta = first ? 0.0 : t / st->rhoOld;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // PCG kernels
 // ---------------------------------------------------------------------------
@@ -279,8 +311,7 @@ __device__ __forceinline__ void openfoam_iteration_end(KState* s) {
 // metalFoamKernels.metal:112-194):
 //   v = A z_{s-1};  w_s = c2 w_{s-1} + c3 rD (r - v);  z_s = z_{s-1} + w_s
 // FIRST: z_0 = w_0 = c1 rD r is formed on the fly from gathered rD and r.
-// LAST:  z_s is the preconditioned residual wA; fuses dot(wA, rA) and the
-//        beta finalisation (reduce_scalars mode 0, metalFoamKernels.metal:336-340).
+// LAST:  z_s is the preconditioned residual wA; fuses dot(wA, rA) and beta.
 template <typename T, bool FIRST, bool LAST>
 __global__ __launch_bounds__(1024) void k_cheb_step(LduOp<T> A, const T* __restrict__ r,
                                                     const T* __restrict__ rD,
@@ -289,15 +320,16 @@ __global__ __launch_bounds__(1024) void k_cheb_step(LduOp<T> A, const T* __restr
                                                     KState* st, double* partials, int pstride,
                                                     int step) {
   if (st->status != ST_RUNNING) return;
-  const int blk = logical_block(A.xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(A.xcdRemap);
+  int r0, r1;
+  block_rows(A.n, lb, r0, r1);
   const T c2 = (T)st->chebC2[step - 1];
   const T c3 = (T)st->chebC3[step - 1];
+  const T c1 = (T)st->chebC1;
   double dot = 0.0;
-  if (i < A.n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     T zi_new, wi;
     if (FIRST) {
-      const T c1 = (T)st->chebC1;
       const T z0 = c1 * rD[i] * r[i];
       const T v = ldu_row(A, i, z0, [&](int j) { return c1 * rD[j] * r[j]; });
       wi = c2 * z0 + c3 * rD[i] * (r[i] - v);
@@ -310,27 +342,11 @@ __global__ __launch_bounds__(1024) void k_cheb_step(LduOp<T> A, const T* __restr
     }
     if (!LAST) w[i] = wi;
     zout[i] = zi_new;
-    if (LAST) dot = (double)(zi_new * r[i]);
+    if (LAST) dot += (double)(zi_new * r[i]);
   }
   if (LAST) {
     double v[1] = {dot};
-    block_partials<1>(v, partials, pstride, blk);
-    if (last_block(&st->counter[0])) {
-      double t[1];
-      fold_partials<1>(partials, pstride, gridDim.x, t);
-      if (threadIdx.x == 0) {
-        const bool first = (st->it == 0);
-        st->rhoOld = st->rho;
-        if (st->mode == MODE_INNER) {
-          const T rho = (T)t[0];
-          st->rho = rho;
-          st->beta = first ? 0.0 : (double)(rho / (T)st->rhoOld);
-        } else {
-          st->rho = t[0];
-          st->beta = first ? 0.0 : t[0] / st->rhoOld;
-        }
-      }
-    }
+    block_partials<1>(v, partials, pstride, lb);
   }
 }
 
@@ -341,32 +357,17 @@ __global__ __launch_bounds__(1024) void k_jacobi_dot(int n, int xcdRemap, const 
                                                      T* __restrict__ wA, KState* st,
                                                      double* partials, int pstride) {
   if (st->status != ST_RUNNING) return;
-  const int blk = logical_block(xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(xcdRemap);
+  int r0, r1;
+  block_rows(n, lb, r0, r1);
   double dot = 0.0;
-  if (i < n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const T wv = r[i] * rD[i];
     wA[i] = wv;
-    dot = (double)(wv * r[i]);
+    dot += (double)(wv * r[i]);
   }
   double v[1] = {dot};
-  block_partials<1>(v, partials, pstride, blk);
-  if (last_block(&st->counter[0])) {
-    double t[1];
-    fold_partials<1>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      const bool first = (st->it == 0);
-      st->rhoOld = st->rho;
-      if (st->mode == MODE_INNER) {
-        const T rho = (T)t[0];
-        st->rho = rho;
-        st->beta = first ? 0.0 : (double)(rho / (T)st->rhoOld);
-      } else {
-        st->rho = t[0];
-        st->beta = first ? 0.0 : t[0] / st->rhoOld;
-      }
-    }
-  }
+  block_partials<1>(v, partials, pstride, lb);
 }
 
 // p = wA + beta p fused into A p (pcg_update_p + ldu_spmv_dotw with w = p,
@@ -380,36 +381,20 @@ __global__ __launch_bounds__(1024) void k_pcg_p_spmv(LduOp<T> A, const T* __rest
                                                      KState* st, double* partials,
                                                      int pstride) {
   if (st->status != ST_RUNNING) return;
-  const int blk = logical_block(A.xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(A.xcdRemap);
+  int r0, r1;
+  block_rows(A.n, lb, r0, r1);
   const T beta = (T)st->beta;
   double dot = 0.0;
-  if (i < A.n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const T pi = wA[i] + beta * pold[i];
     const T y = ldu_row(A, i, pi, [&](int j) { return wA[j] + beta * pold[j]; });
     pnew[i] = pi;
     Ap[i] = y;
-    dot = (double)(pi * y);
+    dot += (double)(pi * y);
   }
   double v[1] = {dot};
-  block_partials<1>(v, partials, pstride, blk);
-  if (last_block(&st->counter[1])) {
-    double t[1];
-    fold_partials<1>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      if (st->mode == MODE_INNER) {
-        const T wApA = (T)t[0];
-        st->wApA = wApA;
-        st->alpha = (double)((T)st->rho / wApA);
-      } else {
-        st->wApA = t[0];
-        if (fabs(t[0]) / st->normFactor < kVSmall)
-          st->status = ST_SINGULAR;
-        else
-          st->alpha = st->rho / t[0];
-      }
-    }
-  }
+  block_partials<1>(v, partials, pstride, lb);
 }
 
 // delta += alpha p; r -= alpha Ap; partial sum|r| (pcg_update_sol_res,
@@ -422,34 +407,22 @@ __global__ __launch_bounds__(1024) void k_pcg_update(int n, int xcdRemap, T* __r
                                                      const T* __restrict__ Ap, KState* st,
                                                      double* partials, int pstride) {
   if (st->status != ST_RUNNING) return;
-  const int blk = logical_block(xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(xcdRemap);
+  int r0, r1;
+  block_rows(n, lb, r0, r1);
   const T alpha = (T)st->alpha;
   const bool copy = st->copyBest != 0;
   double m = 0.0;
-  if (i < n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const T d = delta[i];
     if (copy) deltaBest[i] = d;
     delta[i] = d + alpha * p[i];
     const T ri = r[i] - alpha * Ap[i];
     r[i] = ri;
-    m = (double)fabs(ri);
+    m += (double)fabs(ri);
   }
   double v[1] = {m};
-  block_partials<1>(v, partials, pstride, blk);
-  if (last_block(&st->counter[2])) {
-    double t[1];
-    fold_partials<1>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      if (st->mode == MODE_INNER) {
-        st->res = (double)(T)t[0];
-        inner_iteration_end(st, st->wApA);
-      } else {
-        st->res = t[0] / st->normFactor;
-        openfoam_iteration_end(st);
-      }
-    }
-  }
+  block_partials<1>(v, partials, pstride, lb);
 }
 
 // ---------------------------------------------------------------------------
@@ -462,37 +435,13 @@ __global__ __launch_bounds__(1024) void k_bicg_rho(int n, int xcdRemap, const T*
                                                    const T* __restrict__ r, KState* st,
                                                    double* partials, int pstride) {
   if (st->status != ST_RUNNING) return;
-  const int blk = logical_block(xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(xcdRemap);
+  int r0, r1;
+  block_rows(n, lb, r0, r1);
   double d = 0.0;
-  if (i < n) d = (double)(rA0[i] * r[i]);
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) d += (double)(rA0[i] * r[i]);
   double v[1] = {d};
-  block_partials<1>(v, partials, pstride, blk);
-  if (last_block(&st->counter[0])) {
-    double t[1];
-    fold_partials<1>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      const bool first = (st->it == 0);
-      st->rA0rAold = st->rA0rA;
-      if (st->mode == MODE_INNER) {
-        const T rr = (T)t[0];
-        st->rA0rA = rr;
-        st->beta = first ? 0.0
-                         : (double)((rr / (T)st->rA0rAold) * ((T)st->alpha / (T)st->omega));
-      } else {
-        st->rA0rA = t[0];
-        if (fabs(t[0]) < kVSmall) {
-          st->status = ST_SINGULAR;
-        } else if (first) {
-          st->beta = 0.0;
-        } else if (fabs(st->omega) < kVSmall) {
-          st->status = ST_SINGULAR;
-        } else {
-          st->beta = (t[0] / st->rA0rAold) * (st->alpha / st->omega);
-        }
-      }
-    }
-  }
+  block_partials<1>(v, partials, pstride, lb);
 }
 
 // p = rA + beta (p - omega AyA); yA = rD p   (bicg_update_p + emul)
@@ -503,10 +452,11 @@ __global__ __launch_bounds__(1024) void k_bicg_p_y(int n, int xcdRemap, T* __res
                                                    const T* __restrict__ rD, T* __restrict__ yA,
                                                    KState* st) {
   if (st->status != ST_RUNNING) return;
-  const int blk = logical_block(xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(xcdRemap);
+  int r0, r1;
+  block_rows(n, lb, r0, r1);
   const T beta = (T)st->beta, omega = (T)st->omega;
-  if (i < n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const T pi = r[i] + beta * (p[i] - omega * AyA[i]);
     p[i] = pi;
     yA[i] = rD[i] * pi;
@@ -520,30 +470,17 @@ __global__ __launch_bounds__(1024) void k_bicg_spmv_alpha(LduOp<T> A, const T* _
                                                           T* __restrict__ AyA, KState* st,
                                                           double* partials, int pstride) {
   if (st->status != ST_RUNNING) return;
-  const int blk = logical_block(A.xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(A.xcdRemap);
+  int r0, r1;
+  block_rows(A.n, lb, r0, r1);
   double d = 0.0;
-  if (i < A.n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const T y = ldu_row(A, i, yA[i], [&](int j) { return yA[j]; });
     AyA[i] = y;
-    d = (double)(rA0[i] * y);
+    d += (double)(rA0[i] * y);
   }
   double v[1] = {d};
-  block_partials<1>(v, partials, pstride, blk);
-  if (last_block(&st->counter[1])) {
-    double t[1];
-    fold_partials<1>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      if (st->mode == MODE_INNER) {
-        const T a = (T)t[0];
-        st->rA0AyA = a;
-        st->alpha = (double)((T)st->rA0rA / a);
-      } else {
-        st->rA0AyA = t[0];
-        st->alpha = st->rA0rA / t[0];
-      }
-    }
-  }
+  block_partials<1>(v, partials, pstride, lb);
 }
 
 // sA = rA - alpha AyA, partial |sA|; breakdown and early-exit tests
@@ -554,34 +491,18 @@ __global__ __launch_bounds__(1024) void k_bicg_s(int n, int xcdRemap, T* __restr
                                                  const T* __restrict__ AyA, KState* st,
                                                  double* partials, int pstride) {
   if (st->status != ST_RUNNING) return;
-  const int blk = logical_block(xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(xcdRemap);
+  int r0, r1;
+  block_rows(n, lb, r0, r1);
   const T alpha = (T)st->alpha;
   double m = 0.0;
-  if (i < n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const T s = r[i] - alpha * AyA[i];
     sA[i] = s;
-    m = (double)fabs(s);
+    m += (double)fabs(s);
   }
   double v[1] = {m};
-  block_partials<1>(v, partials, pstride, blk);
-  if (last_block(&st->counter[2])) {
-    double t[1];
-    fold_partials<1>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      if (st->mode == MODE_INNER) {
-        const double res = (double)(T)t[0];
-        st->res = res;
-        if (!isfinite(st->rA0rA) || !isfinite(st->alpha) || !isfinite(res))
-          st->status = ST_SINGULAR;
-        else if (res <= st->target)
-          st->status = ST_EARLY;
-      } else {
-        st->res = t[0] / st->normFactor;
-        if (of_converged(st, st->res)) st->status = ST_EARLY;
-      }
-    }
-  }
+  block_partials<1>(v, partials, pstride, lb);
 }
 
 // zA = rD sA (own row) and tA = A zA with zA gathered on the fly; partials
@@ -593,34 +514,19 @@ __global__ __launch_bounds__(1024) void k_bicg_z_spmv(LduOp<T> A, const T* __res
                                                       KState* st, double* partials,
                                                       int pstride) {
   if (st->status != ST_RUNNING) return;
-  const int blk = logical_block(A.xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(A.xcdRemap);
+  int r0, r1;
+  block_rows(A.n, lb, r0, r1);
   double d[2] = {0.0, 0.0};
-  if (i < A.n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const T zi = rD[i] * sA[i];
     const T t = ldu_row(A, i, zi, [&](int j) { return rD[j] * sA[j]; });
     zA[i] = zi;
     tA[i] = t;
-    d[0] = (double)(sA[i] * t);
-    d[1] = (double)(t * t);
+    d[0] += (double)(sA[i] * t);
+    d[1] += (double)(t * t);
   }
-  block_partials<2>(d, partials, pstride, blk);
-  if (last_block(&st->counter[3])) {
-    double t[2];
-    fold_partials<2>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      if (st->mode == MODE_INNER) {
-        const T tAsA = (T)t[0], tAtA = (T)t[1];
-        st->tAsA = tAsA;
-        st->tAtA = tAtA;
-        st->omega = (double)(tAsA / tAtA);
-      } else {
-        st->tAsA = t[0];
-        st->tAtA = t[1];
-        st->omega = t[0] / t[1];
-      }
-    }
-  }
+  block_partials<2>(d, partials, pstride, lb);
 }
 
 // delta += alpha yA + omega zA; rA = sA - omega tA; partial |rA|
@@ -637,45 +543,27 @@ __global__ __launch_bounds__(1024) void k_bicg_update(int n, int xcdRemap, T* __
                                                       double* partials, int pstride) {
   const int status = st->status;
   if (status != ST_RUNNING && status != ST_EARLY) return;
-  const int blk = logical_block(xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(xcdRemap);
+  int r0, r1;
+  block_rows(n, lb, r0, r1);
   const T alpha = (T)st->alpha;
   if (status == ST_EARLY) {
-    if (i < n) delta[i] += alpha * yA[i];
-    if (last_block(&st->counter[4])) {
-      if (threadIdx.x == 0) {
-        st->it += 1;
-        st->status = ST_CONVERGED;
-      }
-    }
-    return;
+    for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) delta[i] += alpha * yA[i];
+    return;  // FinBicgUpdate completes the |sA| exit
   }
   const T omega = (T)st->omega;
   const bool copy = st->copyBest != 0;
   double m = 0.0;
-  if (i < n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const T d = delta[i];
     if (copy) deltaBest[i] = d;
     delta[i] = d + (alpha * yA[i] + omega * zA[i]);
     const T ri = sA[i] - omega * tA[i];
     r[i] = ri;
-    m = (double)fabs(ri);
+    m += (double)fabs(ri);
   }
   double v[1] = {m};
-  block_partials<1>(v, partials, pstride, blk);
-  if (last_block(&st->counter[4])) {
-    double t[1];
-    fold_partials<1>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      if (st->mode == MODE_INNER) {
-        st->res = (double)(T)t[0];
-        inner_iteration_end(st, st->omega);
-      } else {
-        st->res = t[0] / st->normFactor;
-        openfoam_iteration_end(st);
-      }
-    }
-  }
+  block_partials<1>(v, partials, pstride, lb);
 }
 
 // ---------------------------------------------------------------------------
@@ -709,27 +597,18 @@ __global__ __launch_bounds__(1024) void k_inner_init(int n, const double* __rest
 __global__ __launch_bounds__(1024) void k_rowsum(LduOp<double> A, const double* __restrict__ psi,
                                                  double* __restrict__ u, OState* os,
                                                  double* partials, int pstride) {
-  const int blk = logical_block(A.xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(A.xcdRemap);
+  int r0, r1;
+  block_rows(A.n, lb, r0, r1);
   double v[3] = {0.0, 0.0, 0.0};
-  if (i < A.n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const double ui = ldu_row(A, i, 1.0, [](int) { return 1.0; });
     u[i] = ui;
-    v[0] = psi[i];
-    v[1] = (fabs(ui) > 1e-8 * fabs(A.diag[i])) ? 1.0 : 0.0;
-    v[2] = ui * ui;
+    v[0] += psi[i];
+    v[1] += (fabs(ui) > 1e-8 * fabs(A.diag[i])) ? 1.0 : 0.0;
+    v[2] += ui * ui;
   }
-  block_partials<3>(v, partials, pstride, blk);
-  if (last_block(&os->counter[0])) {
-    double t[3];
-    fold_partials<3>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      os->sumPsi = t[0];
-      os->xbar = t[0] / (double)A.n;
-      os->nPinned = t[1];
-      os->uu = t[2];
-    }
-  }
+  block_partials<3>(v, partials, pstride, lb);
 }
 
 // wA = A psi; r = b - wA; partials: sum|r|, normFactor sum, sum r.
@@ -740,32 +619,21 @@ __global__ __launch_bounds__(1024) void k_initial_residual(LduOp<double> A,
                                                            const double* __restrict__ u,
                                                            double* __restrict__ r, OState* os,
                                                            double* partials, int pstride) {
-  const int blk = logical_block(A.xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(A.xcdRemap);
+  int r0, r1;
+  block_rows(A.n, lb, r0, r1);
   const double xbar = os->xbar;
   double v[3] = {0.0, 0.0, 0.0};
-  if (i < A.n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const double wA = ldu_row(A, i, psi[i], [&](int j) { return psi[j]; });
     const double ri = b[i] - wA;
     r[i] = ri;
     const double t = u[i] * xbar;
-    v[0] = fabs(ri);
-    v[1] = fabs(wA - t) + fabs(b[i] - t);
-    v[2] = ri;
+    v[0] += fabs(ri);
+    v[1] += fabs(wA - t) + fabs(b[i] - t);
+    v[2] += ri;
   }
-  block_partials<3>(v, partials, pstride, blk);
-  if (last_block(&os->counter[1])) {
-    double t[3];
-    fold_partials<3>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      os->normFactor = t[1] + kSmall;
-      os->sumMagR = t[0];
-      os->initRes = t[0] / os->normFactor;
-      os->finalRes = os->initRes;
-      os->sumR = t[2];
-      os->rmean = t[2] / (double)A.n;
-    }
-  }
+  block_partials<3>(v, partials, pstride, lb);
 }
 
 // Inner operator of one restart: diag* = deflate ? diag - u : diag;
@@ -781,15 +649,16 @@ __global__ __launch_bounds__(1024) void k_prepare_inner(LduOp<double> A,
                                                         T* __restrict__ diagT,
                                                         T* __restrict__ rDT, OState* os,
                                                         double* partials, int pstride) {
-  const int blk = logical_block(A.xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(A.xcdRemap);
+  int r0, r1;
+  block_rows(A.n, lb, r0, r1);
   double v[2] = {0.0, -1.0e308};
-  if (i < A.n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const double d = A.diag[i];
     double ds = d;
     if (deflate) {
       ds = d - u[i];
-      if (ds * d <= kVSmall * fabs(ds)) v[0] = 1.0;
+      if (ds * d <= kVSmall * fabs(ds)) v[0] += 1.0;
     }
     diagT[i] = (T)ds;
     rDT[i] = (T)(1.0 / ds);
@@ -798,23 +667,10 @@ __global__ __launch_bounds__(1024) void k_prepare_inner(LduOp<double> A,
       for (int f = A.ownStart[i]; f < A.ownStart[i + 1]; ++f) so += fabs(upperF[f]);
       for (int k = A.losortStart[i]; k < A.losortStart[i + 1]; ++k)
         so += fabs(upperF[losort[k]]);
-      v[1] = 1.0 + so / fabs(ds);
+      v[1] = fmax(v[1], 1.0 + so / fabs(ds));
     }
   }
-  block_partials<2, true>(v, partials, pstride, blk);
-  if (last_block(&os->counter[2])) {
-    double t[2];
-    fold_partials<2, true>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      os->bail = t[0];
-      if (wantBeta) {
-        if (deflate)
-          os->betaDefl = t[1];
-        else
-          os->betaExact = t[1];
-      }
-    }
-  }
+  block_partials<2, true>(v, partials, pstride, lb);
 }
 
 // Face staging, once per matrix: lowerL64[k] = lower[losort[k]] (upper when
@@ -841,15 +697,12 @@ template <typename T>
 __global__ __launch_bounds__(1024) void k_delta_mean(int n, int xcdRemap,
                                                      const T* __restrict__ delta, OState* os,
                                                      double* partials, int pstride) {
-  const int blk = logical_block(xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
-  double v[1] = {i < n ? (double)delta[i] : 0.0};
-  block_partials<1>(v, partials, pstride, blk);
-  if (last_block(&os->counter[3])) {
-    double t[1];
-    fold_partials<1>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) os->dmean = t[0] / (double)n;
-  }
+  const int lb = logical_block(xcdRemap);
+  int r0, r1;
+  block_rows(n, lb, r0, r1);
+  double v[1] = {0.0};
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) v[0] += (double)delta[i];
+  block_partials<1>(v, partials, pstride, lb);
 }
 
 // psiNew = psiOld + (double(delta) - dmean); r = b - A psiNew with psiNew
@@ -859,34 +712,23 @@ __global__ __launch_bounds__(1024) void k_refine_residual(
     LduOp<double> A, const double* __restrict__ psiOld, const T* __restrict__ delta,
     double* __restrict__ psiNew, const double* __restrict__ b, const double* __restrict__ u,
     double* __restrict__ r, int useMean, OState* os, double* partials, int pstride) {
-  const int blk = logical_block(A.xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(A.xcdRemap);
+  int r0, r1;
+  block_rows(A.n, lb, r0, r1);
   const double dm = useMean ? os->dmean : 0.0;
   double v[3] = {0.0, 0.0, 0.0};
-  if (i < A.n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     auto pf = [&](int j) { return psiOld[j] + ((double)delta[j] - dm); };
     const double pi = pf(i);
     const double wA = ldu_row(A, i, pi, pf);
     psiNew[i] = pi;
     const double ri = b[i] - wA;
     r[i] = ri;
-    v[0] = u[i] * ri;
-    v[1] = fabs(ri);
-    v[2] = ri;
+    v[0] += u[i] * ri;
+    v[1] += fabs(ri);
+    v[2] += ri;
   }
-  block_partials<3>(v, partials, pstride, blk);
-  if (last_block(&os->counter[4])) {
-    double t[3];
-    fold_partials<3>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      os->ur = t[0];
-      os->sumMagR = t[1];
-      os->finalRes = t[1] / os->normFactor;
-      os->sumR = t[2];
-      os->rmean = t[2] / (double)A.n;
-      os->c = (os->uu > kVSmall) ? t[0] / os->uu : 0.0;
-    }
-  }
+  block_partials<3>(v, partials, pstride, lb);
 }
 
 // Constant-mode correction psi += c, r -= c u (metalFoamSolver.C:342-361).
@@ -894,47 +736,33 @@ __global__ __launch_bounds__(1024) void k_const_shift(int n, int xcdRemap, doubl
                                                       double* __restrict__ r,
                                                       const double* __restrict__ u, OState* os,
                                                       double* partials, int pstride) {
-  const int blk = logical_block(xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int lb = logical_block(xcdRemap);
+  int r0, r1;
+  block_rows(n, lb, r0, r1);
   const double c = os->c;
   double v[2] = {0.0, 0.0};
-  if (i < n) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     psi[i] += c;
     const double ri = r[i] - c * u[i];
     r[i] = ri;
-    v[0] = fabs(ri);
-    v[1] = ri;
+    v[0] += fabs(ri);
+    v[1] += ri;
   }
-  block_partials<2>(v, partials, pstride, blk);
-  if (last_block(&os->counter[5])) {
-    double t[2];
-    fold_partials<2>(partials, pstride, gridDim.x, t);
-    if (threadIdx.x == 0) {
-      os->sumMagR = t[0];
-      os->finalRes = t[0] / os->normFactor;
-      os->sumR = t[1];
-      os->rmean = t[1] / (double)n;
-    }
-  }
+  block_partials<2>(v, partials, pstride, lb);
 }
 
 // Plain y = A x.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_spmv(LduOp<T> A, const T* __restrict__ x,
                                                T* __restrict__ y) {
-  const int blk = logical_block(A.xcdRemap);
-  const int i = blk * blockDim.x + threadIdx.x;
-  if (i < A.n) y[i] = ldu_row(A, i, x[i], [&](int j) { return x[j]; });
+  const int lb = logical_block(A.xcdRemap);
+  int r0, r1;
+  block_rows(A.n, lb, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x)
+    y[i] = ldu_row(A, i, x[i], [&](int j) { return x[j]; });
 }
 
-template <typename S, typename D>
-__global__ __launch_bounds__(1024) void k_convert(int n, const S* __restrict__ s,
-                                                  D* __restrict__ d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) d[i] = (D)s[i];
-}
-
-// lowerL64 = lower[losort] for the staged FP64 operator, reciprocal diag.
+// reciprocal diagonal
 __global__ __launch_bounds__(1024) void k_recip(int n, const double* __restrict__ d,
                                                 double* __restrict__ rD) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -954,5 +782,262 @@ __global__ __launch_bounds__(1024) void k_copy(int n, const T* __restrict__ s, T
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = s[i];
 }
+
+// ---------------------------------------------------------------------------
+// adjoint gradients (no reference counterpart)
+// ---------------------------------------------------------------------------
+
+// For A x = b and a loss L(x) with A^T lambda = dL/dx:
+//   dL/dupper_f = -lambda[l] x[u]   (upper_f = A[l,u])
+//   dL/dlower_f = -lambda[u] x[l]   (lower_f = A[u,l])
+// A symmetric matrix stores one coefficient per face (upper == lower), whose
+// gradient is the sum of both terms.
+__global__ __launch_bounds__(256) void k_adjoint_faces(int nf, const int32_t* __restrict__ low,
+                                                       const int32_t* __restrict__ upp,
+                                                       const double* __restrict__ x,
+                                                       const double* __restrict__ lam,
+                                                       int symmetric, double* __restrict__ dUpper,
+                                                       double* __restrict__ dLower) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int l = low[f], u = upp[f];
+  const double gu = -lam[l] * x[u];
+  const double gl = -lam[u] * x[l];
+  if (symmetric) {
+    if (dUpper) dUpper[f] = gu + gl;
+  } else {
+    if (dUpper) dUpper[f] = gu;
+    if (dLower) dLower[f] = gl;
+  }
+}
+
+// dL/ddiag_i = -lambda_i x_i, dL/db_i = lambda_i
+__global__ __launch_bounds__(256) void k_adjoint_cells(int n, const double* __restrict__ x,
+                                                       const double* __restrict__ lam,
+                                                       double* __restrict__ dDiag,
+                                                       double* __restrict__ dSource) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (dDiag) dDiag[i] = -lam[i] * x[i];
+  if (dSource) dSource[i] = lam[i];
+}
+
+__global__ __launch_bounds__(256) void k_fill(int n, double v, double* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// finalisers (thread 0 of k_finalize; t = folded totals)
+// ---------------------------------------------------------------------------
+
+// rho = wA.rA, beta (reduce_scalars mode 0)
+template <typename T>
+struct FinBeta {
+  KState* st;
+  __device__ void operator()(const double* t) const { finalize_beta<T>(st, t[0]); }
+};
+
+// wApA, alpha (reduce_scalars mode 1); OpenFOAM mode: checkSingularity(|wApA|/normFactor)
+template <typename T>
+struct FinAlpha {
+  KState* st;
+  __device__ void operator()(const double* t) const {
+    if (st->mode == MODE_INNER) {
+      const T wApA = (T)t[0];
+      st->wApA = wApA;
+      st->alpha = (double)((T)st->rho / wApA);
+    } else {
+      st->wApA = t[0];
+      if (fabs(t[0]) / st->normFactor < kVSmall)
+        st->status = ST_SINGULAR;
+      else
+        st->alpha = st->rho / t[0];
+    }
+  }
+};
+
+// sum|r| and the end-of-iteration checks
+template <typename T>
+struct FinPcgRes {
+  KState* st;
+  __device__ void operator()(const double* t) const {
+    if (st->mode == MODE_INNER) {
+      st->res = (double)(T)t[0];
+      inner_iteration_end(st, st->wApA);
+    } else {
+      st->res = t[0] / st->normFactor;
+      openfoam_iteration_end(st);
+    }
+  }
+};
+
+// rA0.rA, beta (reduce_scalars mode 3); OpenFOAM mode: singularity on rho and omega
+template <typename T>
+struct FinBicgRho {
+  KState* st;
+  __device__ void operator()(const double* t) const {
+    const bool first = (st->it == 0);
+    st->rA0rAold = st->rA0rA;
+    if (st->mode == MODE_INNER) {
+      const T rr = (T)t[0];
+      st->rA0rA = rr;
+      st->beta =
+          first ? 0.0 : (double)((rr / (T)st->rA0rAold) * ((T)st->alpha / (T)st->omega));
+    } else {
+      st->rA0rA = t[0];
+      if (fabs(t[0]) < kVSmall) {
+        st->status = ST_SINGULAR;
+      } else if (first) {
+        st->beta = 0.0;
+      } else if (fabs(st->omega) < kVSmall) {
+        st->status = ST_SINGULAR;
+      } else {
+        st->beta = (t[0] / st->rA0rAold) * (st->alpha / st->omega);
+      }
+    }
+  }
+};
+
+// rA0.AyA, alpha (reduce_scalars mode 4)
+template <typename T>
+struct FinBicgAlpha {
+  KState* st;
+  __device__ void operator()(const double* t) const {
+    if (st->mode == MODE_INNER) {
+      const T a = (T)t[0];
+      st->rA0AyA = a;
+      st->alpha = (double)((T)st->rA0rA / a);
+    } else {
+      st->rA0AyA = t[0];
+      st->alpha = st->rA0rA / t[0];
+    }
+  }
+};
+
+// sum|sA|: breakdown checks and the early-exit test (metalFoamBackend.C:828-866)
+template <typename T>
+struct FinBicgS {
+  KState* st;
+  __device__ void operator()(const double* t) const {
+    if (st->mode == MODE_INNER) {
+      const double res = (double)(T)t[0];
+      st->res = res;
+      if (!isfinite(st->rA0rA) || !isfinite(st->alpha) || !isfinite(res))
+        st->status = ST_SINGULAR;
+      else if (res <= st->target)
+        st->status = ST_EARLY;
+    } else {
+      st->res = t[0] / st->normFactor;
+      if (of_converged(st, st->res)) st->status = ST_EARLY;
+    }
+  }
+};
+
+// tA.sA, tA.tA, omega (reduce_scalars mode 5)
+template <typename T>
+struct FinBicgOmega {
+  KState* st;
+  __device__ void operator()(const double* t) const {
+    if (st->mode == MODE_INNER) {
+      const T tAsA = (T)t[0], tAtA = (T)t[1];
+      st->tAsA = tAsA;
+      st->tAtA = tAtA;
+      st->omega = (double)(tAsA / tAtA);
+    } else {
+      st->tAsA = t[0];
+      st->tAtA = t[1];
+      st->omega = t[0] / t[1];
+    }
+  }
+};
+
+// sum|rA| and the end-of-iteration checks, or completion of an |sA| exit
+template <typename T>
+struct FinBicgUpdate {
+  KState* st;
+  __device__ void operator()(const double* t) const {
+    if (st->status == ST_EARLY) {
+      st->it += 1;
+      st->status = ST_CONVERGED;
+      return;
+    }
+    if (st->mode == MODE_INNER) {
+      st->res = (double)(T)t[0];
+      inner_iteration_end(st, st->omega);
+    } else {
+      st->res = t[0] / st->normFactor;
+      openfoam_iteration_end(st);
+    }
+  }
+};
+
+struct FinRowsum {
+  OState* os;
+  int n;
+  __device__ void operator()(const double* t) const {
+    os->sumPsi = t[0];
+    os->xbar = t[0] / (double)n;
+    os->nPinned = t[1];
+    os->uu = t[2];
+  }
+};
+
+struct FinInitialResidual {
+  OState* os;
+  int n;
+  __device__ void operator()(const double* t) const {
+    os->normFactor = t[1] + kSmall;
+    os->sumMagR = t[0];
+    os->initRes = t[0] / os->normFactor;
+    os->finalRes = os->initRes;
+    os->sumR = t[2];
+    os->rmean = t[2] / (double)n;
+  }
+};
+
+struct FinPrepare {
+  OState* os;
+  int deflate, wantBeta;
+  __device__ void operator()(const double* t) const {
+    os->bail = t[0];
+    if (wantBeta) {
+      if (deflate)
+        os->betaDefl = t[1];
+      else
+        os->betaExact = t[1];
+    }
+  }
+};
+
+struct FinDeltaMean {
+  OState* os;
+  int n;
+  __device__ void operator()(const double* t) const { os->dmean = t[0] / (double)n; }
+};
+
+struct FinRefine {
+  OState* os;
+  int n;
+  __device__ void operator()(const double* t) const {
+    os->ur = t[0];
+    os->sumMagR = t[1];
+    os->finalRes = t[1] / os->normFactor;
+    os->sumR = t[2];
+    os->rmean = t[2] / (double)n;
+    os->c = (os->uu > kVSmall) ? t[0] / os->uu : 0.0;
+  }
+};
+
+struct FinShift {
+  OState* os;
+  int n;
+  __device__ void operator()(const double* t) const {
+    os->sumMagR = t[0];
+    os->finalRes = t[0] / os->normFactor;
+    os->sumR = t[1];
+    os->rmean = t[1] / (double)n;
+  }
+};
 
 }  // namespace hipfoam

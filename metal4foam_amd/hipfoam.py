@@ -217,6 +217,26 @@ class Context:
                        None if residual is None else residual.ptr), "solve_device")
         return perf
 
+    def adjoint(self, solver, diag, upper, lower, x, g, controls=None):
+        """Transpose solve A^T lambda = g and the gradients of L through A x = b.
+
+        Returns (perf, lambda, dUpper, dLower or None, dDiag, dSource)."""
+        controls = controls or default_controls()
+        d, u, lo = _f64(diag), _f64(upper), _f64(lower)
+        x, g = _f64(x), _f64(g)
+        nf = u.shape[0]
+        lam = np.empty(self.n)
+        du = np.empty(nf)
+        dl = np.empty(nf) if lo is not None else None
+        dd = np.empty(self.n)
+        db = np.empty(self.n)
+        perf = Performance()
+        self._check(self.lib.hipfoam_adjoint(self.h, solver, _ptr(d), _ptr(u), _ptr(lo),
+                                             _ptr(x), _ptr(g), _ptr(lam), _ptr(du), _ptr(dl),
+                                             _ptr(dd), _ptr(db), C.byref(controls),
+                                             C.byref(perf)), "adjoint")
+        return perf, lam, du, dl, dd, db
+
     # -- primitives -------------------------------------------------------
     def amul(self, x, precision=FP64):
         dt = np.float64 if precision == FP64 else np.float32

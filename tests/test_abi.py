@@ -62,11 +62,11 @@ def test_schedule_validation_without_device():
     ctx = H.Context(device=999, require=False)
     s = ctx.get_schedule()
     assert (s.spmvSchedule, s.blockSize, s.xcdRemap) == (0, 256, 1)
-    bad = _lib.Schedule(0, 100, 1, 8)
+    bad = _lib.Schedule(0, 100, 1, 8, 0)
     assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(bad)) == _lib.ERR_ARG
-    bad = _lib.Schedule(0, 256, 1, 7)
+    bad = _lib.Schedule(0, 256, 1, 7, 0)
     assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(bad)) == _lib.ERR_ARG
-    ok = _lib.Schedule(1, 512, 0, 16)
+    ok = _lib.Schedule(1, 512, 0, 16, 1024)
     assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(ok)) == _lib.OK
     assert ctx.get_schedule().as_dict() == ok.as_dict()
     ctx.close()

@@ -9,8 +9,8 @@ from metal4foam_amd import meshgen as mg
 
 pytestmark = pytest.mark.gpu
 
-SCHEDULES = [dict(spmvSchedule=s, blockSize=b, xcdRemap=r)
-             for s in (0, 1) for b in (64, 256, 1024) for r in (0, 1)]
+SCHEDULES = [dict(spmvSchedule=s, blockSize=b, xcdRemap=r, gridLimit=g)
+             for s in (0, 1) for b in (64, 256, 1024) for r in (0, 1) for g in (0, 7)]
 
 
 def rand_asym(shape, seed):
@@ -26,7 +26,7 @@ def rand_asym(shape, seed):
 
 @pytest.fixture(autouse=True)
 def _default_schedule(ctx):
-    ctx.set_schedule(spmvSchedule=0, blockSize=256, xcdRemap=1, graphChunk=8)
+    ctx.set_schedule(spmvSchedule=0, blockSize=256, xcdRemap=1, graphChunk=8, gridLimit=0)
     yield
 
 
@@ -38,7 +38,7 @@ def gamma(k, u):
 # SpMV
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("sched", SCHEDULES, ids=lambda s: "s{spmvSchedule}b{blockSize}r{xcdRemap}".format(**s))
+@pytest.mark.parametrize("sched", SCHEDULES, ids=lambda s: "s{spmvSchedule}b{blockSize}r{xcdRemap}g{gridLimit}".format(**s))
 def test_spmv_fp32_integer_inputs_bit_exact(ctx, sched):
     m = rand_asym((37, 23, 11), 1)
     rng = np.random.default_rng(2)
