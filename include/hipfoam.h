@@ -255,6 +255,9 @@ typedef struct
     int32_t xcdRemap;     /* 1 = contiguous row range per XCD */
     int32_t graphChunk;
     int32_t gridLimit;    /* max workgroups per row kernel; 0 = 8 per CU */
+    int32_t rowForm;      /* 0 = unrolled predicated rows when every row has
+                             <= 3 owner and <= 3 neighbour faces, else loops;
+                             1 = always loops */
 } hipfoam_schedule;
 
 int32_t hipfoam_get_schedule(const hipfoam_context* ctx, hipfoam_schedule* s);

@@ -59,7 +59,7 @@ class Performance(C.Structure):
 class Schedule(C.Structure):
     _fields_ = [("spmvSchedule", C.c_int32), ("blockSize", C.c_int32),
                 ("xcdRemap", C.c_int32), ("graphChunk", C.c_int32),
-                ("gridLimit", C.c_int32)]
+                ("gridLimit", C.c_int32), ("rowForm", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -5,6 +5,7 @@
 // Reference interfaces replaced (scotCW/metal4Foam):
 //   metalFoamBackend   src/metal4Foam/backend/metalFoamBackend.{H,C}
 //   metalFoamSolver    src/metal4Foam/solvers/metalFoamSolver.C:77-437
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <fenv.h>
@@ -95,10 +96,11 @@ struct Work {
 enum Kind { KIND_PCG = 0, KIND_BICG = 1 };
 
 struct GraphKey {
-  int kind, prec, order, chunk, direct, remap, block, grid, gen;
+  int kind, prec, order, chunk, direct, remap, block, grid, form, gen;
   bool operator<(const GraphKey& o) const {
-    return std::tie(kind, prec, order, chunk, direct, remap, block, grid, gen) <
-           std::tie(o.kind, o.prec, o.order, o.chunk, o.direct, o.remap, o.block, o.grid, o.gen);
+    return std::tie(kind, prec, order, chunk, direct, remap, block, grid, form, gen) <
+           std::tie(o.kind, o.prec, o.order, o.chunk, o.direct, o.remap, o.block, o.grid,
+                    o.form, o.gen);
   }
 };
 
@@ -125,6 +127,7 @@ struct hipfoam_context {
 
   // topology
   int n = 0, nf = 0;
+  int maxOwn = 0, maxNbr = 0;  // longest owner / neighbour face run of a row
   const int32_t* lastLow = nullptr;
   const int32_t* lastUpp = nullptr;
   int gen = 0;  // bumps whenever device buffers are reallocated
@@ -155,7 +158,7 @@ struct hipfoam_context {
   hipEvent_t evPoll[2] = {nullptr, nullptr};
   hipEvent_t evT0 = nullptr, evT1 = nullptr;
 
-  hipfoam_schedule sched{0, 256, 1, 8, 0};
+  hipfoam_schedule sched{0, 256, 1, 8, 0, 0};
   std::map<GraphKey, hipGraphExec_t> graphs;
   KernelTimer timer;
 };
@@ -229,6 +232,7 @@ LduOp<T> make_op(hipfoam_context* c, const T* diag, const T* upper, const T* low
   op.upper = upper;
   op.n = c->n;
   op.xcdRemap = c->sched.xcdRemap;
+  op.shortRows = c->maxOwn <= kShortRow && c->maxNbr <= kShortRow && c->sched.rowForm == 0;
   op.direct = c->sched.spmvSchedule == 1;
   if (op.direct) {
     op.nbrAddr = c->lowAddr.p;
@@ -292,27 +296,33 @@ int timer_flush(hipfoam_context* c) {
   return HIPFOAM_OK;
 }
 
-// Brackets `count` back-to-back launches of one kernel class with a pair of
-// HIP events on the context stream (only outside graph capture).
-struct TimedLaunch {
-  hipfoam_context* c;
-  bool active;
-  int count;
-  TimedLaunch(hipfoam_context* cc, int kclass, bool capturing, int n = 1) : c(cc), count(n) {
-    KernelTimer& t = c->timer;
-    active = t.on && !capturing && t.kclass == kclass;
-    if (!active) return;
-    if (t.used + 2 > t.ev.size()) timer_flush(c);
-    (void)hipEventRecord(t.ev[t.used], c->stream);
-  }
-  ~TimedLaunch() {
-    if (!active) return;
-    KernelTimer& t = c->timer;
-    (void)hipEventRecord(t.ev[t.used + 1], c->stream);
-    t.count[t.used / 2] = count;
-    t.used += 2;
-  }
-};
+// Kernel-class timing: a timed launch goes through hipExtLaunchKernelGGL with
+// a start/stop event pair that the dispatch packet itself records, so the
+// elapsed time is the kernel's execution, not the gap between launches.
+bool timer_wants(hipfoam_context* c, int kclass, bool capturing) {
+  const KernelTimer& t = c->timer;
+  return t.on && !capturing && t.kclass == kclass;
+}
+
+std::pair<hipEvent_t, hipEvent_t> timer_slot(hipfoam_context* c) {
+  KernelTimer& t = c->timer;
+  if (t.used + 2 > t.ev.size()) timer_flush(c);
+  auto e = std::make_pair(t.ev[t.used], t.ev[t.used + 1]);
+  t.count[t.used / 2] = 1;
+  t.used += 2;
+  return e;
+}
+
+#define LAUNCH_CLASSED(kclass, capturing, kernel, grid, block, ...)                             \
+  do {                                                                                          \
+    if (timer_wants(c, (kclass), (capturing))) {                                                \
+      const auto ev_ = timer_slot(c);                                                           \
+      hipExtLaunchKernelGGL(kernel, grid, block, 0, c->stream, ev_.first, ev_.second, 0,        \
+                            __VA_ARGS__);                                                       \
+    } else {                                                                                    \
+      hipLaunchKernelGGL(kernel, grid, block, 0, c->stream, __VA_ARGS__);                       \
+    }                                                                                           \
+  } while (0)
 
 // -------------------------------------------------------------------------
 // one Krylov iteration, enqueued on the context stream
@@ -329,33 +339,21 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
   const int ps = c->pstride;
   if (order > 1) {
     const T* zin = nullptr;
-    // the order-3 interior steps (2 .. order-2) are consecutive launches,
-    // timed as one region of kernel class 0
-    TimedLaunch* interior = nullptr;
-    alignas(TimedLaunch) unsigned char interiorBuf[sizeof(TimedLaunch)];
     for (int s = 1; s < order; ++s) {
       const bool first = (s == 1), last = (s == order - 1);
       T* zout = last ? w.wA.p : ((s & 1) ? w.z0.p : w.z1.p);
-      if (s == 2 && order > 3) interior = new (interiorBuf) TimedLaunch(c, 0, capturing, order - 3);
-      {
-        TimedLaunch tl(c, first ? 3 : (last ? 4 : -1), capturing);
-        if (first && last)
-          hipLaunchKernelGGL((k_cheb_step<T, true, true>), grid, block, 0, c->stream, A, w.r.p,
-                             rD, zin, w.w.p, zout, st, P, ps, s);
-        else if (first)
-          hipLaunchKernelGGL((k_cheb_step<T, true, false>), grid, block, 0, c->stream, A, w.r.p,
-                             rD, zin, w.w.p, zout, st, P, ps, s);
-        else if (last)
-          hipLaunchKernelGGL((k_cheb_step<T, false, true>), grid, block, 0, c->stream, A, w.r.p,
-                             rD, zin, w.w.p, zout, st, P, ps, s);
-        else
-          hipLaunchKernelGGL((k_cheb_step<T, false, false>), grid, block, 0, c->stream, A,
-                             w.r.p, rD, zin, w.w.p, zout, st, P, ps, s);
-      }
-      if (interior && s == order - 2) {
-        interior->~TimedLaunch();
-        interior = nullptr;
-      }
+      if (first && last)
+        LAUNCH_CLASSED(3, capturing, (k_cheb_step<T, true, true>), grid, block, A, w.r.p, rD, zin,
+                       w.w.p, zout, st, P, ps, s);
+      else if (first)
+        LAUNCH_CLASSED(3, capturing, (k_cheb_step<T, true, false>), grid, block, A, w.r.p, rD,
+                       zin, w.w.p, zout, st, P, ps, s);
+      else if (last)
+        LAUNCH_CLASSED(4, capturing, (k_cheb_step<T, false, true>), grid, block, A, w.r.p, rD,
+                       zin, w.w.p, zout, st, P, ps, s);
+      else
+        LAUNCH_CLASSED(0, capturing, (k_cheb_step<T, false, false>), grid, block, A, w.r.p, rD,
+                       zin, w.w.p, zout, st, P, ps, s);
       if (last) launch_fin<1>(c, grid.x, true, kRunning, FinBeta<T>{st});
       zin = zout;
     }
@@ -366,11 +364,8 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
   }
   T* pold = parity ? w.p1.p : w.p0.p;
   T* pnew = parity ? w.p0.p : w.p1.p;
-  {
-    TimedLaunch tl(c, 1, capturing);
-    hipLaunchKernelGGL((k_pcg_p_spmv<T>), grid, block, 0, c->stream, A, w.wA.p, pold, pnew,
-                       w.Ap.p, st, P, ps);
-  }
+  LAUNCH_CLASSED(1, capturing, (k_pcg_p_spmv<T>), grid, block, A, w.wA.p, pold, pnew, w.Ap.p,
+                 st, P, ps);
   launch_fin<1>(c, grid.x, true, kRunning, FinAlpha<T>{st});
   hipLaunchKernelGGL((k_pcg_update<T>), grid, block, 0, c->stream, c->n, A.xcdRemap,
                      w.delta.p, w.deltaBest.p, w.r.p, pnew, w.Ap.p, st, P, ps);
@@ -392,20 +387,14 @@ void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, bool 
   launch_fin<1>(c, nb, true, kRunning, FinBicgRho<T>{st});
   hipLaunchKernelGGL((k_bicg_p_y<T>), grid, block, 0, c->stream, n, rm, w.p0.p, w.r.p, w.AyA.p,
                      rD, w.yA.p, st);
-  {
-    TimedLaunch tl(c, 2, capturing);
-    hipLaunchKernelGGL((k_bicg_spmv_alpha<T>), grid, block, 0, c->stream, A, w.yA.p, w.rA0.p,
-                       w.AyA.p, st, P, ps);
-  }
+  LAUNCH_CLASSED(2, capturing, (k_bicg_spmv_alpha<T>), grid, block, A, w.yA.p, w.rA0.p,
+                 w.AyA.p, st, P, ps);
   launch_fin<1>(c, nb, true, kRunning, FinBicgAlpha<T>{st});
   hipLaunchKernelGGL((k_bicg_s<T>), grid, block, 0, c->stream, n, rm, w.sA.p, w.r.p, w.AyA.p,
                      st, P, ps);
   launch_fin<1>(c, nb, true, kRunning, FinBicgS<T>{st});
-  {
-    TimedLaunch tl(c, 2, capturing);
-    hipLaunchKernelGGL((k_bicg_z_spmv<T>), grid, block, 0, c->stream, A, rD, w.sA.p, w.zA.p,
-                       w.tA.p, st, P, ps);
-  }
+  LAUNCH_CLASSED(2, capturing, (k_bicg_z_spmv<T>), grid, block, A, rD, w.sA.p, w.zA.p, w.tA.p,
+                 st, P, ps);
   launch_fin<2>(c, nb, true, kRunning, FinBicgOmega<T>{st});
   hipLaunchKernelGGL((k_bicg_update<T>), grid, block, 0, c->stream, n, rm, w.delta.p,
                      w.deltaBest.p, w.yA.p, w.zA.p, w.r.p, w.sA.p, w.tA.p, st, P, ps);
@@ -429,7 +418,7 @@ template <typename T>
 int get_graph(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
               int chunk, hipGraphExec_t* out) {
   GraphKey key{(int)kind, (int)sizeof(T), order, chunk, A.direct, A.xcdRemap,
-               c->sched.blockSize, row_grid(c), c->gen};
+               c->sched.blockSize, row_grid(c), A.shortRows, c->gen};
   auto it = c->graphs.find(key);
   if (it != c->graphs.end()) {
     *out = it->second;
@@ -924,10 +913,7 @@ int amul_impl(hipfoam_context* c, int precision, const void* x, void* y, bool de
       xd = c->w32.z0.p;
       yd = c->w32.z1.p;
     }
-    {
-      TimedLaunch tl(c, 5, false);
-      hipLaunchKernelGGL((k_spmv<float>), g, bl, 0, c->stream, op32(c), xd, yd);
-    }
+    LAUNCH_CLASSED(5, false, (k_spmv<float>), g, bl, op32(c), xd, yd);
     CKL();
     if (!device)
       CK(hipMemcpyAsync(y, yd, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
@@ -942,10 +928,7 @@ int amul_impl(hipfoam_context* c, int precision, const void* x, void* y, bool de
       xd = c->psi1.p;
       yd = c->scratch64.p;
     }
-    {
-      TimedLaunch tl(c, 5, false);
-      hipLaunchKernelGGL((k_spmv<double>), g, bl, 0, c->stream, op64(c), xd, yd);
-    }
+    LAUNCH_CLASSED(5, false, (k_spmv<double>), g, bl, op64(c), xd, yd);
     CKL();
     if (!device)
       CK(hipMemcpyAsync(y, yd, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
@@ -1181,6 +1164,11 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
   c->gen += 1;
   c->n = nCells;
   c->nf = nFaces;
+  c->maxOwn = c->maxNbr = 0;
+  for (int32_t i = 0; i < nCells; ++i) {
+    c->maxOwn = std::max(c->maxOwn, ownStart[i + 1] - ownStart[i]);
+    c->maxNbr = std::max(c->maxNbr, losortStart[i + 1] - losortStart[i]);
+  }
   c->lastLow = lowAddr;
   c->lastUpp = uppAddr;
   c->have32 = c->have64 = false;
@@ -1415,6 +1403,7 @@ int32_t hipfoam_set_schedule(hipfoam_context* c, const hipfoam_schedule* s) {
   if (s->graphChunk < 0 || s->graphChunk > 4096 || (s->graphChunk & 1))
     return fail(c, HIPFOAM_ERR_ARG, "graphChunk must be even in [0, 4096]");
   if (s->gridLimit < 0) return fail(c, HIPFOAM_ERR_ARG, "gridLimit must be >= 0");
+  if (s->rowForm < 0 || s->rowForm > 1) return fail(c, HIPFOAM_ERR_ARG, "rowForm must be 0 or 1");
   if (c->good) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
@@ -1441,10 +1430,12 @@ int32_t hipfoam_autotune(hipfoam_context* c, hipfoam_schedule* best) {
   CK(hipEventCreate(&e1));
   const int reps = 20;
   const int cus = std::max(c->numCUs, 1);
+  for (int form = 0; form <= 1; ++form)
   for (int sch = 0; sch <= 1; ++sch)
     for (int bs : {128, 256, 512, 1024})
       for (int rm = 0; rm <= 1; ++rm)
         for (int gl : {0, 4 * cus, 16 * cus, 1 << 30}) {
+        c->sched.rowForm = form;
         c->sched.spmvSchedule = sch;
         c->sched.blockSize = bs;
         c->sched.xcdRemap = rm;

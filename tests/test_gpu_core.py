@@ -9,8 +9,9 @@ from metal4foam_amd import meshgen as mg
 
 pytestmark = pytest.mark.gpu
 
-SCHEDULES = [dict(spmvSchedule=s, blockSize=b, xcdRemap=r, gridLimit=g)
-             for s in (0, 1) for b in (64, 256, 1024) for r in (0, 1) for g in (0, 7)]
+SCHEDULES = [dict(spmvSchedule=s, blockSize=b, xcdRemap=r, gridLimit=g, rowForm=f)
+             for s in (0, 1) for b in (64, 256, 1024) for r in (0, 1) for g in (0, 7)
+             for f in (0, 1)]
 
 
 def rand_asym(shape, seed):
@@ -26,7 +27,8 @@ def rand_asym(shape, seed):
 
 @pytest.fixture(autouse=True)
 def _default_schedule(ctx):
-    ctx.set_schedule(spmvSchedule=0, blockSize=256, xcdRemap=1, graphChunk=8, gridLimit=0)
+    ctx.set_schedule(spmvSchedule=0, blockSize=256, xcdRemap=1, graphChunk=8, gridLimit=0,
+                     rowForm=0)
     yield
 
 
@@ -38,7 +40,7 @@ def gamma(k, u):
 # SpMV
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("sched", SCHEDULES, ids=lambda s: "s{spmvSchedule}b{blockSize}r{xcdRemap}g{gridLimit}".format(**s))
+@pytest.mark.parametrize("sched", SCHEDULES, ids=lambda s: "s{spmvSchedule}b{blockSize}r{xcdRemap}g{gridLimit}f{rowForm}".format(**s))
 def test_spmv_fp32_integer_inputs_bit_exact(ctx, sched):
     m = rand_asym((37, 23, 11), 1)
     rng = np.random.default_rng(2)
@@ -208,8 +210,9 @@ def test_solve_device_entry_and_graph_modes_agree_bitwise(ctx):
     m = mg.poisson(24, 20, 16, seed=7, lognormal=True)
     ctx.set_topology(m)
     outs = []
-    for chunk in (8, 0, 2):
-        ctx.set_schedule(graphChunk=chunk)
+    # graph chunks, eager launches, and the looped row form all give the same bits
+    for chunk, form in ((8, 0), (0, 0), (2, 0), (8, 1)):
+        ctx.set_schedule(graphChunk=chunk, rowForm=form)
         psi = np.zeros(m.n)
         perf = ctx.solve(H.PCG, m.diag, m.upper, None, psi, m.source)
         outs.append((psi, perf.nIterations))
