@@ -208,6 +208,49 @@ int32_t hipfoam_adjoint_device(hipfoam_context* ctx, int32_t solver,
                                const hipfoam_controls* controls,
                                hipfoam_performance* perf);
 
+/* ---- batched solves (no reference counterpart) ------------------------ */
+
+/* Many independent LDU systems, each with nRhs right-hand sides sharing the
+ * system's matrix (segregated Ux/Uy/Uz/... or independent meshes), solved
+ * together: every step of the refined solve is one launch over all systems
+ * and right-hand sides, each (system, rhs) segment keeping its own scalars,
+ * convergence test, deflation, restarts and polish exactly as hipfoam_solve
+ * does for one system.  A batch holds either symmetric systems (lower NULL,
+ * HIPFOAM_PCG or HIPFOAM_PBICGSTAB) or asymmetric ones (HIPFOAM_PBICGSTAB).
+ * Batched hipPCG uses the Chebyshev-Jacobi preconditioner (order >= 2). */
+typedef struct hipfoam_batch hipfoam_batch;
+
+typedef struct
+{
+    int32_t nCells, nFaces;
+    const int32_t* lowAddr;      /* [nFaces] */
+    const int32_t* uppAddr;      /* [nFaces] */
+    const int32_t* ownStart;     /* [nCells + 1] */
+    const int32_t* losort;       /* [nFaces] */
+    const int32_t* losortStart;  /* [nCells + 1] */
+    const double* diag;          /* [nCells] */
+    const double* upper;         /* [nFaces] */
+    const double* lower;         /* [nFaces] or NULL (symmetric) */
+} hipfoam_system;
+
+/* Uploads addressing and coefficients of every system (host pointers, copied). */
+int32_t hipfoam_batch_create(hipfoam_context* ctx, int32_t solver,
+                             int32_t nSystems, const hipfoam_system* systems,
+                             int32_t nRhs, hipfoam_batch** out);
+/* source, psi: [nRhs][nCells] (rhs-major); psi NULL starts from zero. */
+int32_t hipfoam_batch_set_rhs(hipfoam_batch* batch, int32_t system,
+                              const double* source, const double* psi);
+/* Sets every psi of the batch to zero on the device. */
+int32_t hipfoam_batch_reset_psi(hipfoam_batch* batch);
+/* Refined solve of every segment; perf (may be NULL) receives
+ * nSystems * nRhs records, segment = system * nRhs + rhs. */
+int32_t hipfoam_batch_solve(hipfoam_batch* batch,
+                            const hipfoam_controls* controls,
+                            hipfoam_performance* perf);
+/* psi: [nRhs][nCells] of one system. */
+int32_t hipfoam_batch_get_psi(hipfoam_batch* batch, int32_t system, double* psi);
+void hipfoam_batch_destroy(hipfoam_batch* batch);
+
 /* ---- operator primitives --------------------------------------------- */
 
 /* y = A x with the operator last staged: precision HIPFOAM_FP32 uses the

@@ -65,6 +65,13 @@ class Schedule(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class System(C.Structure):
+    _fields_ = [("nCells", C.c_int32), ("nFaces", C.c_int32), ("lowAddr", _i32p),
+                ("uppAddr", _i32p), ("ownStart", _i32p), ("losort", _i32p),
+                ("losortStart", _i32p), ("diag", _f64p), ("upper", _f64p),
+                ("lower", _f64p)]
+
+
 _P = C.c_void_p  # hipfoam_context*
 
 SIGNATURES = {
@@ -100,6 +107,13 @@ SIGNATURES = {
     "hipfoam_adjoint_device": (C.c_int32, [_P, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp,
                                            _vp, _vp, _vp, _vp, C.POINTER(Controls),
                                            C.POINTER(Performance)]),
+    "hipfoam_batch_create": (C.c_int32, [_P, C.c_int32, C.c_int32, C.POINTER(System),
+                                         C.c_int32, C.POINTER(_vp)]),
+    "hipfoam_batch_set_rhs": (C.c_int32, [_vp, C.c_int32, _vp, _vp]),
+    "hipfoam_batch_reset_psi": (C.c_int32, [_vp]),
+    "hipfoam_batch_solve": (C.c_int32, [_vp, C.POINTER(Controls), C.POINTER(Performance)]),
+    "hipfoam_batch_get_psi": (C.c_int32, [_vp, C.c_int32, _vp]),
+    "hipfoam_batch_destroy": (None, [_vp]),
     "hipfoam_amul": (C.c_int32, [_P, C.c_int32, _vp, _vp]),
     "hipfoam_amul_device": (C.c_int32, [_P, C.c_int32, _vp, _vp]),
     "hipfoam_set_matrix": (C.c_int32, [_P, _vp, _vp, _vp]),

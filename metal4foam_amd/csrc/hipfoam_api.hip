@@ -16,12 +16,14 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <new>
 #include <string>
 #include <tuple>
 #include <vector>
 
 #include "../../include/hipfoam.h"
+#include "hipfoam_batched.hpp"
 #include "hipfoam_device.hpp"
 
 using namespace hipfoam;
@@ -195,6 +197,15 @@ void launch_fin(hipfoam_context* c, int nblk, bool checkStatus, int runMask, F f
     hipError_t e_ = (expr);                                                   \
     if (e_ != hipSuccess) {                                                   \
       c->err = std::string(#expr) + ": " + hipGetErrorString(e_);             \
+      return HIPFOAM_ERR_DEVICE;                                              \
+    }                                                                         \
+  } while (0)
+
+#define CK_C(ctx, expr)                                                       \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);         \
       return HIPFOAM_ERR_DEVICE;                                              \
     }                                                                         \
   } while (0)
@@ -1036,6 +1047,8 @@ int adjoint_host(hipfoam_context* c, int solver, const double* diag, const doubl
   return rc;
 }
 
+#include "hipfoam_batch.inc"
+
 }  // namespace
 
 // ===========================================================================
@@ -1321,6 +1334,82 @@ int32_t hipfoam_adjoint_device(hipfoam_context* c, int32_t solver, const double*
   FpTrapGuard fpe;
   return adjoint_device(c, solver, diag, upper, lower, x, g, lambda, dUpper, dLower, dDiag,
                         dSource, controls, perf);
+}
+
+int32_t hipfoam_batch_create(hipfoam_context* c, int32_t solver, int32_t nSystems,
+                             const hipfoam_system* systems, int32_t nRhs, hipfoam_batch** out) {
+  FpTrapGuard fpe;
+  if (!c) return HIPFOAM_ERR_ARG;
+  return batch_create_impl(c, solver, nSystems, systems, nRhs, out);
+}
+
+int32_t hipfoam_batch_set_rhs(hipfoam_batch* bt, int32_t system, const double* source,
+                              const double* psi) {
+  FpTrapGuard fpe;
+  if (!bt) return HIPFOAM_ERR_ARG;
+  hipfoam_context* c = bt->c;
+  if (system < 0 || system >= bt->nsys || !source)
+    return fail(c, HIPFOAM_ERR_ARG, "bad system index or NULL source");
+  CK(hipSetDevice(c->device));
+  const int n = bt->sysRow0[system + 1] - bt->sysRow0[system];
+  double* psiCur = bt->psiIs0 ? bt->psi0.p : bt->psi1.p;
+  for (int r = 0; r < bt->nrhs; ++r) {
+    const long off = (long)r * bt->N + bt->sysRow0[system];
+    CK(hipMemcpyAsync(bt->b.p + off, source + (long)r * n, sizeof(double) * n,
+                      hipMemcpyHostToDevice, c->stream));
+    if (psi)
+      CK(hipMemcpyAsync(psiCur + off, psi + (long)r * n, sizeof(double) * n,
+                        hipMemcpyHostToDevice, c->stream));
+    else
+      hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, 0.0,
+                         psiCur + off);
+  }
+  CKL();
+  CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_batch_reset_psi(hipfoam_batch* bt) {
+  FpTrapGuard fpe;
+  if (!bt) return HIPFOAM_ERR_ARG;
+  hipfoam_context* c = bt->c;
+  CK(hipSetDevice(c->device));
+  const long NR = (long)bt->N * bt->nrhs;
+  double* psiCur = bt->psiIs0 ? bt->psi0.p : bt->psi1.p;
+  if (NR > 0)
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for((int)NR, 256)), dim3(256), 0, c->stream, (int)NR,
+                       0.0, psiCur);
+  CKL();
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_batch_solve(hipfoam_batch* bt, const hipfoam_controls* controls,
+                            hipfoam_performance* perf) {
+  FpTrapGuard fpe;
+  if (!bt) return HIPFOAM_ERR_ARG;
+  CK_C(bt->c, hipSetDevice(bt->c->device));
+  return batch_solve_impl(bt, controls, perf);
+}
+
+int32_t hipfoam_batch_get_psi(hipfoam_batch* bt, int32_t system, double* psi) {
+  FpTrapGuard fpe;
+  if (!bt) return HIPFOAM_ERR_ARG;
+  hipfoam_context* c = bt->c;
+  if (system < 0 || system >= bt->nsys || !psi)
+    return fail(c, HIPFOAM_ERR_ARG, "bad system index or NULL psi");
+  CK(hipSetDevice(c->device));
+  const int n = bt->sysRow0[system + 1] - bt->sysRow0[system];
+  const double* psiCur = bt->psiIs0 ? bt->psi0.p : bt->psi1.p;
+  for (int r = 0; r < bt->nrhs; ++r)
+    CK(hipMemcpyAsync(psi + (long)r * n, psiCur + (long)r * bt->N + bt->sysRow0[system],
+                      sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  CK(hipStreamSynchronize(c->stream));
+  return HIPFOAM_OK;
+}
+
+void hipfoam_batch_destroy(hipfoam_batch* bt) {
+  FpTrapGuard fpe;
+  batch_destroy_impl(bt);
 }
 
 int32_t hipfoam_amul(hipfoam_context* c, int32_t precision, const void* x, void* y) {

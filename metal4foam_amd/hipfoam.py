@@ -282,6 +282,70 @@ class Context:
         return n.value, s.value
 
 
+class Batch:
+    """Many LDU systems x nrhs right-hand sides solved together (hipfoam_batch_*)."""
+
+    def __init__(self, ctx: Context, solver, systems, nrhs):
+        self.ctx = ctx
+        self.nrhs = nrhs
+        self.systems = systems
+        self._keep = []
+        arr = (_lib.System * len(systems))()
+        i32p, f64p = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+        for k, m in enumerate(systems):
+            keep = [_i32(m.lower_addr), _i32(m.upper_addr), _i32(m.owner_start),
+                    _i32(m.losort), _i32(m.losort_start), _f64(m.diag), _f64(m.upper),
+                    _f64(m.lower)]
+            self._keep.append(keep)
+            lo, up, os_, ls, lss, d, u, lw = keep
+            arr[k] = _lib.System(m.n, lo.shape[0], _ptr(lo, i32p), _ptr(up, i32p),
+                                 _ptr(os_, i32p), _ptr(ls, i32p), _ptr(lss, i32p),
+                                 _ptr(d, f64p), _ptr(u, f64p),
+                                 None if lw is None else _ptr(lw, f64p))
+        h = C.c_void_p()
+        ctx._check(ctx.lib.hipfoam_batch_create(ctx.h, solver, len(systems), arr, nrhs,
+                                                C.byref(h)), "batch_create")
+        self.h = h
+        self._keep = None  # the library copied everything
+
+    def set_rhs(self, k, source, psi=None):
+        """source, psi: [nrhs, n] (or [n, nrhs] from the generators: transposed here)."""
+        n = self.systems[k].n
+        src = np.ascontiguousarray(np.asarray(source, dtype=np.float64).reshape(-1, n)
+                                   if np.asarray(source).shape[0] == self.nrhs
+                                   else np.asarray(source, dtype=np.float64).T)
+        ps = None if psi is None else np.ascontiguousarray(psi, dtype=np.float64)
+        self.ctx._check(self.ctx.lib.hipfoam_batch_set_rhs(self.h, k, _ptr(src), _ptr(ps)),
+                        "batch_set_rhs")
+
+    def reset_psi(self):
+        self.ctx._check(self.ctx.lib.hipfoam_batch_reset_psi(self.h), "batch_reset_psi")
+
+    def solve(self, controls=None):
+        controls = controls or default_controls()
+        perf = (Performance * (len(self.systems) * self.nrhs))()
+        self.ctx._check(self.ctx.lib.hipfoam_batch_solve(self.h, C.byref(controls), perf),
+                        "batch_solve")
+        return list(perf)
+
+    def psi(self, k):
+        out = np.empty((self.nrhs, self.systems[k].n))
+        self.ctx._check(self.ctx.lib.hipfoam_batch_get_psi(self.h, k, _ptr(out)),
+                        "batch_get_psi")
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.hipfoam_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------------
 # Reference-interface mirror: run-time-selected solvers
 # ---------------------------------------------------------------------------

@@ -1,0 +1,134 @@
+"""Summarise rocprofv3 CSV output of the LDU solve into a short Markdown table.
+
+Inputs are what these commands write (run from the repository root):
+
+    rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o run -- python3 bench.py ...
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- python3 bench.py ...
+    rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- python3 bench.py ...
+
+    python -m metal4foam_amd.profsum --trace OUT/trace/run_kernel_trace.csv \\
+        --fetch OUT/fetch/run_counter_collection.csv \\
+        --write OUT/write/run_counter_collection.csv --cells N --faces F \\
+        [--asym] --title "..." --command "..." > profiles/<kernel>_<workload>.md
+
+Launches shorter than --noop-us are counted apart: once the device-side
+convergence test has stopped the loop, the launches still queued return at
+entry, and would otherwise dilute the averages.
+
+FETCH_SIZE / WRITE_SIZE (KiB per dispatch) are scaled by what the same
+counter reads for k_copy<double>, a copy of known size (8 B read + 8 B
+written per element).
+"""
+
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import re
+
+
+def _bytes(name: str, n: int, nf: int, asym: bool) -> int | None:
+    """Algorithmic bytes per launch of the kernels on the solve path."""
+    face32 = 20 if asym else 16
+    face64 = 28 if asym else 24
+    if "k_cheb_step<float, false, false>" in name:
+        return 36 * n + 8 + face32 * nf
+    if "k_cheb_step<float, true, false>" in name:
+        return 28 * n + 8 + face32 * nf
+    if "k_cheb_step<float, false, true>" in name:
+        return 32 * n + 8 + face32 * nf
+    if "k_pcg_p_spmv<float>" in name:
+        return 28 * n + 8 + face32 * nf
+    if "k_pcg_update<float>" in name:
+        return 24 * n
+    if "k_spmv<float>" in name:
+        return 20 * n + 8 + face32 * nf
+    if "k_cheb_step<double, false, false>" in name:
+        return 64 * n + 8 + face64 * nf
+    if "k_pcg_p_spmv<double>" in name:
+        return 48 * n + 8 + face64 * nf
+    if "k_bicg_spmv_alpha<float>" in name:
+        return 24 * n + 8 + face32 * nf
+    if "k_bicg_z_spmv<float>" in name:
+        return 28 * n + 8 + face32 * nf
+    return None
+
+
+def _short(name: str) -> str:
+    name = re.sub(r"\(.*", "", name)
+    return name.replace("void ", "").replace("hipfoam::", "")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--cells", type=int, required=True)
+    ap.add_argument("--faces", type=int, required=True)
+    ap.add_argument("--asym", action="store_true")
+    ap.add_argument("--noop-us", type=float, default=6.0)
+    ap.add_argument("--top", type=int, default=12)
+    ap.add_argument("--title", default="rocprofv3 kernel summary")
+    ap.add_argument("--command", action="append", default=[])
+    ap.add_argument("--peak-gbs", type=float, default=8000.0)
+    a = ap.parse_args()
+
+    durs = collections.defaultdict(list)
+    for r in csv.DictReader(open(a.trace)):
+        durs[r["Kernel_Name"]].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+
+    def counters(path):
+        out = collections.defaultdict(list)
+        if path:
+            for r in csv.DictReader(open(path)):
+                out[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        return out
+
+    fetch, write = counters(a.fetch), counters(a.write)
+
+    def mean(v):
+        return sum(v) / len(v) if v else float("nan")
+
+    copy = [k for k in fetch if "k_copy<double>" in k]
+    fscale = wscale = None
+    if copy and a.fetch and a.write:
+        # k_copy<double> over the cells: 8 B read and 8 B written per cell
+        known = 8.0 * a.cells
+        fscale = known / (1024.0 * mean(fetch[copy[0]]))
+        wscale = known / (1024.0 * mean(write[copy[0]]))
+
+    print(f"# {a.title}\n")
+    for c in a.command:
+        print(f"    {c}")
+    print()
+    print(f"Workload: {a.cells:,} cells, {a.faces:,} faces, "
+          f"{'asymmetric' if a.asym else 'symmetric'}. Durations: rocprofv3 kernel trace, "
+          f"launches of at least {a.noop_us} us (shorter launches returned at entry after the "
+          "device-side stop and are listed apart).\n")
+    if fscale:
+        print(f"HBM counter calibration on k_copy<double> ({8 * a.cells:,} B read, "
+              f"{8 * a.cells:,} B written): FETCH_SIZE x {fscale:.3f}, WRITE_SIZE x {wscale:.3f}.\n")
+    print("| kernel | launches (stopped) | avg us | algorithmic MB | GB/s | % of 8 TB/s "
+          "| traffic MB (fetch+write) | traffic / algorithmic |")
+    print("|---|---:|---:|---:|---:|---:|---:|---:|")
+    rows = sorted(durs.items(), key=lambda kv: -sum(kv[1]))[: a.top]
+    for name, v in rows:
+        real = [x for x in v if x >= a.noop_us]
+        stopped = len(v) - len(real)
+        avg = mean(real) if real else mean(v)
+        b = _bytes(name, a.cells, a.faces, a.asym)
+        gbs = b / (avg * 1e3) if (b and real) else None
+        tr = None
+        if fscale and name in fetch and name in write:
+            tr = (mean(fetch[name]) * fscale + mean(write[name]) * wscale) * 1024.0
+        print(f"| `{_short(name)}` | {len(real)} ({stopped}) | {avg:.2f} | "
+              f"{'' if b is None else f'{b / 1e6:.1f}'} | {'' if gbs is None else f'{gbs:.0f}'} | "
+              f"{'' if gbs is None else f'{100 * gbs / a.peak_gbs:.1f}'} | "
+              f"{'' if tr is None else f'{tr / 1e6:.1f}'} | "
+              f"{'' if (tr is None or b is None) else f'{tr / b:.2f}'} |")
+
+
+if __name__ == "__main__":
+    main()
