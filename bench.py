@@ -40,6 +40,13 @@ WORKLOADS = {
     "poisson2d100": ("config0", "pcg",
                      "2D pressure Poisson 100x100x1 (10,000 cells), unit faces; "
                      "hipPCG refined"),
+    "batch64": ("config3", "batch",
+                "64 independent systems, cube sides U{20..80} (8k-512k cells), 32 lognormal "
+                "Poisson (hipPCG) + 32 convection-diffusion (hipPBiCGStab), 5 RHS each: "
+                "320 refined solves per step"),
+    "adjoint128": ("config1", "adjoint",
+                   "adjoint of L = 0.5|x - x_t|^2 through the 128^3 lognormal Poisson: "
+                   "forward hipPCG + transpose solve + face/diag/source gradients"),
 }
 
 
@@ -118,6 +125,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--fd-faces", type=int, default=0,
+                    help="adjoint128: finite-difference check on this many faces and cells")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -130,6 +139,10 @@ def main():
     from metal4foam_amd import meshgen as mg
 
     gen, solver, desc = WORKLOADS[args.workload]
+    if solver == "batch":
+        return run_batch(args, H, mg, desc)
+    if solver == "adjoint":
+        return run_adjoint(args, H, mg, desc)
     m = getattr(mg, gen)()
     sym = m.lower is None
     kind = H.PCG if solver == "pcg" else H.PBICGSTAB
@@ -289,6 +302,160 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = round(cpu["value"] / ms, 1)
+    print(json.dumps(out), flush=True)
+
+
+def _common(args, value, unit, ms, desc, higher):
+    return {
+        "metric": METRIC,
+        "value": value,
+        "unit": unit,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": higher,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": desc},
+    }
+
+
+def run_batch(args, H, mg, desc):
+    """Batched throughput: 320 refined solves (64 systems x 5 RHS) per step."""
+    systems = mg.config3()
+    nrhs = systems[0].source.shape[1]
+    sym = [m for m in systems if m.lower is None]
+    asym = [m for m in systems if m.lower is not None]
+    log(f"batch: {len(sym)} symmetric + {len(asym)} asymmetric systems, {nrhs} RHS, "
+        f"{sum(m.n for m in systems):,} cells")
+    ctx = H.Context(0)
+    log(ctx.description)
+    batches = [H.Batch(ctx, H.PCG, sym, nrhs), H.Batch(ctx, H.PBICGSTAB, asym, nrhs)]
+    for bt in batches:
+        for k, m in enumerate(bt.systems):
+            bt.set_rhs(k, m.source)
+    ctl = H.default_controls(maxRestarts=8, maxIter=20000)
+
+    def step():
+        out = []
+        for bt in batches:
+            bt.reset_psi()
+            out.append(bt.solve(ctl))
+        return out
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        perfs = step()
+    ctx.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
+    nsolves = nrhs * len(systems)
+    flat = [p for ps in perfs for p in ps]
+    conv = sum(1 for p in flat if p.converged)
+    # independent check of every segment's FP64 residual with numpy
+    worst = 0.0
+    q = 0
+    for bt, ps in zip(batches, perfs):
+        for k, m in enumerate(bt.systems):
+            psi = bt.psi(k)
+            for r in range(nrhs):
+                mm = mg.LduMatrix(m.n, m.lower_addr, m.upper_addr, m.owner_start, m.losort,
+                                  m.losort_start, m.diag, m.upper, m.lower,
+                                  m.source[:, r].copy())
+                rs, _ = numpy_residual(mm, psi[r])
+                worst = max(worst, rs / ps[k * nrhs + r].normFactor)
+                q += 1
+    iters = [p.nIterations for p in flat]
+    log(f"batch: {ms:.2f} ms per step, {nsolves / (ms / 1e3):.1f} solves/s, converged "
+        f"{conv}/{nsolves}, inner iterations min/median/max {min(iters)}/"
+        f"{int(np.median(iters))}/{max(iters)}, worst numpy residual {worst:.2e}")
+    out = _common(args, round(nsolves / (ms / 1e3), 2), "solves/s", ms, desc, True)
+    out["config"].update({"systems": len(systems), "rhs": nrhs,
+                          "cells": int(sum(m.n for m in systems)), "tolerance": 1e-6})
+    out["batch"] = {"solves": nsolves, "converged": conv, "worst_numpy_residual": worst,
+                    "iterations_min_median_max": [min(iters), int(np.median(iters)), max(iters)]}
+    out["roofline"] = None
+    out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def run_adjoint(args, H, mg, desc):
+    """Forward + adjoint time on the 128^3 Poisson, with a finite-difference check
+    of 32 face and 32 cell gradients (h = 1e-6, tight plain-FP64 GPU solves)."""
+    m = mg.config1()
+    xt = mg.config4_target(m.n)
+    ctx = H.Context(0)
+    log(ctx.description)
+    ctx.set_topology(m)
+    ctl = H.default_controls(maxRestarts=8, maxIter=20000)
+
+    def fwd_bwd():
+        x = np.zeros(m.n)
+        pf = ctx.solve(H.PCG, m.diag, m.upper, None, x, m.source, ctl)
+        pb = ctx.adjoint(H.PCG, m.diag, m.upper, None, x, x - xt, ctl)
+        return x, pf, pb
+
+    for _ in range(args.warmup):
+        fwd_bwd()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x, pf, (pb, lam, du, _dl, dd, db) = fwd_bwd()
+    ms = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
+    log(f"adjoint: forward+backward {ms:.1f} ms (host arrays), forward it={pf.nIterations} "
+        f"res={pf.finalResidual:.2e}, adjoint it={pb.nIterations} res={pb.finalResidual:.2e}")
+    fd = None
+    if args.fd_faces > 0:
+        rng = np.random.default_rng(1004)
+        tight = H.default_controls(tolerance=1e-14, maxIter=20000)
+        # gradients of the tightly converged solution
+        x = np.zeros(m.n)
+        ctx.solve(H.PCG, m.diag, m.upper, None, x, m.source, tight, fp64=True)
+        pa, lam, du, _dl, dd, db = ctx.adjoint(H.PCG, m.diag, m.upper, None, x, x - xt,
+                                               tight, precision=H.FP64)
+        log(f"adjoint (FP64 transpose solve, tol 1e-14): it={pa.nIterations} "
+            f"res={pa.finalResidual:.2e}")
+        h = 1e-6
+
+        def solve(d, u):
+            y = np.zeros(m.n)
+            ctx.solve(H.PCG, d, u, None, y, m.source, tight, fp64=True)
+            return y
+
+        def dloss(yp, ym):
+            # L(+h) - L(-h) = 0.5 sum (yp - ym)(yp + ym - 2 x_t): no cancellation of L
+            return 0.5 * np.dot(yp - ym, yp + ym - 2.0 * xt)
+
+        worst = 0.0
+        faces = rng.choice(m.nfaces, args.fd_faces, replace=False)
+        cells = rng.choice(m.n, args.fd_faces, replace=False)
+        scale = max(np.abs(du).max(), np.abs(dd).max())
+        for f in faces:
+            u1, u2 = m.upper.copy(), m.upper.copy()
+            u1[f] += h
+            u2[f] -= h
+            g = dloss(solve(m.diag, u1), solve(m.diag, u2)) / (2 * h)
+            worst = max(worst, abs(g - du[f]) / max(abs(du[f]), 1e-3 * scale))
+        for i in cells:
+            d1, d2 = m.diag.copy(), m.diag.copy()
+            d1[i] += h
+            d2[i] -= h
+            g = dloss(solve(d1, m.upper), solve(d2, m.upper)) / (2 * h)
+            worst = max(worst, abs(g - dd[i]) / max(abs(dd[i]), 1e-3 * scale))
+        fd = {"faces": int(args.fd_faces), "cells": int(args.fd_faces), "h": h,
+              "max_rel_err": worst}
+        log(f"adjoint FD check: max relative error {worst:.2e} over {args.fd_faces} faces and "
+            f"{args.fd_faces} cells")
+    out = _common(args, round(ms, 3), "ms", ms, desc, False)
+    out["config"].update({"cells": m.n, "faces": m.nfaces, "tolerance": 1e-6})
+    out["adjoint"] = {"forward_iterations": pf.nIterations, "adjoint_iterations": pb.nIterations,
+                      "fd_check": fd}
+    out["roofline"] = None
+    out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
 
 

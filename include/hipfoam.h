@@ -192,8 +192,11 @@ int32_t hipfoam_solve_fp64_device(hipfoam_context* ctx, int32_t solver,
  * For a symmetric matrix (lower == NULL) the single face coefficient's
  * gradient -(lambda[l] x[u] + lambda[u] x[l]) goes to dUpper and dLower is
  * not written.  Any gradient pointer may be NULL.  perf describes the
- * transpose solve. */
-int32_t hipfoam_adjoint(hipfoam_context* ctx, int32_t solver,
+ * transpose solve.  precision selects the transpose solver: HIPFOAM_FP32 =
+ * the refined FP32-inner solve of hipfoam_solve, HIPFOAM_FP64 = the plain
+ * FP64 solve of hipfoam_solve_fp64 (for tolerances below what FP32 inner
+ * iterations reach on the right-hand side g). */
+int32_t hipfoam_adjoint(hipfoam_context* ctx, int32_t solver, int32_t precision,
                         const double* diag, const double* upper,
                         const double* lower, const double* x, const double* g,
                         double* lambda, double* dUpper, double* dLower,
@@ -201,7 +204,8 @@ int32_t hipfoam_adjoint(hipfoam_context* ctx, int32_t solver,
                         const hipfoam_controls* controls,
                         hipfoam_performance* perf);
 int32_t hipfoam_adjoint_device(hipfoam_context* ctx, int32_t solver,
-                               const double* diag, const double* upper,
+                               int32_t precision, const double* diag,
+                               const double* upper,
                                const double* lower, const double* x,
                                const double* g, double* lambda, double* dUpper,
                                double* dLower, double* dDiag, double* dSource,

@@ -101,11 +101,11 @@ SIGNATURES = {
     "hipfoam_solve_fp64_device": (C.c_int32, [_P, C.c_int32, _vp, _vp, _vp, _vp,
                                               _vp, C.POINTER(Controls),
                                               C.POINTER(Performance), _vp]),
-    "hipfoam_adjoint": (C.c_int32, [_P, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                    _vp, _vp, _vp, C.POINTER(Controls),
+    "hipfoam_adjoint": (C.c_int32, [_P, C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, _vp,
+                                    _vp, _vp, _vp, _vp, _vp, C.POINTER(Controls),
                                     C.POINTER(Performance)]),
-    "hipfoam_adjoint_device": (C.c_int32, [_P, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp,
-                                           _vp, _vp, _vp, _vp, C.POINTER(Controls),
+    "hipfoam_adjoint_device": (C.c_int32, [_P, C.c_int32, C.c_int32, _vp, _vp, _vp, _vp,
+                                           _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(Controls),
                                            C.POINTER(Performance)]),
     "hipfoam_batch_create": (C.c_int32, [_P, C.c_int32, C.c_int32, C.POINTER(System),
                                          C.c_int32, C.POINTER(_vp)]),

@@ -968,10 +968,10 @@ int set_matrix_impl(hipfoam_context* c, const double* diag, const double* upper,
 // A^T lambda = g runs through the same refined path with upper and lower
 // swapped on the same addressing (lduMatrix::Tmul semantics), then the face,
 // diagonal and source gradients are formed from x and lambda.
-int adjoint_device(hipfoam_context* c, int solver, const double* diag, const double* upper,
-                   const double* lower, const double* x, const double* g, double* lambda,
-                   double* dUpper, double* dLower, double* dDiag, double* dSource,
-                   const hipfoam_controls* ctl, hipfoam_performance* perf) {
+int adjoint_device(hipfoam_context* c, int solver, int precision, const double* diag,
+                   const double* upper, const double* lower, const double* x, const double* g,
+                   double* lambda, double* dUpper, double* dLower, double* dDiag,
+                   double* dSource, const hipfoam_controls* ctl, hipfoam_performance* perf) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (!x || !g || !lambda || !ctl) return fail(c, HIPFOAM_ERR_ARG, "NULL x/g/lambda/controls");
@@ -980,8 +980,13 @@ int adjoint_device(hipfoam_context* c, int solver, const double* diag, const dou
   CKL();
   const double* upT = lower ? lower : upper;
   const double* loT = lower ? upper : nullptr;
-  rc = refined_solve(c, solver, diag, upT, loT, lambda, g, ctl, perf, nullptr,
-                     hipMemcpyDeviceToDevice);
+  if (precision != HIPFOAM_FP32 && precision != HIPFOAM_FP64)
+    return fail(c, HIPFOAM_ERR_ARG, "precision must be HIPFOAM_FP32 or HIPFOAM_FP64");
+  rc = precision == HIPFOAM_FP64
+           ? fp64_solve(c, solver, diag, upT, loT, lambda, g, ctl, perf, nullptr,
+                        hipMemcpyDeviceToDevice)
+           : refined_solve(c, solver, diag, upT, loT, lambda, g, ctl, perf, nullptr,
+                           hipMemcpyDeviceToDevice);
   if (rc) return rc;
   if (nf > 0 && (dUpper || dLower)) {
     hipLaunchKernelGGL(k_adjoint_faces, dim3(blocks_for(nf, 256)), dim3(256), 0, c->stream, nf,
@@ -997,10 +1002,10 @@ int adjoint_device(hipfoam_context* c, int solver, const double* diag, const dou
   return HIPFOAM_OK;
 }
 
-int adjoint_host(hipfoam_context* c, int solver, const double* diag, const double* upper,
-                 const double* lower, const double* x, const double* g, double* lambda,
-                 double* dUpper, double* dLower, double* dDiag, double* dSource,
-                 const hipfoam_controls* ctl, hipfoam_performance* perf) {
+int adjoint_host(hipfoam_context* c, int solver, int precision, const double* diag,
+                 const double* upper, const double* lower, const double* x, const double* g,
+                 double* lambda, double* dUpper, double* dLower, double* dDiag,
+                 double* dSource, const hipfoam_controls* ctl, hipfoam_performance* perf) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (!diag || !upper || !x || !g || !lambda || !ctl)
@@ -1028,7 +1033,8 @@ int adjoint_host(hipfoam_context* c, int solver, const double* diag, const doubl
   CK(hipMemcpyAsync(d.p, diag, sizeof(double) * n, h2d, c->stream));
   CK(hipMemcpyAsync(u.p, upper, sizeof(double) * nf, h2d, c->stream));
   if (lower) CK(hipMemcpyAsync(l.p, lower, sizeof(double) * nf, h2d, c->stream));
-  rc = adjoint_device(c, solver, d.p, u.p, lower ? l.p : nullptr, c->adjX.p, c->adjG.p,
+  rc = adjoint_device(c, solver, precision, d.p, u.p, lower ? l.p : nullptr, c->adjX.p,
+                      c->adjG.p,
                       c->adjLam.p, dUpper ? c->adjDU.p : nullptr,
                       (dLower && lower) ? c->adjDL.p : nullptr, dDiag ? c->adjDD.p : nullptr,
                       dSource ? c->adjDB.p : nullptr, ctl, perf);
@@ -1316,24 +1322,24 @@ int32_t hipfoam_solve_fp64_device(hipfoam_context* c, int32_t solver, const doub
                     hipMemcpyDeviceToDevice);
 }
 
-int32_t hipfoam_adjoint(hipfoam_context* c, int32_t solver, const double* diag,
-                        const double* upper, const double* lower, const double* x,
-                        const double* g, double* lambda, double* dUpper, double* dLower,
-                        double* dDiag, double* dSource, const hipfoam_controls* controls,
-                        hipfoam_performance* perf) {
+int32_t hipfoam_adjoint(hipfoam_context* c, int32_t solver, int32_t precision,
+                        const double* diag, const double* upper, const double* lower,
+                        const double* x, const double* g, double* lambda, double* dUpper,
+                        double* dLower, double* dDiag, double* dSource,
+                        const hipfoam_controls* controls, hipfoam_performance* perf) {
   FpTrapGuard fpe;
-  return adjoint_host(c, solver, diag, upper, lower, x, g, lambda, dUpper, dLower, dDiag,
-                      dSource, controls, perf);
+  return adjoint_host(c, solver, precision, diag, upper, lower, x, g, lambda, dUpper, dLower,
+                      dDiag, dSource, controls, perf);
 }
 
-int32_t hipfoam_adjoint_device(hipfoam_context* c, int32_t solver, const double* diag,
-                               const double* upper, const double* lower, const double* x,
-                               const double* g, double* lambda, double* dUpper,
-                               double* dLower, double* dDiag, double* dSource,
+int32_t hipfoam_adjoint_device(hipfoam_context* c, int32_t solver, int32_t precision,
+                               const double* diag, const double* upper, const double* lower,
+                               const double* x, const double* g, double* lambda,
+                               double* dUpper, double* dLower, double* dDiag, double* dSource,
                                const hipfoam_controls* controls, hipfoam_performance* perf) {
   FpTrapGuard fpe;
-  return adjoint_device(c, solver, diag, upper, lower, x, g, lambda, dUpper, dLower, dDiag,
-                        dSource, controls, perf);
+  return adjoint_device(c, solver, precision, diag, upper, lower, x, g, lambda, dUpper, dLower,
+                        dDiag, dSource, controls, perf);
 }
 
 int32_t hipfoam_batch_create(hipfoam_context* c, int32_t solver, int32_t nSystems,

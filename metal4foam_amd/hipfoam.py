@@ -217,9 +217,10 @@ class Context:
                        None if residual is None else residual.ptr), "solve_device")
         return perf
 
-    def adjoint(self, solver, diag, upper, lower, x, g, controls=None):
+    def adjoint(self, solver, diag, upper, lower, x, g, controls=None, precision=FP32):
         """Transpose solve A^T lambda = g and the gradients of L through A x = b.
 
+        precision FP32: refined transpose solve; FP64: plain-FP64 transpose solve.
         Returns (perf, lambda, dUpper, dLower or None, dDiag, dSource)."""
         controls = controls or default_controls()
         d, u, lo = _f64(diag), _f64(upper), _f64(lower)
@@ -231,7 +232,8 @@ class Context:
         dd = np.empty(self.n)
         db = np.empty(self.n)
         perf = Performance()
-        self._check(self.lib.hipfoam_adjoint(self.h, solver, _ptr(d), _ptr(u), _ptr(lo),
+        self._check(self.lib.hipfoam_adjoint(self.h, solver, precision, _ptr(d), _ptr(u),
+                                             _ptr(lo),
                                              _ptr(x), _ptr(g), _ptr(lam), _ptr(du), _ptr(dl),
                                              _ptr(dd), _ptr(db), C.byref(controls),
                                              C.byref(perf)), "adjoint")

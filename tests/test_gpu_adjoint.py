@@ -90,12 +90,20 @@ def test_adjoint_matches_finite_differences(ctx, orc, bicg):
     _, lam, du, dl, dd, db = ctx.adjoint(H.PBICGSTAB if bicg else H.PCG, m.diag, m.upper,
                                          m.lower if bicg else None, x, x - xt,
                                          H.default_controls(**TIGHT))
+    # the plain-FP64 transpose solve gives the same gradients
+    p64, lam64, du64, _, dd64, _ = ctx.adjoint(H.PBICGSTAB if bicg else H.PCG, m.diag, m.upper,
+                                               m.lower if bicg else None, x, x - xt,
+                                               H.default_controls(**FD), precision=H.FP64)
+    assert p64.finalResidual < 1e-12
+    np.testing.assert_allclose(lam64, lam, rtol=0, atol=1e-8 * np.abs(lam).max())
+    np.testing.assert_allclose(du64, du, rtol=0, atol=1e-8 * np.abs(du).max())
     h = 1e-6
     faces = rng.choice(m.nfaces, 6, replace=False)
     cells = rng.choice(m.n, 6, replace=False)
 
     def fd(kind, idx):
-        vals = []
+        # L(+h) - L(-h) = 0.5 sum (y+ - y-)(y+ + y- - 2 x_t): no cancellation of L
+        ys = []
         for sgn in (1, -1):
             d, u, lo, b = m.diag.copy(), m.upper.copy(), None if m.lower is None else \
                 m.lower.copy(), m.source.copy()
@@ -107,8 +115,8 @@ def test_adjoint_matches_finite_differences(ctx, orc, bicg):
                 d[idx] += sgn * h
             else:
                 b[idx] += sgn * h
-            vals.append(_loss(_solve(ctx, m, bicg, d, u, lo, b, fp64=True), xt))
-        return (vals[0] - vals[1]) / (2 * h)
+            ys.append(_solve(ctx, m, bicg, d, u, lo, b, fp64=True))
+        return 0.5 * np.dot(ys[0] - ys[1], ys[0] + ys[1] - 2 * xt) / (2 * h)
 
     worst = 0.0
     for f in faces:
