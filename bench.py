@@ -80,6 +80,14 @@ def cheb_step_bytes(n, nf, symmetric):
     return 36 * n + 8 + face * nf
 
 
+def to_slot_bytes(ldu_bytes, n, nf, symmetric):
+    """The same kernel on the row-slot layout: the two row-start arrays and
+    the per-face arrays are replaced by six FP32 coefficient slots and three
+    packed int16 offset words per row (36 B/row, every array read once)."""
+    face = 16 if symmetric else 20
+    return ldu_bytes - (8 * n + 8 + face * nf) + 36 * n
+
+
 def cpu_baseline(m, solver, sample_iters):
     """Oracle PCG(DIC) / PBiCGStab(DILU) in FP64 on one host core, timed on a
     bounded number of iterations and scaled to the iteration count of the full
@@ -122,9 +130,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="poisson128", choices=sorted(WORKLOADS))
     ap.add_argument("--fp64-steps", type=int, default=2)
-    ap.add_argument("--cpu-iters", type=int, default=100)
+    ap.add_argument("--host-steps", type=int, default=2,
+                    help="solves timed with host arrays (PCIe copies included)")
+    ap.add_argument("--cpu-iters", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--cold-reps", type=int, default=10,
+                    help="SpMV launches timed after a MALL-flushing copy")
     ap.add_argument("--fd-faces", type=int, default=0,
                     help="adjoint128: finite-difference check on this many faces and cells")
     args = ap.parse_args()
@@ -185,6 +197,19 @@ def main():
     if not (perf.converged and true_res < 1e-6):
         log("ERROR: refined solve did not reach 1e-6")
 
+    # ---- PCIe-inclusive: host arrays in and out, as the OpenFOAM plugin passes them
+    host_ms = None
+    if args.host_steps > 0:
+        hpsi = np.zeros(m.n)
+        lower_h = None if sym else m.lower
+        ctx.solve(kind, m.diag, m.upper, lower_h, hpsi, m.source, ctl)
+        t0 = time.perf_counter()
+        for _ in range(args.host_steps):
+            hpsi[:] = 0.0
+            ctx.solve(kind, m.diag, m.upper, lower_h, hpsi, m.source, ctl)
+        host_ms = 1e3 * (time.perf_counter() - t0) / args.host_steps
+        log(f"refined with host arrays (PCIe included): {host_ms:.3f} ms/solve")
+
     # ---- plain FP64 comparator ----------------------------------------------------
     fp64_ms = None
     fp64_perf = None
@@ -205,16 +230,21 @@ def main():
     step()
     nl, secs = ctx.kernel_timing()
     ctx.set_kernel_timing(False, 0)
+    layout = ctx.active_schedule()
+    slots = layout == 2
     if solver == "pcg":
-        kbytes = cheb_step_bytes(m.n, m.nfaces, sym)
+        ref_kbytes = cheb_step_bytes(m.n, m.nfaces, sym)
         kname = "k_cheb_step<float, false, false> (interior Chebyshev step fused with its SpMV)"
     else:
-        kbytes = spmv_bytes(m.n, m.nfaces, sym, extra_vectors=2)
+        ref_kbytes = spmv_bytes(m.n, m.nfaces, sym, extra_vectors=2)
         kname = "k_bicg_spmv_alpha / k_bicg_z_spmv (BiCGStab SpMV + dot)"
+    kbytes = to_slot_bytes(ref_kbytes, m.n, m.nfaces, sym) if slots else ref_kbytes
     kavg = secs / max(nl, 1)
     kgbs = kbytes / kavg / 1e9 if nl else 0.0
-    log(f"dominant kernel {kname}: {nl} launches, avg {1e6 * kavg:.2f} us, "
-        f"{kgbs:.0f} GB/s ({100 * kgbs / HBM_PEAK_GBS:.1f}% of HBM peak)")
+    ref_kgbs = ref_kbytes / kavg / 1e9 if nl else 0.0
+    log(f"dominant kernel {kname} ({'row-slot' if slots else 'LDU face'} layout): {nl} "
+        f"launches, avg {1e6 * kavg:.2f} us, {kgbs:.0f} GB/s ({100 * kgbs / HBM_PEAK_GBS:.1f}% "
+        f"of HBM peak); LDU-format equivalent {ref_kgbs:.0f} GB/s")
 
     # plain FP32 SpMV of the inner operator (the metric's SpMV GB/s)
     x = D.from_host(ctx, np.random.default_rng(0).standard_normal(m.n).astype(np.float32))
@@ -224,15 +254,43 @@ def main():
         ctx.amul_device(x, y, H.FP32)
     ns, ssecs = ctx.kernel_timing()
     ctx.set_kernel_timing(False, 0)
-    sbytes = spmv_bytes(m.n, m.nfaces, sym)
+    ref_sbytes = spmv_bytes(m.n, m.nfaces, sym)
+    sbytes = to_slot_bytes(ref_sbytes, m.n, m.nfaces, sym) if slots else ref_sbytes
     savg = ssecs / max(ns, 1)
     sgbs = sbytes / savg / 1e9 if ns else 0.0
-    log(f"plain FP32 SpMV: avg {1e6 * savg:.2f} us, {sgbs:.0f} GB/s")
+    ref_sgbs = ref_sbytes / savg / 1e9 if ns else 0.0
+    log(f"plain FP32 SpMV: avg {1e6 * savg:.2f} us, {sgbs:.0f} GB/s "
+        f"(LDU-format equivalent {ref_sgbs:.0f} GB/s)")
+
+    # the same SpMV with the 256 MB MALL (Infinity Cache) flushed before every
+    # launch by a 2 GiB device-to-device copy
+    cold = None
+    if args.cold_reps > 0:
+        big = 256 * 1024 * 1024
+        fa, fb = D(ctx, (big,), np.float64), D(ctx, (big,), np.float64)
+        ctx.set_kernel_timing(True, 5)
+        for _ in range(args.cold_reps):
+            fb.copy_from(fa)
+            ctx.amul_device(x, y, H.FP32)
+        nc, csecs = ctx.kernel_timing()
+        ctx.set_kernel_timing(False, 0)
+        fa.free()
+        fb.free()
+        cavg = csecs / max(nc, 1)
+        cold = {"avg_launch_us": round(1e6 * cavg, 3),
+                "GB_s": round(sbytes / cavg / 1e9, 1),
+                "frac": round(sbytes / cavg / 1e9 / HBM_PEAK_GBS, 4),
+                "reference_format_GB_s": round(ref_sbytes / cavg / 1e9, 1)}
+        log(f"plain FP32 SpMV after a MALL flush: avg {1e6 * cavg:.2f} us, "
+            f"{cold['GB_s']:.0f} GB/s")
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
+        tj = json.load(open(tfile))
+        # counters measured on the layout this run uses only
+        if tj.get("layout", "LDU face") == ("row-slot" if slots else "LDU face"):
+            traffic = tj.get("bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
@@ -271,16 +329,23 @@ def main():
             "frac": round(kgbs / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "kernel": kname,
+            "layout": "row-slot" if slots else "LDU face",
             "bytes_per_launch": kbytes,
+            "reference_format_bytes": ref_kbytes,
+            "reference_format_GB_s": round(ref_kgbs, 1),
             "avg_launch_us": round(1e6 * kavg, 3),
             "launches": nl,
         },
         "spmv": {
-            "kernel": "k_spmv<float> (plain FP32 LDU SpMV)",
+            "kernel": "k_spmv<float> (plain FP32 SpMV of the inner operator)",
+            "layout": "row-slot" if slots else "LDU face",
             "bytes_per_launch": sbytes,
             "avg_launch_us": round(1e6 * savg, 3),
             "GB_s": round(sgbs, 1),
             "frac": round(sgbs / HBM_PEAK_GBS, 4),
+            "reference_format_bytes": ref_sbytes,
+            "reference_format_GB_s": round(ref_sgbs, 1),
+            "mall_flushed": cold,
         },
         "solve": {
             "iterations": perf.nIterations,
@@ -290,6 +355,7 @@ def main():
             "converged": bool(perf.converged and true_res < 1e-6),
             "deflated": perf.deflated,
             "polished": perf.polished,
+            "host_arrays_ms": None if host_ms is None else round(host_ms, 3),
         },
         "fp64_plain": None if fp64_perf is None else {
             "ms": round(fp64_ms, 4),

@@ -291,6 +291,12 @@ int32_t hipfoam_synchronize(hipfoam_context* ctx);
 /* SpMV schedules of the inner operator:
  *   0 = losort-ordered lower coefficients (one indirection),
  *   1 = direct lower[losort[k]] / lowAddr[losort[k]] (two indirections),
+ *   2 = row slots (the default): every row's <= 3 owner and <= 3 neighbour
+ *       coefficients in six slot-major arrays with int16 column offsets, no
+ *       row-start indirection; used when every row has <= 3 owner and
+ *       <= 3 neighbour faces and every |upperAddr - lowerAddr| <= 32767,
+ *       otherwise schedule 0 runs.  Same terms in the same order as 0 and 1,
+ *       so all three give identical results,
  * and the iteration driver:
  *   graphChunk = iterations captured per hipGraph launch (0 = eager).
  * A row kernel launches min(ceil(n / blockSize), gridLimit) workgroups; each
@@ -308,6 +314,11 @@ typedef struct
 } hipfoam_schedule;
 
 int32_t hipfoam_get_schedule(const hipfoam_context* ctx, hipfoam_schedule* s);
+/* The SpMV schedule the inner operator of the current mesh runs: the
+ * requested one, or 0 when row slots (2) were requested and the mesh does
+ * not allow them.  No reference counterpart (the reference has one kernel). */
+int32_t hipfoam_active_schedule(const hipfoam_context* ctx,
+                                int32_t* spmvSchedule);
 int32_t hipfoam_set_schedule(hipfoam_context* ctx, const hipfoam_schedule* s);
 /* Times the candidate SpMV schedules on the staged operator and keeps the
  * fastest; *best (may be NULL) receives it. */

@@ -125,6 +125,7 @@ SIGNATURES = {
     "hipfoam_copy_on_device": (C.c_int32, [_P, _vp, _vp, C.c_uint64]),
     "hipfoam_synchronize": (C.c_int32, [_P]),
     "hipfoam_get_schedule": (C.c_int32, [_P, C.POINTER(Schedule)]),
+    "hipfoam_active_schedule": (C.c_int32, [_P, C.POINTER(C.c_int32)]),
     "hipfoam_set_schedule": (C.c_int32, [_P, C.POINTER(Schedule)]),
     "hipfoam_autotune": (C.c_int32, [_P, C.POINTER(Schedule)]),
     "hipfoam_set_kernel_timing": (C.c_int32, [_P, C.c_int32, C.c_int32]),

@@ -130,6 +130,9 @@ struct hipfoam_context {
   // topology
   int n = 0, nf = 0;
   int maxOwn = 0, maxNbr = 0;  // longest owner / neighbour face run of a row
+  bool slotOK = false;  // row-slot layout possible: short rows, |column - row| <= 32767
+  Buf<uint32_t> slotOff;
+  Buf<float> slotCoef32;
   const int32_t* lastLow = nullptr;
   const int32_t* lastUpp = nullptr;
   int gen = 0;  // bumps whenever device buffers are reallocated
@@ -160,7 +163,7 @@ struct hipfoam_context {
   hipEvent_t evPoll[2] = {nullptr, nullptr};
   hipEvent_t evT0 = nullptr, evT1 = nullptr;
 
-  hipfoam_schedule sched{0, 256, 1, 8, 0, 0};
+  hipfoam_schedule sched{2, 256, 1, 8, 0, 0};
   std::map<GraphKey, hipGraphExec_t> graphs;
   KernelTimer timer;
 };
@@ -231,9 +234,11 @@ void clear_graphs(hipfoam_context* c) {
   c->graphs.clear();
 }
 
+// slotCoef: row-slot coefficients of this operator (nullptr: none staged); used
+// under spmvSchedule 2 when the mesh allows the row-slot layout.
 template <typename T>
 LduOp<T> make_op(hipfoam_context* c, const T* diag, const T* upper, const T* lowerL,
-                 const T* lowerF) {
+                 const T* lowerF, const T* slotCoef) {
   LduOp<T> op;
   op.ownStart = c->ownStart.p;
   op.losortStart = c->losortStart.p;
@@ -245,6 +250,9 @@ LduOp<T> make_op(hipfoam_context* c, const T* diag, const T* upper, const T* low
   op.xcdRemap = c->sched.xcdRemap;
   op.shortRows = c->maxOwn <= kShortRow && c->maxNbr <= kShortRow && c->sched.rowForm == 0;
   op.direct = c->sched.spmvSchedule == 1;
+  op.slots = slotCoef && c->sched.spmvSchedule == 2 && c->slotOK;
+  op.slotCoef = op.slots ? slotCoef : nullptr;
+  op.slotOff = op.slots ? c->slotOff.p : nullptr;
   if (op.direct) {
     op.nbrAddr = c->lowAddr.p;
     op.lowerL = lowerF;
@@ -257,12 +265,12 @@ LduOp<T> make_op(hipfoam_context* c, const T* diag, const T* upper, const T* low
 
 LduOp<float> op32(hipfoam_context* c) {
   return make_op<float>(c, c->diag32.p, c->upper32.p, c->lowerL32.p,
-                        c->sym32 ? c->upper32.p : c->lower32.p);
+                        c->sym32 ? c->upper32.p : c->lower32.p, c->slotCoef32.p);
 }
 
 LduOp<double> op64(hipfoam_context* c) {
   return make_op<double>(c, c->diag64.p, c->upper64.p, c->lowerL64.p,
-                         c->sym64 ? c->upper64.p : c->lower64.p);
+                         c->sym64 ? c->upper64.p : c->lower64.p, nullptr);
 }
 
 // Host copy of the Chebyshev constants (encCheb, metalFoamBackend.C:349-372).
@@ -428,7 +436,7 @@ void enqueue_chunk(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD
 template <typename T>
 int get_graph(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
               int chunk, hipGraphExec_t* out) {
-  GraphKey key{(int)kind, (int)sizeof(T), order, chunk, A.direct, A.xcdRemap,
+  GraphKey key{(int)kind, (int)sizeof(T), order, chunk, A.direct + 2 * A.slots, A.xcdRemap,
                c->sched.blockSize, row_grid(c), A.shortRows, c->gen};
   auto it = c->graphs.find(key);
   if (it != c->graphs.end()) {
@@ -615,6 +623,13 @@ int stage_matrix(hipfoam_context* c, const double* diag, const double* upper,
                        c->losort.p, c->upper64.p, lower ? c->lower64.p : nullptr,
                        c->lowerL64.p, c->upper32.p, c->lowerL32.p,
                        lower ? c->lower32.p : nullptr);
+    CKL();
+  }
+  if (c->slotOK && n > 0) {
+    CK(c->slotCoef32.ensure((size_t)2 * kShortRow * n));
+    hipLaunchKernelGGL((k_build_slot_coeffs<float>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                       c->stream, n, c->ownStart.p, c->losortStart.p, c->upper32.p,
+                       c->lowerL32.p, c->slotCoef32.p);
     CKL();
   }
   c->have64 = true;
@@ -1112,7 +1127,8 @@ int32_t hipfoam_create(int32_t device, hipfoam_context** out) {
   (void)hipMemset(c->ost.p, 0, sizeof(OState));
   (void)hipDeviceSynchronize();
   char buf[512];
-  snprintf(buf, sizeof(buf), "hipfoam: %s (%s, %d CUs, %.0f GiB HBM)", prop.name,
+  snprintf(buf, sizeof(buf), "hipfoam: device %d %s(%s, %d CUs, %.0f GiB HBM)", device,
+           prop.name[0] ? (std::string(prop.name) + " ").c_str() : "",
            prop.gcnArchName, prop.multiProcessorCount,
            (double)prop.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
   c->desc = buf;
@@ -1188,6 +1204,10 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
     c->maxOwn = std::max(c->maxOwn, ownStart[i + 1] - ownStart[i]);
     c->maxNbr = std::max(c->maxNbr, losortStart[i + 1] - losortStart[i]);
   }
+  int64_t maxSpan = 0;
+  for (int32_t f = 0; f < nFaces; ++f)
+    maxSpan = std::max(maxSpan, std::abs((int64_t)uppAddr[f] - (int64_t)lowAddr[f]));
+  c->slotOK = c->maxOwn <= kShortRow && c->maxNbr <= kShortRow && maxSpan <= 32767;
   c->lastLow = lowAddr;
   c->lastUpp = uppAddr;
   c->have32 = c->have64 = false;
@@ -1211,6 +1231,13 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
                       c->stream));
     hipLaunchKernelGGL(k_gather_addr, dim3(blocks_for(nFaces, 256)), dim3(256), 0, c->stream,
                        nFaces, c->losort.p, c->lowAddr.p, c->losortLow.p);
+    CKL();
+  }
+  if (c->slotOK && nCells > 0) {
+    CK(c->slotOff.ensure((size_t)kShortRow * nCells));
+    hipLaunchKernelGGL(k_build_slot_offsets, dim3(blocks_for(nCells, 256)), dim3(256), 0,
+                       c->stream, nCells, c->ownStart.p, c->uppAddr.p, c->losortStart.p,
+                       c->losortLow.p, c->slotOff.p);
     CKL();
   }
   c->pstride = std::max(1, blocks_for(nCells, 64));
@@ -1488,11 +1515,18 @@ int32_t hipfoam_get_schedule(const hipfoam_context* c, hipfoam_schedule* s) {
   return HIPFOAM_OK;
 }
 
+int32_t hipfoam_active_schedule(const hipfoam_context* c, int32_t* spmvSchedule) {
+  if (!c || !spmvSchedule) return HIPFOAM_ERR_ARG;
+  const int s = c->sched.spmvSchedule;
+  *spmvSchedule = (s == 2 && !c->slotOK) ? 0 : s;
+  return HIPFOAM_OK;
+}
+
 int32_t hipfoam_set_schedule(hipfoam_context* c, const hipfoam_schedule* s) {
   FpTrapGuard fpe;
   if (!c || !s) return HIPFOAM_ERR_ARG;
-  if (s->spmvSchedule < 0 || s->spmvSchedule > 1)
-    return fail(c, HIPFOAM_ERR_ARG, "spmvSchedule must be 0 or 1");
+  if (s->spmvSchedule < 0 || s->spmvSchedule > 2)
+    return fail(c, HIPFOAM_ERR_ARG, "spmvSchedule must be 0, 1 or 2");
   if (s->blockSize < 64 || s->blockSize > 1024 || s->blockSize % 64)
     return fail(c, HIPFOAM_ERR_ARG, "blockSize must be a multiple of 64 in [64, 1024]");
   if (s->graphChunk < 0 || s->graphChunk > 4096 || (s->graphChunk & 1))
@@ -1526,7 +1560,7 @@ int32_t hipfoam_autotune(hipfoam_context* c, hipfoam_schedule* best) {
   const int reps = 20;
   const int cus = std::max(c->numCUs, 1);
   for (int form = 0; form <= 1; ++form)
-  for (int sch = 0; sch <= 1; ++sch)
+  for (int sch = 0; sch <= (c->slotOK ? 2 : 1); ++sch)
     for (int bs : {128, 256, 512, 1024})
       for (int rm = 0; rm <= 1; ++rm)
         for (int gl : {0, 4 * cus, 16 * cus, 1 << 30}) {

@@ -84,10 +84,17 @@ struct LduOp {
   const T* __restrict__ diag;
   const T* __restrict__ upper;
   const T* __restrict__ lowerL;  // staged: lower[losort[k]]; direct: lower[f]
+  // Row-slot layout (slots != 0): row i's owner faces in slots 0..2 and its
+  // neighbour faces in slots 3..5, in the order the face loops visit them.
+  // slotCoef[q*n + i] is the coefficient of slot q; slotOff[p*n + i] packs the
+  // column offsets (column - i, int16; 0 = empty slot) of slots 2p and 2p+1.
+  const T* __restrict__ slotCoef;
+  const uint32_t* __restrict__ slotOff;
   int32_t n;
   int32_t direct;
   int32_t xcdRemap;
   int32_t shortRows;  // every row has <= kShortRow owner and <= kShortRow neighbour faces
+  int32_t slots;
 };
 
 // Rows of hexahedral meshes have at most 3 owner and 3 neighbour faces.
@@ -230,6 +237,30 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
 // added in exactly the order of the loop form, so both give identical bits.
 template <typename T, typename XF>
 __device__ __forceinline__ T ldu_row(const LduOp<T>& A, int i, T xi, XF xf) {
+  if (A.slots) {
+    // coefficients and offsets are independent coalesced loads (no row-start
+    // indirection); empty slots are skipped, so the terms and their order are
+    // those of the face loops below
+    const size_t n = (size_t)A.n;
+    T s = A.diag[i] * xi;
+    T cq[2 * kShortRow], xq[2 * kShortRow];
+    int oq[2 * kShortRow];
+#pragma unroll
+    for (int p = 0; p < kShortRow; ++p) {
+      const uint32_t w = A.slotOff[p * n + i];
+      oq[2 * p] = (int)(int16_t)(w & 0xffffu);
+      oq[2 * p + 1] = (int)(int16_t)(w >> 16);
+    }
+#pragma unroll
+    for (int q = 0; q < 2 * kShortRow; ++q) cq[q] = A.slotCoef[q * n + i];
+#pragma unroll
+    for (int q = 0; q < 2 * kShortRow; ++q)
+      if (oq[q] != 0) xq[q] = xf(i + oq[q]);
+#pragma unroll
+    for (int q = 0; q < 2 * kShortRow; ++q)
+      if (oq[q] != 0) s += cq[q] * xq[q];
+    return s;
+  }
   const int f0 = A.ownStart[i], f1 = A.ownStart[i + 1];
   const int k0 = A.losortStart[i], k1 = A.losortStart[i + 1];
   T s = A.diag[i] * xi;
@@ -812,6 +843,48 @@ __global__ __launch_bounds__(1024) void k_gather_addr(int nf, const int32_t* __r
                                                       int32_t* __restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < nf) out[k] = low[losort[k]];
+}
+
+// Row-slot column offsets (layout of LduOp::slotOff) from the LDU addressing.
+// Requires every row to have <= kShortRow owner and neighbour faces and every
+// |column - row| <= 32767; both are checked on the host.
+__global__ __launch_bounds__(256) void k_build_slot_offsets(
+    int n, const int32_t* __restrict__ ownStart, const int32_t* __restrict__ uppAddr,
+    const int32_t* __restrict__ losortStart, const int32_t* __restrict__ losortLow,
+    uint32_t* __restrict__ slotOff) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int off[2 * kShortRow];
+  const int f0 = ownStart[i], f1 = ownStart[i + 1];
+  const int k0 = losortStart[i], k1 = losortStart[i + 1];
+#pragma unroll
+  for (int q = 0; q < kShortRow; ++q) {
+    off[q] = f0 + q < f1 ? uppAddr[f0 + q] - i : 0;
+    off[kShortRow + q] = k0 + q < k1 ? losortLow[k0 + q] - i : 0;
+  }
+#pragma unroll
+  for (int p = 0; p < kShortRow; ++p)
+    slotOff[(size_t)p * n + i] =
+        ((uint32_t)(uint16_t)(int16_t)off[2 * p]) |
+        ((uint32_t)(uint16_t)(int16_t)off[2 * p + 1] << 16);
+}
+
+// Row-slot coefficients (layout of LduOp::slotCoef) from the face-ordered
+// upper and losort-ordered lower coefficients; empty slots hold 0.
+template <typename T>
+__global__ __launch_bounds__(256) void k_build_slot_coeffs(
+    int n, const int32_t* __restrict__ ownStart, const int32_t* __restrict__ losortStart,
+    const T* __restrict__ upper, const T* __restrict__ lowerL, T* __restrict__ slotCoef) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int f0 = ownStart[i], f1 = ownStart[i + 1];
+  const int k0 = losortStart[i], k1 = losortStart[i + 1];
+  const size_t nn = (size_t)n;
+#pragma unroll
+  for (int q = 0; q < kShortRow; ++q) {
+    slotCoef[q * nn + i] = f0 + q < f1 ? upper[f0 + q] : T(0);
+    slotCoef[(kShortRow + q) * nn + i] = k0 + q < k1 ? lowerL[k0 + q] : T(0);
+  }
 }
 
 template <typename T>

@@ -260,6 +260,12 @@ class Context:
         self._check(self.lib.hipfoam_get_schedule(self.h, C.byref(s)), "get_schedule")
         return s
 
+    def active_schedule(self) -> int:
+        """SpMV schedule the current mesh's inner operator runs (0, 1 or 2)."""
+        v = C.c_int32()
+        self._check(self.lib.hipfoam_active_schedule(self.h, C.byref(v)), "active_schedule")
+        return v.value
+
     def set_schedule(self, **kw):
         s = self.get_schedule()
         for k, v in kw.items():

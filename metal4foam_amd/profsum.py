@@ -17,7 +17,13 @@ entry, and would otherwise dilute the averages.
 
 FETCH_SIZE / WRITE_SIZE (KiB per dispatch) are scaled by what the same
 counter reads for k_copy<double>, a copy of known size (8 B read + 8 B
-written per element).
+written per element).  Counter values below 5 % of a kernel's largest
+dispatch are the stopped launches and are left out of its mean.
+
+--layout slots prices the FP32 SpMV kernels on the row-slot layout (six
+FP32 coefficients and three packed int16 offset words per row instead of the
+row-start arrays and per-face arrays).  --traffic-json writes the calibrated
+HBM bytes per dispatch of one kernel (--traffic-kernel) as JSON.
 """
 
 from __future__ import annotations
@@ -28,8 +34,16 @@ import csv
 import re
 
 
-def _bytes(name: str, n: int, nf: int, asym: bool) -> int | None:
+def _bytes(name: str, n: int, nf: int, asym: bool, slots: bool = False) -> int | None:
     """Algorithmic bytes per launch of the kernels on the solve path."""
+    b = _ldu_bytes(name, n, nf, asym)
+    if b is None or not slots or "<float" not in name or "k_pcg_update" in name:
+        return b
+    face32 = 20 if asym else 16
+    return b - (8 * n + 8 + face32 * nf) + 36 * n
+
+
+def _ldu_bytes(name: str, n: int, nf: int, asym: bool) -> int | None:
     face32 = 20 if asym else 16
     face64 = 28 if asym else 24
     if "k_cheb_step<float, false, false>" in name:
@@ -73,7 +87,12 @@ def main():
     ap.add_argument("--title", default="rocprofv3 kernel summary")
     ap.add_argument("--command", action="append", default=[])
     ap.add_argument("--peak-gbs", type=float, default=8000.0)
+    ap.add_argument("--layout", choices=["faces", "slots"], default="faces")
+    ap.add_argument("--traffic-json")
+    ap.add_argument("--traffic-kernel", default="k_cheb_step<float, false, false>")
+    ap.add_argument("--workload", default="")
     a = ap.parse_args()
+    slots = a.layout == "slots"
 
     durs = collections.defaultdict(list)
     for r in csv.DictReader(open(a.trace)):
@@ -84,6 +103,9 @@ def main():
         if path:
             for r in csv.DictReader(open(path)):
                 out[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        for k, v in out.items():
+            top = max(v)
+            out[k] = [x for x in v if x >= 0.05 * top] if top > 0 else v
         return out
 
     fetch, write = counters(a.fetch), counters(a.write)
@@ -118,7 +140,7 @@ def main():
         real = [x for x in v if x >= a.noop_us]
         stopped = len(v) - len(real)
         avg = mean(real) if real else mean(v)
-        b = _bytes(name, a.cells, a.faces, a.asym)
+        b = _bytes(name, a.cells, a.faces, a.asym, slots)
         gbs = b / (avg * 1e3) if (b and real) else None
         tr = None
         if fscale and name in fetch and name in write:
@@ -128,6 +150,31 @@ def main():
               f"{'' if gbs is None else f'{100 * gbs / a.peak_gbs:.1f}'} | "
               f"{'' if tr is None else f'{tr / 1e6:.1f}'} | "
               f"{'' if (tr is None or b is None) else f'{tr / b:.2f}'} |")
+
+
+    if a.traffic_json and fscale:
+        import json
+
+        names = [k for k in fetch if a.traffic_kernel in k and k in write]
+        if names:
+            k = names[0]
+            f_kib, w_kib = mean(fetch[k]), mean(write[k])
+            out = {
+                "kernel": a.traffic_kernel,
+                "workload": a.workload,
+                "layout": "row-slot" if slots else "LDU face",
+                "fetch_kib_per_dispatch": round(f_kib, 1),
+                "write_kib_per_dispatch": round(w_kib, 1),
+                "calibration": {"kernel": "k_copy<double>", "bytes_read": 8 * a.cells,
+                                "bytes_written": 8 * a.cells, "fetch_scale": round(fscale, 4),
+                                "write_scale": round(wscale, 4)},
+                "bytes_per_launch": int(round((f_kib * fscale + w_kib * wscale) * 1024.0)),
+                "algorithmic_bytes_per_launch": _bytes(k, a.cells, a.faces, a.asym, slots),
+                "commands": a.command,
+            }
+            with open(a.traffic_json, "w") as fh:
+                json.dump(out, fh, indent=1)
+                fh.write("\n")
 
 
 if __name__ == "__main__":

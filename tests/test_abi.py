@@ -61,7 +61,9 @@ def test_bad_device_gives_context_that_is_not_good():
 def test_schedule_validation_without_device():
     ctx = H.Context(device=999, require=False)
     s = ctx.get_schedule()
-    assert (s.spmvSchedule, s.blockSize, s.xcdRemap) == (0, 256, 1)
+    assert (s.spmvSchedule, s.blockSize, s.xcdRemap) == (2, 256, 1)
+    bad = _lib.Schedule(3, 256, 1, 8, 0)
+    assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(bad)) == _lib.ERR_ARG
     bad = _lib.Schedule(0, 100, 1, 8, 0)
     assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(bad)) == _lib.ERR_ARG
     bad = _lib.Schedule(0, 256, 1, 7, 0)

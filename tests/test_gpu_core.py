@@ -10,7 +10,7 @@ from metal4foam_amd import meshgen as mg
 pytestmark = pytest.mark.gpu
 
 SCHEDULES = [dict(spmvSchedule=s, blockSize=b, xcdRemap=r, gridLimit=g, rowForm=f)
-             for s in (0, 1) for b in (64, 256, 1024) for r in (0, 1) for g in (0, 7)
+             for s in (0, 1, 2) for b in (64, 256, 1024) for r in (0, 1) for g in (0, 7)
              for f in (0, 1)]
 
 
@@ -27,7 +27,7 @@ def rand_asym(shape, seed):
 
 @pytest.fixture(autouse=True)
 def _default_schedule(ctx):
-    ctx.set_schedule(spmvSchedule=0, blockSize=256, xcdRemap=1, graphChunk=8, gridLimit=0,
+    ctx.set_schedule(spmvSchedule=2, blockSize=256, xcdRemap=1, graphChunk=8, gridLimit=0,
                      rowForm=0)
     yield
 
@@ -310,7 +310,7 @@ def test_autotune_picks_a_valid_schedule(ctx):
     ctx.set_topology(m)
     ctx.set_matrix(m.diag, m.upper, None)
     s = ctx.autotune()
-    assert s.spmvSchedule in (0, 1) and s.blockSize in (128, 256, 512, 1024)
+    assert s.spmvSchedule in (0, 1, 2) and s.blockSize in (128, 256, 512, 1024)
     psi = np.zeros(m.n)
     perf = ctx.solve(H.PCG, m.diag, m.upper, None, psi, m.source)
     assert perf.converged
