@@ -236,8 +236,10 @@ def main():
         ref_kbytes = cheb_step_bytes(m.n, m.nfaces, sym)
         kname = "k_cheb_step<float, false, false> (interior Chebyshev step fused with its SpMV)"
     else:
-        ref_kbytes = spmv_bytes(m.n, m.nfaces, sym, extra_vectors=2)
-        kname = "k_bicg_spmv_alpha / k_bicg_z_spmv (BiCGStab SpMV + dot)"
+        # mean of k_bicg_py_spmv (reads r, rD, p, AyA, rA0; writes p, yA, AyA)
+        # and k_bicg_sz_spmv (reads r, rD, AyA; writes sA, zA, tA)
+        ref_kbytes = spmv_bytes(m.n, m.nfaces, sym, extra_vectors=5)
+        kname = "k_bicg_py_spmv / k_bicg_sz_spmv (BiCGStab SpMVs fused with their vector updates)"
     kbytes = to_slot_bytes(ref_kbytes, m.n, m.nfaces, sym) if slots else ref_kbytes
     kavg = secs / max(nl, 1)
     kgbs = kbytes / kavg / 1e9 if nl else 0.0

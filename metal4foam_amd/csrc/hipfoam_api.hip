@@ -73,12 +73,12 @@ struct Buf {
 // Krylov workspace of one precision.
 template <typename T>
 struct Work {
-  Buf<T> r, p0, p1, wA, Ap, delta, deltaBest, z0, z1, w, rA0, yA, AyA, sA, zA, tA;
+  Buf<T> r, p0, p1, wA, Ap, delta, deltaBest, z0, z1, w, rA0, yA, AyA, AyA1, sA, zA, tA;
   size_t n = 0;
   hipError_t ensure(size_t nn) {
     if (n == nn && r.p) return hipSuccess;
     Buf<T>* all[] = {&r, &p0, &p1, &wA, &Ap, &delta, &deltaBest, &z0,
-                     &z1, &w, &rA0, &yA, &AyA, &sA, &zA, &tA};
+                     &z1, &w, &rA0, &yA, &AyA, &AyA1, &sA, &zA, &tA};
     for (auto* b : all) {
       b->release();
       hipError_t e = b->ensure(nn);
@@ -89,7 +89,7 @@ struct Work {
   }
   void release() {
     Buf<T>* all[] = {&r, &p0, &p1, &wA, &Ap, &delta, &deltaBest, &z0,
-                     &z1, &w, &rA0, &yA, &AyA, &sA, &zA, &tA};
+                     &z1, &w, &rA0, &yA, &AyA, &AyA1, &sA, &zA, &tA};
     for (auto* b : all) b->release();
     n = 0;
   }
@@ -133,6 +133,7 @@ struct hipfoam_context {
   bool slotOK = false;  // row-slot layout possible: short rows, |column - row| <= 32767
   Buf<uint32_t> slotOff;
   Buf<float> slotCoef32;
+  Buf<double> slotCoef64;
   const int32_t* lastLow = nullptr;
   const int32_t* lastUpp = nullptr;
   int gen = 0;  // bumps whenever device buffers are reallocated
@@ -270,7 +271,7 @@ LduOp<float> op32(hipfoam_context* c) {
 
 LduOp<double> op64(hipfoam_context* c) {
   return make_op<double>(c, c->diag64.p, c->upper64.p, c->lowerL64.p,
-                         c->sym64 ? c->upper64.p : c->lower64.p, nullptr);
+                         c->sym64 ? c->upper64.p : c->lower64.p, c->slotCoef64.p);
 }
 
 // Host copy of the Chebyshev constants (encCheb, metalFoamBackend.C:349-372).
@@ -392,7 +393,8 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
 }
 
 template <typename T>
-void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, bool capturing) {
+void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int parity,
+                       bool capturing) {
   Work<T>& w = work<T>(c);
   const int bs = c->sched.blockSize;
   const dim3 grid(row_grid(c)), block(bs);
@@ -401,23 +403,32 @@ void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, bool 
   const int ps = c->pstride;
   const int n = c->n, rm = A.xcdRemap;
   const int nb = grid.x;
-  hipLaunchKernelGGL((k_bicg_rho<T>), grid, block, 0, c->stream, n, rm, w.rA0.p, w.r.p, st, P,
-                     ps);
-  launch_fin<1>(c, nb, true, kRunning, FinBicgRho<T>{st});
-  hipLaunchKernelGGL((k_bicg_p_y<T>), grid, block, 0, c->stream, n, rm, w.p0.p, w.r.p, w.AyA.p,
-                     rD, w.yA.p, st);
-  LAUNCH_CLASSED(2, capturing, (k_bicg_spmv_alpha<T>), grid, block, A, w.yA.p, w.rA0.p,
-                 w.AyA.p, st, P, ps);
+  T* pold = parity ? w.p1.p : w.p0.p;
+  T* pnew = parity ? w.p0.p : w.p1.p;
+  T* AyAold = parity ? w.AyA1.p : w.AyA.p;
+  T* AyAnew = parity ? w.AyA.p : w.AyA1.p;
+  LAUNCH_CLASSED(2, capturing, (k_bicg_py_spmv<T>), grid, block, A, w.r.p, rD, pold, pnew,
+                 AyAold, AyAnew, w.yA.p, w.rA0.p, st, P, ps);
   launch_fin<1>(c, nb, true, kRunning, FinBicgAlpha<T>{st});
-  hipLaunchKernelGGL((k_bicg_s<T>), grid, block, 0, c->stream, n, rm, w.sA.p, w.r.p, w.AyA.p,
-                     st, P, ps);
-  launch_fin<1>(c, nb, true, kRunning, FinBicgS<T>{st});
-  LAUNCH_CLASSED(2, capturing, (k_bicg_z_spmv<T>), grid, block, A, rD, w.sA.p, w.zA.p, w.tA.p,
-                 st, P, ps);
-  launch_fin<2>(c, nb, true, kRunning, FinBicgOmega<T>{st});
-  hipLaunchKernelGGL((k_bicg_update<T>), grid, block, 0, c->stream, n, rm, w.delta.p,
-                     w.deltaBest.p, w.yA.p, w.zA.p, w.r.p, w.sA.p, w.tA.p, st, P, ps);
-  launch_fin<1>(c, nb, true, kRunningOrEarly, FinBicgUpdate<T>{st});
+  LAUNCH_CLASSED(2, capturing, (k_bicg_sz_spmv<T>), grid, block, A, w.r.p, rD, AyAnew, w.sA.p,
+                 w.zA.p, w.tA.p, st, P, ps);
+  launch_fin<3>(c, nb, true, kRunning, FinBicgSOmega<T>{st});
+  hipLaunchKernelGGL((k_bicg_update_rho<T>), grid, block, 0, c->stream, n, rm, w.delta.p,
+                     w.deltaBest.p, w.yA.p, w.zA.p, w.r.p, w.sA.p, w.tA.p, w.rA0.p, st, P, ps);
+  launch_fin<2>(c, nb, true, kRunningOrEarly, FinBicgUpdateRho<T>{st});
+}
+
+// rho = rA0.rA and beta of the first BiCGStab iteration (later iterations
+// reduce rho in their update kernel); *kst must be initialised.
+template <typename T>
+int enqueue_bicg_start(hipfoam_context* c, const LduOp<T>& A) {
+  Work<T>& w = work<T>(c);
+  const dim3 grid(row_grid(c)), block(c->sched.blockSize);
+  hipLaunchKernelGGL((k_bicg_rho<T>), grid, block, 0, c->stream, c->n, A.xcdRemap, w.rA0.p,
+                     w.r.p, c->kst.p, c->partials.p, c->pstride);
+  launch_fin<1>(c, grid.x, true, kRunning, FinBicgRho<T>{c->kst.p});
+  CKL();
+  return HIPFOAM_OK;
 }
 
 // Iteration `first + k` uses p-buffer parity (first + k) & 1; graphs are
@@ -429,7 +440,7 @@ void enqueue_chunk(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD
     if (kind == KIND_PCG)
       enqueue_pcg_iter<T>(c, A, rD, order, (int)((first + k) & 1), capturing);
     else
-      enqueue_bicg_iter<T>(c, A, rD, capturing);
+      enqueue_bicg_iter<T>(c, A, rD, (int)((first + k) & 1), capturing);
   }
 }
 
@@ -555,6 +566,10 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
   double gpu = 0;
   if (s.status == ST_RUNNING) {
     const LduOp<float> A = op32(c);
+    if (kind == KIND_BICG) {
+      int rc = enqueue_bicg_start<float>(c, A);
+      if (rc) return rc;
+    }
     int rc = run_loop<float>(c, kind, A, c->rD32.p, kind == KIND_PCG ? order : 1, maxIter, &gpu);
     if (rc) return rc;
   } else {
@@ -627,9 +642,13 @@ int stage_matrix(hipfoam_context* c, const double* diag, const double* upper,
   }
   if (c->slotOK && n > 0) {
     CK(c->slotCoef32.ensure((size_t)2 * kShortRow * n));
+    CK(c->slotCoef64.ensure((size_t)2 * kShortRow * n));
     hipLaunchKernelGGL((k_build_slot_coeffs<float>), dim3(blocks_for(n, 256)), dim3(256), 0,
                        c->stream, n, c->ownStart.p, c->losortStart.p, c->upper32.p,
                        c->lowerL32.p, c->slotCoef32.p);
+    hipLaunchKernelGGL((k_build_slot_coeffs<double>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                       c->stream, n, c->ownStart.p, c->losortStart.p, c->upper64.p,
+                       c->lowerL64.p, c->slotCoef64.p);
     CKL();
   }
   c->have64 = true;
@@ -896,6 +915,10 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
     const LduOp<double> A = op64(c);
     double gpu = 0;
     const double tw = now_seconds();
+    if (useBiCG) {
+      rc = enqueue_bicg_start<double>(c, A);
+      if (rc) return rc;
+    }
     rc = run_loop<double>(c, useBiCG ? KIND_BICG : KIND_PCG, A, c->rD64.p, order,
                           std::max(ctl->maxIter, ctl->minIter), &gpu);
     if (rc) return rc;

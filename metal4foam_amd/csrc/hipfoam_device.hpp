@@ -512,103 +512,97 @@ __global__ __launch_bounds__(1024) void k_bicg_rho(int n, int xcdRemap, const T*
   block_partials<1>(v, partials, pstride, lb);
 }
 
-// p = rA + beta (p - omega AyA); yA = rD p   (bicg_update_p + emul)
-template <typename T>
-__global__ __launch_bounds__(1024) void k_bicg_p_y(int n, int xcdRemap, T* __restrict__ p,
-                                                   const T* __restrict__ r,
-                                                   const T* __restrict__ AyA,
-                                                   const T* __restrict__ rD, T* __restrict__ yA,
-                                                   KState* st) {
-  if (st->status != ST_RUNNING) return;
-  const int lb = logical_block(xcdRemap);
-  int r0, r1;
-  block_rows(n, lb, r0, r1);
-  const T beta = (T)st->beta, omega = (T)st->omega;
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-    const T pi = r[i] + beta * (p[i] - omega * AyA[i]);
-    p[i] = pi;
-    yA[i] = rD[i] * pi;
-  }
-}
+// One BiCGStab iteration is three row kernels, each followed by a one-
+// workgroup finalise:
+//   k_bicg_py_spmv   p = rA + beta (p_old - omega AyA_old), yA = rD p,
+//                    AyA = A yA, partial rA0.AyA          -> FinBicgAlpha
+//   k_bicg_sz_spmv   sA = rA - alpha AyA, zA = rD sA, tA = A zA,
+//                    partials |sA|, tA.sA, tA.tA          -> FinBicgSOmega
+//   k_bicg_update_rho delta += alpha yA + omega zA, rA = sA - omega tA,
+//                    partials |rA|, rA0.rA                -> FinBicgUpdateRho
+// (bicg_update_p, emul, ldu_spmv_dotw, bicg_s_res, dot_partial,
+// bicg_update_sol_res and reduce_scalars modes 3-5 of
+// metalFoamKernels.metal:279-458).  The SpMV kernels form the gathered
+// neighbour values on the fly from the vectors they derive from, so p and
+// AyA ping-pong between iterations; rho of the next iteration is reduced in
+// the update kernel.  Every value is the same float expression the separate
+// kernels evaluate, so the results are the same.
 
-// AyA = A yA, partial rA0.AyA, alpha (ldu_spmv_dotw w = rA0; mode 4)
 template <typename T>
-__global__ __launch_bounds__(1024) void k_bicg_spmv_alpha(LduOp<T> A, const T* __restrict__ yA,
-                                                          const T* __restrict__ rA0,
-                                                          T* __restrict__ AyA, KState* st,
-                                                          double* partials, int pstride) {
+__global__ __launch_bounds__(1024) void k_bicg_py_spmv(LduOp<T> A, const T* __restrict__ r,
+                                                       const T* __restrict__ rD,
+                                                       const T* __restrict__ pold,
+                                                       T* __restrict__ pnew,
+                                                       const T* __restrict__ AyAold,
+                                                       T* __restrict__ AyAnew,
+                                                       T* __restrict__ yA,
+                                                       const T* __restrict__ rA0, KState* st,
+                                                       double* partials, int pstride) {
   if (st->status != ST_RUNNING) return;
   const int lb = logical_block(A.xcdRemap);
   int r0, r1;
   block_rows(A.n, lb, r0, r1);
+  const T beta = (T)st->beta, omega = (T)st->omega;
   double d = 0.0;
   for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-    const T y = ldu_row(A, i, yA[i], [&](int j) { return yA[j]; });
-    AyA[i] = y;
+    const T pi = r[i] + beta * (pold[i] - omega * AyAold[i]);
+    const T yi = rD[i] * pi;
+    const T y = ldu_row(A, i, yi, [&](int j) {
+      const T pj = r[j] + beta * (pold[j] - omega * AyAold[j]);
+      return rD[j] * pj;
+    });
+    pnew[i] = pi;
+    yA[i] = yi;
+    AyAnew[i] = y;
     d += (double)(rA0[i] * y);
   }
   double v[1] = {d};
   block_partials<1>(v, partials, pstride, lb);
 }
 
-// sA = rA - alpha AyA, partial |sA|; breakdown and early-exit tests
-// (bicg_s_res; metalFoamBackend.C:828-866).
 template <typename T>
-__global__ __launch_bounds__(1024) void k_bicg_s(int n, int xcdRemap, T* __restrict__ sA,
-                                                 const T* __restrict__ r,
-                                                 const T* __restrict__ AyA, KState* st,
-                                                 double* partials, int pstride) {
-  if (st->status != ST_RUNNING) return;
-  const int lb = logical_block(xcdRemap);
-  int r0, r1;
-  block_rows(n, lb, r0, r1);
-  const T alpha = (T)st->alpha;
-  double m = 0.0;
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-    const T s = r[i] - alpha * AyA[i];
-    sA[i] = s;
-    m += (double)fabs(s);
-  }
-  double v[1] = {m};
-  block_partials<1>(v, partials, pstride, lb);
-}
-
-// zA = rD sA (own row) and tA = A zA with zA gathered on the fly; partials
-// tA.sA and tA.tA; omega (emul + ldu_spmv_dotw w = sA + dot_partial; mode 5).
-template <typename T>
-__global__ __launch_bounds__(1024) void k_bicg_z_spmv(LduOp<T> A, const T* __restrict__ rD,
-                                                      const T* __restrict__ sA,
-                                                      T* __restrict__ zA, T* __restrict__ tA,
-                                                      KState* st, double* partials,
-                                                      int pstride) {
+__global__ __launch_bounds__(1024) void k_bicg_sz_spmv(LduOp<T> A, const T* __restrict__ r,
+                                                       const T* __restrict__ rD,
+                                                       const T* __restrict__ AyA,
+                                                       T* __restrict__ sA, T* __restrict__ zA,
+                                                       T* __restrict__ tA, KState* st,
+                                                       double* partials, int pstride) {
   if (st->status != ST_RUNNING) return;
   const int lb = logical_block(A.xcdRemap);
   int r0, r1;
   block_rows(A.n, lb, r0, r1);
-  double d[2] = {0.0, 0.0};
+  const T alpha = (T)st->alpha;
+  double d[3] = {0.0, 0.0, 0.0};
   for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-    const T zi = rD[i] * sA[i];
-    const T t = ldu_row(A, i, zi, [&](int j) { return rD[j] * sA[j]; });
+    const T si = r[i] - alpha * AyA[i];
+    const T zi = rD[i] * si;
+    const T t = ldu_row(A, i, zi, [&](int j) {
+      const T sj = r[j] - alpha * AyA[j];
+      return rD[j] * sj;
+    });
+    sA[i] = si;
     zA[i] = zi;
     tA[i] = t;
-    d[0] += (double)(sA[i] * t);
-    d[1] += (double)(t * t);
+    d[0] += (double)fabs(si);
+    d[1] += (double)(si * t);
+    d[2] += (double)(t * t);
   }
-  block_partials<2>(d, partials, pstride, lb);
+  block_partials<3>(d, partials, pstride, lb);
 }
 
-// delta += alpha yA + omega zA; rA = sA - omega tA; partial |rA|
-// (bicg_update_sol_res); or, after an |sA| exit, delta += alpha yA
+// After an |sA| exit (status EARLY) only delta += alpha yA is applied
 // (bicg_early_update, metalFoamKernels.metal:415-426).
 template <typename T>
-__global__ __launch_bounds__(1024) void k_bicg_update(int n, int xcdRemap, T* __restrict__ delta,
-                                                      T* __restrict__ deltaBest,
-                                                      const T* __restrict__ yA,
-                                                      const T* __restrict__ zA,
-                                                      T* __restrict__ r,
-                                                      const T* __restrict__ sA,
-                                                      const T* __restrict__ tA, KState* st,
-                                                      double* partials, int pstride) {
+__global__ __launch_bounds__(1024) void k_bicg_update_rho(int n, int xcdRemap,
+                                                          T* __restrict__ delta,
+                                                          T* __restrict__ deltaBest,
+                                                          const T* __restrict__ yA,
+                                                          const T* __restrict__ zA,
+                                                          T* __restrict__ r,
+                                                          const T* __restrict__ sA,
+                                                          const T* __restrict__ tA,
+                                                          const T* __restrict__ rA0, KState* st,
+                                                          double* partials, int pstride) {
   const int status = st->status;
   if (status != ST_RUNNING && status != ST_EARLY) return;
   const int lb = logical_block(xcdRemap);
@@ -617,21 +611,21 @@ __global__ __launch_bounds__(1024) void k_bicg_update(int n, int xcdRemap, T* __
   const T alpha = (T)st->alpha;
   if (status == ST_EARLY) {
     for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) delta[i] += alpha * yA[i];
-    return;  // FinBicgUpdate completes the |sA| exit
+    return;  // FinBicgUpdateRho completes the |sA| exit
   }
   const T omega = (T)st->omega;
   const bool copy = st->copyBest != 0;
-  double m = 0.0;
+  double v[2] = {0.0, 0.0};
   for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const T d = delta[i];
     if (copy) deltaBest[i] = d;
     delta[i] = d + (alpha * yA[i] + omega * zA[i]);
     const T ri = sA[i] - omega * tA[i];
     r[i] = ri;
-    m += (double)fabs(ri);
+    v[0] += (double)fabs(ri);
+    v[1] += (double)(rA0[i] * ri);
   }
-  double v[1] = {m};
-  block_partials<1>(v, partials, pstride, lb);
+  block_partials<2>(v, partials, pstride, lb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1079,6 +1073,28 @@ struct FinBicgUpdate {
       st->res = t[0] / st->normFactor;
       openfoam_iteration_end(st);
     }
+  }
+};
+
+// sum|sA| (breakdown and early-exit tests), then tA.sA, tA.tA and omega
+// when the iteration goes on: FinBicgS followed by FinBicgOmega.
+template <typename T>
+struct FinBicgSOmega {
+  KState* st;
+  __device__ void operator()(const double* t) const {
+    FinBicgS<T>{st}(t);
+    if (st->status == ST_RUNNING) FinBicgOmega<T>{st}(t + 1);
+  }
+};
+
+// End of an iteration (FinBicgUpdate), then rho = rA0.rA and beta of the next
+// iteration (FinBicgRho) while the loop goes on.
+template <typename T>
+struct FinBicgUpdateRho {
+  KState* st;
+  __device__ void operator()(const double* t) const {
+    FinBicgUpdate<T>{st}(t);
+    if (st->status == ST_RUNNING) FinBicgRho<T>{st}(t + 1);
   }
 };
 

@@ -37,7 +37,8 @@ import re
 def _bytes(name: str, n: int, nf: int, asym: bool, slots: bool = False) -> int | None:
     """Algorithmic bytes per launch of the kernels on the solve path."""
     b = _ldu_bytes(name, n, nf, asym)
-    if b is None or not slots or "<float" not in name or "k_pcg_update" in name:
+    if (b is None or not slots or "<float" not in name or "k_pcg_update" in name
+            or "k_bicg_update" in name):
         return b
     face32 = 20 if asym else 16
     return b - (8 * n + 8 + face32 * nf) + 36 * n
@@ -62,10 +63,12 @@ def _ldu_bytes(name: str, n: int, nf: int, asym: bool) -> int | None:
         return 64 * n + 8 + face64 * nf
     if "k_pcg_p_spmv<double>" in name:
         return 48 * n + 8 + face64 * nf
-    if "k_bicg_spmv_alpha<float>" in name:
-        return 24 * n + 8 + face32 * nf
-    if "k_bicg_z_spmv<float>" in name:
-        return 28 * n + 8 + face32 * nf
+    if "k_bicg_py_spmv<float>" in name:
+        return 44 * n + 8 + face32 * nf
+    if "k_bicg_sz_spmv<float>" in name:
+        return 36 * n + 8 + face32 * nf
+    if "k_bicg_update_rho<float>" in name:
+        return 32 * n
     return None
 
 
