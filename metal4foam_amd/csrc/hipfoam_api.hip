@@ -50,6 +50,10 @@ template <typename T>
 struct Buf {
   T* p = nullptr;
   size_t cap = 0;
+  Buf() = default;
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  ~Buf() { release(); }
   hipError_t ensure(size_t n) {
     if (n == 0) n = 1;
     if (p && n <= cap) return hipSuccess;

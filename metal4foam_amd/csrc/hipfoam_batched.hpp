@@ -8,10 +8,13 @@
 // are shared by every RHS of a system, the FP32 diagonal / rD are per RHS
 // (a segment may switch to the exact operator on its own).
 //
-// Launch shape.  Rows are cut into tiles that never straddle a system.  One
-// workgroup handles one (tile, rhs): logical index lb = tile * nRhs + rhs, so
-// the nRhs workgroups reading the same matrix rows are logically adjacent and,
-// with the XCD remap, run on the same XCD and share its L2.
+// Launch shape.  Rows are cut into 256-row tiles that never straddle a
+// system.  The Krylov iteration kernels run one workgroup per (tile, group of
+// RHS) and apply each row's coefficients, loaded once, to every RHS of the
+// group; the FP64 refinement kernels run one workgroup per (tile, rhs),
+// logical index lb = tile * nRhs + rhs, so the workgroups reading the same
+// matrix rows are logically adjacent and, with the XCD remap, share an XCD's
+// L2.
 //
 // Segments.  seg = system * nRhs + rhs owns a KState / OState.  A workgroup of
 // a segment that has stopped returns at entry.  Reductions write one partial
@@ -55,77 +58,171 @@ __device__ __forceinline__ LduOp<T> seg_op(LduOp<T> A, long voff) {
 }
 
 // ---------------------------------------------------------------------------
-// PCG
+// Krylov iteration kernels: one workgroup per (tile, group of up to
+// kRhsGroup right-hand sides).  Each thread owns one row: it loads the row's
+// matrix slots once and applies them to every RHS of the group whose segment
+// is still running, one RHS after another.  Each RHS's values are reduced
+// across the wavefront at once into LDS, so no per-RHS accumulator stays
+// live; the partials keep the one-per-(tile, rhs) layout
+// partials[j*pstride + tile*nRhs + rhs] that k_seg_finalize folds, summed in
+// the order of block_partials (wavefront butterflies, then the waves).
+// ---------------------------------------------------------------------------
+
+constexpr int kRhsGroup = 8;
+
+struct TileGroup {
+  int t, sys, r0, r1, rb, nr;  // tile, system, rows [r0, r1), RHS [rb, rb + nr)
+};
+
+__device__ __forceinline__ TileGroup tile_group(const BatchMap& B, int lb) {
+  const int ng = (B.nrhs + kRhsGroup - 1) / kRhsGroup;
+  TileGroup u;
+  u.t = lb / ng;
+  u.rb = (lb - u.t * ng) * kRhsGroup;
+  u.nr = B.nrhs - u.rb < kRhsGroup ? B.nrhs - u.rb : kRhsGroup;
+  u.r0 = B.tileRow0[u.t];
+  u.r1 = B.tileRow0[u.t + 1];
+  u.sys = B.tileSys[u.t];
+  return u;
+}
+
+// bit q set: the segment of RHS rb + q has a status in `mask`
+__device__ __forceinline__ unsigned status_bits(const KState* stBase, const TileGroup& u, int nrhs,
+                                                int mask) {
+  unsigned run = 0;
+  for (int q = 0; q < u.nr; ++q)
+    if ((mask >> stBase[u.sys * nrhs + u.rb + q].status) & 1) run |= 1u << q;
+  return run;
+}
+
+// Per-RHS partial sums of NV values: add() reduces one RHS's values across
+// the wavefront into LDS; write() (after every add, called by all threads)
+// folds the waves in order and stores the partials of the RHS in `run`.
+template <int NV>
+struct RhsPartials {
+  double (*sh)[16];  // [kRhsGroup * NV][16] in LDS
+  __device__ __forceinline__ void add(int q, const double (&v)[NV]) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const double t = wave_sum(v[j]);
+      if (lane == 0) sh[q * NV + j][wid] = t;
+    }
+  }
+  __device__ __forceinline__ void write(double* partials, int pstride, int base, unsigned run) {
+    __syncthreads();
+    const int k = threadIdx.x;
+    if (k < kRhsGroup * NV) {
+      const int q = k / NV, j = k - q * NV;
+      if ((run >> q) & 1) {
+        const int nw = (blockDim.x + 63) >> 6;
+        double t = sh[k][0];
+        for (int w = 1; w < nw; ++w) t += sh[k][w];
+        partials[j * pstride + base + q] = t;
+      }
+    }
+  }
+};
+
+#define HIPFOAM_RHS_PARTIALS(NV, name)              \
+  __shared__ double name##_sh[kRhsGroup * NV][16]; \
+  RhsPartials<NV> name{name##_sh}
+
+// Row i of A x for one RHS: the slots loaded once per thread when the
+// operator uses them, the face loops otherwise.
+template <typename XF>
+__device__ __forceinline__ float batch_row(const LduOp<float>& A, const RowSlots<float>& rs,
+                                           long voff, int i, float xi, XF xf) {
+  if (A.slots) return rs.apply(A.diag + voff, i, xi, xf);
+  return ldu_row(seg_op(A, voff), i, xi, xf);
+}
+
+// ---------------------------------------------------------------------------
+// PCG (metalFoamKernels.metal:112-276 per segment)
 // ---------------------------------------------------------------------------
 
 template <bool FIRST, bool LAST>
-__global__ __launch_bounds__(256) void kb_cheb_step(BatchMap B, LduOp<float> A0,
+__global__ __launch_bounds__(256) void kb_cheb_step(BatchMap B, LduOp<float> A,
                                                     const float* __restrict__ r,
                                                     const float* __restrict__ rD,
                                                     const float* __restrict__ zin,
                                                     float* __restrict__ w, float* __restrict__ zout,
-                                                    KState* stBase, double* partials, int step) {
-  const int lb = logical_block(B.xcdRemap);
-  const BlockSeg g = block_seg(B, lb);
-  KState* st = stBase + g.seg;
-  if (st->status != ST_RUNNING) return;
-  const LduOp<float> A = seg_op(A0, g.voff);
-  const float* rr = r + g.voff;
-  const float* dd = rD + g.voff;
-  const float* zi_ = zin + g.voff;
-  const float c2 = (float)st->chebC2[step - 1];
-  const float c3 = (float)st->chebC3[step - 1];
-  const float c1 = (float)st->chebC1;
-  double dot = 0.0;
-  const int i = g.r0 + threadIdx.x;
-  if (i < g.r1) {
-    float zi_new, wi;
-    if (FIRST) {
-      const float z0 = c1 * dd[i] * rr[i];
-      const float v = ldu_row(A, i, z0, [&](int j) { return c1 * dd[j] * rr[j]; });
-      wi = c2 * z0 + c3 * dd[i] * (rr[i] - v);
-      zi_new = z0 + wi;
-    } else {
-      const float zi = zi_[i];
-      const float v = ldu_row(A, i, zi, [&](int j) { return zi_[j]; });
-      wi = c2 * w[g.voff + i] + c3 * dd[i] * (rr[i] - v);
-      zi_new = zi + wi;
+                                                    KState* stBase, double* partials, int pstride,
+                                                    int step) {
+  const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
+  const unsigned run = status_bits(stBase, u, B.nrhs, 1 << ST_RUNNING);
+  if (!run) return;
+  HIPFOAM_RHS_PARTIALS(1, part);
+  const int i = u.r0 + threadIdx.x;
+  const bool in = i < u.r1;
+  RowSlots<float> rs;
+  if (in && A.slots) rs.load(A, i);
+#pragma unroll 1
+  for (int q = 0; q < u.nr; ++q) {
+    if (!((run >> q) & 1)) continue;
+    double v[1] = {0.0};
+    if (in) {
+      const KState* st = stBase + u.sys * B.nrhs + u.rb + q;
+      const long voff = (long)(u.rb + q) * B.N;
+      const float* rr = r + voff;
+      const float* dd = rD + voff;
+      const float c2 = (float)st->chebC2[step - 1];
+      const float c3 = (float)st->chebC3[step - 1];
+      const float c1 = (float)st->chebC1;
+      float zi_new, wi;
+      if (FIRST) {
+        const float z0 = c1 * dd[i] * rr[i];
+        const float y = batch_row(A, rs, voff, i, z0, [&](int j) { return c1 * dd[j] * rr[j]; });
+        wi = c2 * z0 + c3 * dd[i] * (rr[i] - y);
+        zi_new = z0 + wi;
+      } else {
+        const float* zi_ = zin + voff;
+        const float zi = zi_[i];
+        const float y = batch_row(A, rs, voff, i, zi, [&](int j) { return zi_[j]; });
+        wi = c2 * w[voff + i] + c3 * dd[i] * (rr[i] - y);
+        zi_new = zi + wi;
+      }
+      if (!LAST) w[voff + i] = wi;
+      zout[voff + i] = zi_new;
+      if (LAST) v[0] = (double)(zi_new * rr[i]);
     }
-    if (!LAST) w[g.voff + i] = wi;
-    zout[g.voff + i] = zi_new;
-    if (LAST) dot = (double)(zi_new * rr[i]);
+    if (LAST) part.add(q, v);
   }
-  if (LAST) {
-    double v[1] = {dot};
-    block_partials<1>(v, partials, 0, lb);
-  }
+  if (LAST) part.write(partials, pstride, u.t * B.nrhs + u.rb, run);
 }
 
-__global__ __launch_bounds__(256) void kb_pcg_p_spmv(BatchMap B, LduOp<float> A0,
+__global__ __launch_bounds__(256) void kb_pcg_p_spmv(BatchMap B, LduOp<float> A,
                                                      const float* __restrict__ wA,
                                                      const float* __restrict__ pold,
                                                      float* __restrict__ pnew,
                                                      float* __restrict__ Ap, KState* stBase,
-                                                     double* partials) {
-  const int lb = logical_block(B.xcdRemap);
-  const BlockSeg g = block_seg(B, lb);
-  KState* st = stBase + g.seg;
-  if (st->status != ST_RUNNING) return;
-  const LduOp<float> A = seg_op(A0, g.voff);
-  const float beta = (float)st->beta;
-  const float* w_ = wA + g.voff;
-  const float* po = pold + g.voff;
-  double dot = 0.0;
-  const int i = g.r0 + threadIdx.x;
-  if (i < g.r1) {
-    const float pi = w_[i] + beta * po[i];
-    const float y = ldu_row(A, i, pi, [&](int j) { return w_[j] + beta * po[j]; });
-    pnew[g.voff + i] = pi;
-    Ap[g.voff + i] = y;
-    dot = (double)(pi * y);
+                                                     double* partials, int pstride) {
+  const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
+  const unsigned run = status_bits(stBase, u, B.nrhs, 1 << ST_RUNNING);
+  if (!run) return;
+  HIPFOAM_RHS_PARTIALS(1, part);
+  const int i = u.r0 + threadIdx.x;
+  const bool in = i < u.r1;
+  RowSlots<float> rs;
+  if (in && A.slots) rs.load(A, i);
+#pragma unroll 1
+  for (int q = 0; q < u.nr; ++q) {
+    if (!((run >> q) & 1)) continue;
+    double v[1] = {0.0};
+    if (in) {
+      const long voff = (long)(u.rb + q) * B.N;
+      const float beta = (float)stBase[u.sys * B.nrhs + u.rb + q].beta;
+      const float* w_ = wA + voff;
+      const float* po = pold + voff;
+      const float pi = w_[i] + beta * po[i];
+      const float y = batch_row(A, rs, voff, i, pi, [&](int j) { return w_[j] + beta * po[j]; });
+      pnew[voff + i] = pi;
+      Ap[voff + i] = y;
+      v[0] = (double)(pi * y);
+    }
+    part.add(q, v);
   }
-  double v[1] = {dot};
-  block_partials<1>(v, partials, 0, lb);
+  part.write(partials, pstride, u.t * B.nrhs + u.rb, run);
 }
 
 __global__ __launch_bounds__(256) void kb_pcg_update(BatchMap B, float* __restrict__ delta,
@@ -133,162 +230,192 @@ __global__ __launch_bounds__(256) void kb_pcg_update(BatchMap B, float* __restri
                                                      float* __restrict__ r,
                                                      const float* __restrict__ p,
                                                      const float* __restrict__ Ap,
-                                                     KState* stBase, double* partials) {
-  const int lb = logical_block(B.xcdRemap);
-  const BlockSeg g = block_seg(B, lb);
-  KState* st = stBase + g.seg;
-  if (st->status != ST_RUNNING) return;
-  const float alpha = (float)st->alpha;
-  const bool copy = st->copyBest != 0;
-  double m = 0.0;
-  const long i = g.voff + g.r0 + threadIdx.x;
-  if (g.r0 + (int)threadIdx.x < g.r1) {
-    const float d = delta[i];
-    if (copy) deltaBest[i] = d;
-    delta[i] = d + alpha * p[i];
-    const float ri = r[i] - alpha * Ap[i];
-    r[i] = ri;
-    m = (double)fabsf(ri);
+                                                     KState* stBase, double* partials,
+                                                     int pstride) {
+  const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
+  const unsigned run = status_bits(stBase, u, B.nrhs, 1 << ST_RUNNING);
+  if (!run) return;
+  HIPFOAM_RHS_PARTIALS(1, part);
+  const int i = u.r0 + threadIdx.x;
+  const bool in = i < u.r1;
+#pragma unroll 1
+  for (int q = 0; q < u.nr; ++q) {
+    if (!((run >> q) & 1)) continue;
+    double v[1] = {0.0};
+    if (in) {
+      const KState* st = stBase + u.sys * B.nrhs + u.rb + q;
+      const long k = (long)(u.rb + q) * B.N + i;
+      const float alpha = (float)st->alpha;
+      const float d = delta[k];
+      if (st->copyBest != 0) deltaBest[k] = d;
+      delta[k] = d + alpha * p[k];
+      const float ri = r[k] - alpha * Ap[k];
+      r[k] = ri;
+      v[0] = (double)fabsf(ri);
+    }
+    part.add(q, v);
   }
-  double v[1] = {m};
-  block_partials<1>(v, partials, 0, lb);
+  part.write(partials, pstride, u.t * B.nrhs + u.rb, run);
 }
 
 // ---------------------------------------------------------------------------
-// BiCGStab
+// BiCGStab: the three fused row kernels of the single-system path, per RHS
+// (p and AyA ping-pong; rho of the next iteration reduced in the update)
 // ---------------------------------------------------------------------------
 
+// rho = rA0.rA of the first iteration
 __global__ __launch_bounds__(256) void kb_bicg_rho(BatchMap B, const float* __restrict__ rA0,
                                                    const float* __restrict__ r, KState* stBase,
-                                                   double* partials) {
-  const int lb = logical_block(B.xcdRemap);
-  const BlockSeg g = block_seg(B, lb);
-  if (stBase[g.seg].status != ST_RUNNING) return;
-  double d = 0.0;
-  const long i = g.voff + g.r0 + threadIdx.x;
-  if (g.r0 + (int)threadIdx.x < g.r1) d = (double)(rA0[i] * r[i]);
-  double v[1] = {d};
-  block_partials<1>(v, partials, 0, lb);
-}
-
-__global__ __launch_bounds__(256) void kb_bicg_p_y(BatchMap B, float* __restrict__ p,
-                                                   const float* __restrict__ r,
-                                                   const float* __restrict__ AyA,
-                                                   const float* __restrict__ rD,
-                                                   float* __restrict__ yA, KState* stBase) {
-  const int lb = logical_block(B.xcdRemap);
-  const BlockSeg g = block_seg(B, lb);
-  const KState* st = stBase + g.seg;
-  if (st->status != ST_RUNNING) return;
-  const float beta = (float)st->beta, omega = (float)st->omega;
-  const long i = g.voff + g.r0 + threadIdx.x;
-  if (g.r0 + (int)threadIdx.x < g.r1) {
-    const float pi = r[i] + beta * (p[i] - omega * AyA[i]);
-    p[i] = pi;
-    yA[i] = rD[i] * pi;
+                                                   double* partials, int pstride) {
+  const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
+  const unsigned run = status_bits(stBase, u, B.nrhs, 1 << ST_RUNNING);
+  if (!run) return;
+  HIPFOAM_RHS_PARTIALS(1, part);
+  const int i = u.r0 + threadIdx.x;
+#pragma unroll 1
+  for (int q = 0; q < u.nr; ++q) {
+    if (!((run >> q) & 1)) continue;
+    const long k = (long)(u.rb + q) * B.N + i;
+    double v[1] = {i < u.r1 ? (double)(rA0[k] * r[k]) : 0.0};
+    part.add(q, v);
   }
+  part.write(partials, pstride, u.t * B.nrhs + u.rb, run);
 }
 
-__global__ __launch_bounds__(256) void kb_bicg_spmv_alpha(BatchMap B, LduOp<float> A0,
+__global__ __launch_bounds__(256) void kb_bicg_py_spmv(BatchMap B, LduOp<float> A,
+                                                       const float* __restrict__ r,
+                                                       const float* __restrict__ rD,
+                                                       const float* __restrict__ pold,
+                                                       float* __restrict__ pnew,
+                                                       const float* __restrict__ AyAold,
+                                                       float* __restrict__ AyAnew,
+                                                       float* __restrict__ yA,
+                                                       const float* __restrict__ rA0,
+                                                       KState* stBase, double* partials,
+                                                       int pstride) {
+  const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
+  const unsigned run = status_bits(stBase, u, B.nrhs, 1 << ST_RUNNING);
+  if (!run) return;
+  HIPFOAM_RHS_PARTIALS(1, part);
+  const int i = u.r0 + threadIdx.x;
+  const bool in = i < u.r1;
+  RowSlots<float> rs;
+  if (in && A.slots) rs.load(A, i);
+#pragma unroll 1
+  for (int q = 0; q < u.nr; ++q) {
+    if (!((run >> q) & 1)) continue;
+    double v[1] = {0.0};
+    if (in) {
+      const KState* st = stBase + u.sys * B.nrhs + u.rb + q;
+      const long voff = (long)(u.rb + q) * B.N;
+      const float beta = (float)st->beta, omega = (float)st->omega;
+      const float* r_ = r + voff;
+      const float* d_ = rD + voff;
+      const float* po = pold + voff;
+      const float* ao = AyAold + voff;
+      const float pi = r_[i] + beta * (po[i] - omega * ao[i]);
+      const float yi = d_[i] * pi;
+      const float y = batch_row(A, rs, voff, i, yi, [&](int j) {
+        const float pj = r_[j] + beta * (po[j] - omega * ao[j]);
+        return d_[j] * pj;
+      });
+      pnew[voff + i] = pi;
+      yA[voff + i] = yi;
+      AyAnew[voff + i] = y;
+      v[0] = (double)(rA0[voff + i] * y);
+    }
+    part.add(q, v);
+  }
+  part.write(partials, pstride, u.t * B.nrhs + u.rb, run);
+}
+
+__global__ __launch_bounds__(256) void kb_bicg_sz_spmv(BatchMap B, LduOp<float> A,
+                                                       const float* __restrict__ r,
+                                                       const float* __restrict__ rD,
+                                                       const float* __restrict__ AyA,
+                                                       float* __restrict__ sA,
+                                                       float* __restrict__ zA,
+                                                       float* __restrict__ tA, KState* stBase,
+                                                       double* partials, int pstride) {
+  const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
+  const unsigned run = status_bits(stBase, u, B.nrhs, 1 << ST_RUNNING);
+  if (!run) return;
+  HIPFOAM_RHS_PARTIALS(3, part);
+  const int i = u.r0 + threadIdx.x;
+  const bool in = i < u.r1;
+  RowSlots<float> rs;
+  if (in && A.slots) rs.load(A, i);
+#pragma unroll 1
+  for (int q = 0; q < u.nr; ++q) {
+    if (!((run >> q) & 1)) continue;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (in) {
+      const long voff = (long)(u.rb + q) * B.N;
+      const float alpha = (float)stBase[u.sys * B.nrhs + u.rb + q].alpha;
+      const float* r_ = r + voff;
+      const float* d_ = rD + voff;
+      const float* a_ = AyA + voff;
+      const float si = r_[i] - alpha * a_[i];
+      const float zi = d_[i] * si;
+      const float t = batch_row(A, rs, voff, i, zi, [&](int j) {
+        const float sj = r_[j] - alpha * a_[j];
+        return d_[j] * sj;
+      });
+      sA[voff + i] = si;
+      zA[voff + i] = zi;
+      tA[voff + i] = t;
+      v[0] = (double)fabsf(si);
+      v[1] = (double)(si * t);
+      v[2] = (double)(t * t);
+    }
+    part.add(q, v);
+  }
+  part.write(partials, pstride, u.t * B.nrhs + u.rb, run);
+}
+
+__global__ __launch_bounds__(256) void kb_bicg_update_rho(BatchMap B, float* __restrict__ delta,
+                                                          float* __restrict__ deltaBest,
                                                           const float* __restrict__ yA,
+                                                          const float* __restrict__ zA,
+                                                          float* __restrict__ r,
+                                                          const float* __restrict__ sA,
+                                                          const float* __restrict__ tA,
                                                           const float* __restrict__ rA0,
-                                                          float* __restrict__ AyA,
-                                                          KState* stBase, double* partials) {
-  const int lb = logical_block(B.xcdRemap);
-  const BlockSeg g = block_seg(B, lb);
-  if (stBase[g.seg].status != ST_RUNNING) return;
-  const LduOp<float> A = seg_op(A0, g.voff);
-  const float* y_ = yA + g.voff;
-  double d = 0.0;
-  const int i = g.r0 + threadIdx.x;
-  if (i < g.r1) {
-    const float y = ldu_row(A, i, y_[i], [&](int j) { return y_[j]; });
-    AyA[g.voff + i] = y;
-    d = (double)(rA0[g.voff + i] * y);
+                                                          KState* stBase, double* partials,
+                                                          int pstride) {
+  const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
+  const unsigned run = status_bits(stBase, u, B.nrhs, 1 << ST_RUNNING);
+  const unsigned early = status_bits(stBase, u, B.nrhs, 1 << ST_EARLY);
+  if (!(run | early)) return;
+  HIPFOAM_RHS_PARTIALS(2, part);
+  const int i = u.r0 + threadIdx.x;
+  const bool in = i < u.r1;
+#pragma unroll 1
+  for (int q = 0; q < u.nr; ++q) {
+    const KState* st = stBase + u.sys * B.nrhs + u.rb + q;
+    const long k = (long)(u.rb + q) * B.N + i;
+    if ((early >> q) & 1) {
+      if (in) delta[k] += (float)st->alpha * yA[k];  // FinBicgUpdateRho completes the exit
+      continue;
+    }
+    if (!((run >> q) & 1)) continue;
+    double v[2] = {0.0, 0.0};
+    if (in) {
+      const float alpha = (float)st->alpha, omega = (float)st->omega;
+      const float d = delta[k];
+      if (st->copyBest != 0) deltaBest[k] = d;
+      delta[k] = d + (alpha * yA[k] + omega * zA[k]);
+      const float ri = sA[k] - omega * tA[k];
+      r[k] = ri;
+      v[0] = (double)fabsf(ri);
+      v[1] = (double)(rA0[k] * ri);
+    }
+    part.add(q, v);
   }
-  double v[1] = {d};
-  block_partials<1>(v, partials, 0, lb);
+  if (run) part.write(partials, pstride, u.t * B.nrhs + u.rb, run);
 }
 
-__global__ __launch_bounds__(256) void kb_bicg_s(BatchMap B, float* __restrict__ sA,
-                                                 const float* __restrict__ r,
-                                                 const float* __restrict__ AyA, KState* stBase,
-                                                 double* partials) {
-  const int lb = logical_block(B.xcdRemap);
-  const BlockSeg g = block_seg(B, lb);
-  const KState* st = stBase + g.seg;
-  if (st->status != ST_RUNNING) return;
-  const float alpha = (float)st->alpha;
-  double m = 0.0;
-  const long i = g.voff + g.r0 + threadIdx.x;
-  if (g.r0 + (int)threadIdx.x < g.r1) {
-    const float s = r[i] - alpha * AyA[i];
-    sA[i] = s;
-    m = (double)fabsf(s);
-  }
-  double v[1] = {m};
-  block_partials<1>(v, partials, 0, lb);
-}
-
-__global__ __launch_bounds__(256) void kb_bicg_z_spmv(BatchMap B, LduOp<float> A0,
-                                                      const float* __restrict__ rD,
-                                                      const float* __restrict__ sA,
-                                                      float* __restrict__ zA,
-                                                      float* __restrict__ tA, KState* stBase,
-                                                      double* partials, int pstride) {
-  const int lb = logical_block(B.xcdRemap);
-  const BlockSeg g = block_seg(B, lb);
-  if (stBase[g.seg].status != ST_RUNNING) return;
-  const LduOp<float> A = seg_op(A0, g.voff);
-  const float* d_ = rD + g.voff;
-  const float* s_ = sA + g.voff;
-  double d[2] = {0.0, 0.0};
-  const int i = g.r0 + threadIdx.x;
-  if (i < g.r1) {
-    const float zi = d_[i] * s_[i];
-    const float t = ldu_row(A, i, zi, [&](int j) { return d_[j] * s_[j]; });
-    zA[g.voff + i] = zi;
-    tA[g.voff + i] = t;
-    d[0] = (double)(s_[i] * t);
-    d[1] = (double)(t * t);
-  }
-  block_partials<2>(d, partials, pstride, lb);
-}
-
-__global__ __launch_bounds__(256) void kb_bicg_update(BatchMap B, float* __restrict__ delta,
-                                                      float* __restrict__ deltaBest,
-                                                      const float* __restrict__ yA,
-                                                      const float* __restrict__ zA,
-                                                      float* __restrict__ r,
-                                                      const float* __restrict__ sA,
-                                                      const float* __restrict__ tA,
-                                                      KState* stBase, double* partials) {
-  const int lb = logical_block(B.xcdRemap);
-  const BlockSeg g = block_seg(B, lb);
-  const KState* st = stBase + g.seg;
-  const int status = st->status;
-  if (status != ST_RUNNING && status != ST_EARLY) return;
-  const float alpha = (float)st->alpha;
-  const long i = g.voff + g.r0 + threadIdx.x;
-  const bool in = g.r0 + (int)threadIdx.x < g.r1;
-  if (status == ST_EARLY) {
-    if (in) delta[i] += alpha * yA[i];
-    return;
-  }
-  const float omega = (float)st->omega;
-  const bool copy = st->copyBest != 0;
-  double m = 0.0;
-  if (in) {
-    const float d = delta[i];
-    if (copy) deltaBest[i] = d;
-    delta[i] = d + (alpha * yA[i] + omega * zA[i]);
-    const float ri = sA[i] - omega * tA[i];
-    r[i] = ri;
-    m = (double)fabsf(ri);
-  }
-  double v[1] = {m};
-  block_partials<1>(v, partials, 0, lb);
-}
+#undef HIPFOAM_RHS_PARTIALS
 
 // ---------------------------------------------------------------------------
 // segmented second stage

@@ -235,31 +235,47 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
 // before any x gather, and every gather before the sums -- three waves of
 // independent memory requests instead of a chain per face.  The terms are
 // added in exactly the order of the loop form, so both give identical bits.
+// One row of the row-slot layout held in registers: loaded once, applied to
+// any number of vectors (the right-hand sides of a batch share the matrix).
+template <typename T>
+struct RowSlots {
+  T c[2 * kShortRow];
+  int o[2 * kShortRow];
+  __device__ __forceinline__ void load(const LduOp<T>& A, int i) {
+    const size_t n = (size_t)A.n;
+#pragma unroll
+    for (int p = 0; p < kShortRow; ++p) {
+      const uint32_t w = A.slotOff[p * n + i];
+      o[2 * p] = (int)(int16_t)(w & 0xffffu);
+      o[2 * p + 1] = (int)(int16_t)(w >> 16);
+    }
+#pragma unroll
+    for (int q = 0; q < 2 * kShortRow; ++q) c[q] = A.slotCoef[q * n + i];
+  }
+  // diag term, then the owner slots, then the neighbour slots; empty slots
+  // are skipped, so the terms and their order are those of the face loops
+  template <typename XF>
+  __device__ __forceinline__ T apply(const T* diag, int i, T xi, XF xf) const {
+    T s = diag[i] * xi;
+    T xq[2 * kShortRow];
+#pragma unroll
+    for (int q = 0; q < 2 * kShortRow; ++q)
+      if (o[q] != 0) xq[q] = xf(i + o[q]);
+#pragma unroll
+    for (int q = 0; q < 2 * kShortRow; ++q)
+      if (o[q] != 0) s += c[q] * xq[q];
+    return s;
+  }
+};
+
 template <typename T, typename XF>
 __device__ __forceinline__ T ldu_row(const LduOp<T>& A, int i, T xi, XF xf) {
   if (A.slots) {
     // coefficients and offsets are independent coalesced loads (no row-start
-    // indirection); empty slots are skipped, so the terms and their order are
-    // those of the face loops below
-    const size_t n = (size_t)A.n;
-    T s = A.diag[i] * xi;
-    T cq[2 * kShortRow], xq[2 * kShortRow];
-    int oq[2 * kShortRow];
-#pragma unroll
-    for (int p = 0; p < kShortRow; ++p) {
-      const uint32_t w = A.slotOff[p * n + i];
-      oq[2 * p] = (int)(int16_t)(w & 0xffffu);
-      oq[2 * p + 1] = (int)(int16_t)(w >> 16);
-    }
-#pragma unroll
-    for (int q = 0; q < 2 * kShortRow; ++q) cq[q] = A.slotCoef[q * n + i];
-#pragma unroll
-    for (int q = 0; q < 2 * kShortRow; ++q)
-      if (oq[q] != 0) xq[q] = xf(i + oq[q]);
-#pragma unroll
-    for (int q = 0; q < 2 * kShortRow; ++q)
-      if (oq[q] != 0) s += cq[q] * xq[q];
-    return s;
+    // indirection)
+    RowSlots<T> rs;
+    rs.load(A, i);
+    return rs.apply(A.diag, i, xi, xf);
   }
   const int f0 = A.ownStart[i], f1 = A.ownStart[i + 1];
   const int k0 = A.losortStart[i], k1 = A.losortStart[i + 1];
