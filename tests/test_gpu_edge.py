@@ -1,0 +1,106 @@
+"""Edge cases of the solve path on the GPU: empty meshes, meshes without
+faces (diagonal matrices), one cell, disconnected blocks, and a 256^3 mesh
+whose column offsets exceed int16 (the row-slot layout falls back to the LDU
+face loops) checked through the true FP64 residual."""
+
+import numpy as np
+import pytest
+
+from metal4foam_amd import hipfoam as H
+from metal4foam_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def diagonal_system(n, seed):
+    rng = np.random.default_rng(seed)
+    e = np.zeros(0, dtype=np.int32)
+    own = np.zeros(n + 1, dtype=np.int32)
+    return mg.LduMatrix(n, e, e, own, e, own.copy(), rng.uniform(1.0, 3.0, n), np.zeros(0),
+                        None, rng.standard_normal(n), (n,), "diagonal")
+
+
+def block_diagonal(a, b):
+    """Two independent systems as one LDU matrix (cells of b after a)."""
+    low = np.concatenate([a.lower_addr, b.lower_addr + a.n]).astype(np.int32)
+    upp = np.concatenate([a.upper_addr, b.upper_addr + a.n]).astype(np.int32)
+    n = a.n + b.n
+    own, los, loss = mg.derived_addressing(n, low, upp)
+    lower = None if a.lower is None else np.concatenate([a.lower, b.lower])
+    return mg.LduMatrix(n, low, upp, own, los, loss, np.concatenate([a.diag, b.diag]),
+                        np.concatenate([a.upper, b.upper]), lower,
+                        np.concatenate([a.source, b.source]), (n,), "blocks")
+
+
+def true_res(m, psi, nf):
+    lower = m.upper if m.lower is None else m.lower
+    ax = m.diag * psi
+    ax += np.bincount(m.lower_addr, weights=m.upper * psi[m.upper_addr], minlength=m.n)
+    ax += np.bincount(m.upper_addr, weights=lower * psi[m.lower_addr], minlength=m.n)
+    return np.abs(m.source - ax).sum() / nf
+
+
+@pytest.mark.parametrize("solver", [H.PCG, H.PBICGSTAB])
+def test_empty_mesh(ctx, solver):
+    m = diagonal_system(0, 1)
+    ctx.set_topology(m)
+    psi = np.zeros(0)
+    perf = ctx.solve(solver, m.diag, m.upper, None, psi, m.source)
+    assert perf.converged and perf.nIterations == 0
+    perf = ctx.solve(solver, m.diag, m.upper, None, psi, m.source, fp64=True)
+    assert perf.converged and perf.nIterations == 0
+
+
+@pytest.mark.parametrize("solver", [H.PCG, H.PBICGSTAB])
+@pytest.mark.parametrize("n", [1, 300])
+def test_diagonal_matrix(ctx, solver, n):
+    m = diagonal_system(n, 2)
+    ctx.set_topology(m)
+    psi = np.zeros(n)
+    perf = ctx.solve(solver, m.diag, m.upper, None, psi, m.source,
+                     H.default_controls(tolerance=1e-12))
+    assert perf.converged
+    np.testing.assert_allclose(psi, m.source / m.diag, rtol=1e-12, atol=0)
+    psi = np.zeros(n)
+    perf = ctx.solve(solver, m.diag, m.upper, None, psi, m.source,
+                     H.default_controls(tolerance=1e-12), fp64=True)
+    assert perf.converged and perf.nIterations <= 1
+    np.testing.assert_allclose(psi, m.source / m.diag, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("kind", ["sym", "asym"])
+def test_disconnected_blocks_match_separate_solves(ctx, orc, kind):
+    if kind == "sym":
+        a = mg.poisson(9, 8, 7, seed=3, pin=1.0)
+        b = mg.poisson(6, 5, 4, seed=4, pin=1.0)
+        solver = H.PCG
+    else:
+        a = mg.convection_diffusion(9, 8, 7, seed=5)
+        b = mg.convection_diffusion(6, 5, 4, seed=6)
+        solver = H.PBICGSTAB
+    m = block_diagonal(a, b)
+    ctx.set_topology(m)
+    psi = np.zeros(m.n)
+    perf = ctx.solve(solver, m.diag, m.upper, m.lower, psi, m.source,
+                     H.default_controls(tolerance=1e-10, maxRestarts=20, maxIter=5000))
+    assert perf.converged
+    x = np.concatenate([np.linalg.solve(mg.dense(a), a.source),
+                        np.linalg.solve(mg.dense(b), b.source)])
+    assert np.abs(psi - x).max() <= 1e-6 * np.abs(x).max()
+    assert true_res(m, psi, perf.normFactor) < 1e-10
+
+
+@pytest.mark.slow
+def test_256_cubed_poisson_beyond_int16_offsets(ctx):
+    m = mg.poisson(256, 256, 256, seed=1001, lognormal=True)
+    print(f"[edge] 256^3: n={m.n} faces={m.nfaces}", flush=True)
+    ctx.set_topology(m)
+    assert ctx.active_schedule() == 0  # z offset 65536 does not fit int16
+    psi = np.zeros(m.n)
+    perf = ctx.solve(H.PCG, m.diag, m.upper, None, psi, m.source,
+                     H.default_controls(maxRestarts=8, maxIter=20000))
+    tr = true_res(m, psi, perf.normFactor)
+    print(f"[edge] 256^3 refined: it={perf.nIterations} restarts={perf.restarts} "
+          f"res={perf.finalResidual:.3e} true={tr:.3e} t={perf.tTotal:.3f}s", flush=True)
+    assert perf.converged and tr < 1e-6
+    assert tr == pytest.approx(perf.finalResidual, rel=1e-6)
