@@ -27,6 +27,10 @@ HIPCC_FLAGS = [
     # packed FP32 (v_pk_*_f32) stays off: the compiler would emit it for
     # float2-shaped code, and this build keeps to scalar VALU FP32.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # every float expression rounds as written (no contraction into FMA): the
+    # results do not depend on how a kernel was specialised or inlined, and
+    # the FP32 row sums round exactly like the CPU oracle's
+    "-ffp-contract=off",
     "-I", os.path.join(ROOT, "include"),
 ]
 

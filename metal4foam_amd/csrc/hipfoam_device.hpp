@@ -86,10 +86,14 @@ struct LduOp {
   const T* __restrict__ lowerL;  // staged: lower[losort[k]]; direct: lower[f]
   // Row-slot layout (slots != 0): row i's owner faces in slots 0..2 and its
   // neighbour faces in slots 3..5, in the order the face loops visit them.
-  // slotCoef[q*n + i] is the coefficient of slot q; slotOff[p*n + i] packs the
-  // column offsets (column - i, int16; 0 = empty slot) of slots 2p and 2p+1.
+  // slotCoef[q*n + i] is the coefficient of slot q.  The column of slot q is
+  // i + slotBase[(i / kSlotTile)*6 + q] + delta, with the int16 deltas of
+  // slots 2p and 2p+1 packed in slotOff[p*n + i] and kSlotEmpty marking an
+  // empty slot; without slotBase (every offset fits int16) the delta is the
+  // offset itself and 0 marks an empty slot.
   const T* __restrict__ slotCoef;
   const uint32_t* __restrict__ slotOff;
+  const int32_t* __restrict__ slotBase;
   int32_t n;
   int32_t direct;
   int32_t xcdRemap;
@@ -99,6 +103,9 @@ struct LduOp {
 
 // Rows of hexahedral meshes have at most 3 owner and 3 neighbour faces.
 constexpr int kShortRow = 3;
+// Rows per offset base of the row-slot layout, and the empty-slot delta.
+constexpr int kSlotTile = 256;
+constexpr int kSlotEmpty = -32768;
 
 // ---------------------------------------------------------------------------
 // launch geometry
@@ -243,14 +250,31 @@ struct RowSlots {
   int o[2 * kShortRow];
   __device__ __forceinline__ void load(const LduOp<T>& A, int i) {
     const size_t n = (size_t)A.n;
+    // every load issued before any decoding branch
+    uint32_t w[kShortRow];
 #pragma unroll
-    for (int p = 0; p < kShortRow; ++p) {
-      const uint32_t w = A.slotOff[p * n + i];
-      o[2 * p] = (int)(int16_t)(w & 0xffffu);
-      o[2 * p + 1] = (int)(int16_t)(w >> 16);
-    }
+    for (int p = 0; p < kShortRow; ++p) w[p] = A.slotOff[p * n + i];
 #pragma unroll
     for (int q = 0; q < 2 * kShortRow; ++q) c[q] = A.slotCoef[q * n + i];
+    if (!A.slotBase) {
+      // every offset is its own int16 delta; 0 marks an empty slot
+#pragma unroll
+      for (int p = 0; p < kShortRow; ++p) {
+        o[2 * p] = (int)(int16_t)(w[p] & 0xffffu);
+        o[2 * p + 1] = (int)(int16_t)(w[p] >> 16);
+      }
+    } else {
+      // the tile's bases, read once per wavefront when its rows share a tile
+      const int t = i / kSlotTile;
+      const int t0 = __builtin_amdgcn_readfirstlane(t);
+      const int32_t* base = A.slotBase + (__all(t == t0) ? t0 : t) * (2 * kShortRow);
+#pragma unroll
+      for (int p = 0; p < kShortRow; ++p) {
+        const int d0 = (int)(int16_t)(w[p] & 0xffffu), d1 = (int)(int16_t)(w[p] >> 16);
+        o[2 * p] = d0 == kSlotEmpty ? 0 : base[2 * p] + d0;
+        o[2 * p + 1] = d1 == kSlotEmpty ? 0 : base[2 * p + 1] + d1;
+      }
+    }
   }
   // diag term, then the owner slots, then the neighbour slots; empty slots
   // are skipped, so the terms and their order are those of the face loops
@@ -855,45 +879,26 @@ __global__ __launch_bounds__(1024) void k_gather_addr(int nf, const int32_t* __r
   if (k < nf) out[k] = low[losort[k]];
 }
 
-// Row-slot column offsets (layout of LduOp::slotOff) from the LDU addressing.
-// Requires every row to have <= kShortRow owner and neighbour faces and every
-// |column - row| <= 32767; both are checked on the host.
-__global__ __launch_bounds__(256) void k_build_slot_offsets(
-    int n, const int32_t* __restrict__ ownStart, const int32_t* __restrict__ uppAddr,
-    const int32_t* __restrict__ losortStart, const int32_t* __restrict__ losortLow,
-    uint32_t* __restrict__ slotOff) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int off[2 * kShortRow];
-  const int f0 = ownStart[i], f1 = ownStart[i + 1];
-  const int k0 = losortStart[i], k1 = losortStart[i + 1];
-#pragma unroll
-  for (int q = 0; q < kShortRow; ++q) {
-    off[q] = f0 + q < f1 ? uppAddr[f0 + q] - i : 0;
-    off[kShortRow + q] = k0 + q < k1 ? losortLow[k0 + q] - i : 0;
-  }
-#pragma unroll
-  for (int p = 0; p < kShortRow; ++p)
-    slotOff[(size_t)p * n + i] =
-        ((uint32_t)(uint16_t)(int16_t)off[2 * p]) |
-        ((uint32_t)(uint16_t)(int16_t)off[2 * p + 1] << 16);
-}
-
 // Row-slot coefficients (layout of LduOp::slotCoef) from the face-ordered
-// upper and losort-ordered lower coefficients; empty slots hold 0.
+// upper and losort-ordered lower coefficients: a side's faces fill, in order,
+// the non-empty slots of the row's slotOff words; empty slots hold 0.
 template <typename T>
 __global__ __launch_bounds__(256) void k_build_slot_coeffs(
     int n, const int32_t* __restrict__ ownStart, const int32_t* __restrict__ losortStart,
-    const T* __restrict__ upper, const T* __restrict__ lowerL, T* __restrict__ slotCoef) {
+    const uint32_t* __restrict__ slotOff, int based, const T* __restrict__ upper,
+    const T* __restrict__ lowerL, T* __restrict__ slotCoef) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int f0 = ownStart[i], f1 = ownStart[i + 1];
-  const int k0 = losortStart[i], k1 = losortStart[i + 1];
   const size_t nn = (size_t)n;
+  const int empty = based ? kSlotEmpty : 0;
+  int f = ownStart[i], k = losortStart[i];
 #pragma unroll
-  for (int q = 0; q < kShortRow; ++q) {
-    slotCoef[q * nn + i] = f0 + q < f1 ? upper[f0 + q] : T(0);
-    slotCoef[(kShortRow + q) * nn + i] = k0 + q < k1 ? lowerL[k0 + q] : T(0);
+  for (int q = 0; q < 2 * kShortRow; ++q) {
+    const uint32_t w = slotOff[(q >> 1) * nn + i];
+    const int d = (int)(int16_t)((q & 1) ? (w >> 16) : (w & 0xffffu));
+    T v = T(0);
+    if (d != empty) v = q < kShortRow ? upper[f++] : lowerL[k++];
+    slotCoef[q * nn + i] = v;
   }
 }
 

@@ -1,8 +1,9 @@
 """The row-slot SpMV schedule (spmvSchedule 2): the same terms in the same
 order as the face-loop schedules 0 and 1, so SpMV results, inner solves and
 refined solves must agree bit for bit; meshes that do not allow the layout
-(a row with more than three owner or neighbour faces, or a column offset
-beyond int16) fall back to schedule 0."""
+(a row with more than three owner or neighbour faces, or one slot's column
+offsets spanning more than an int16 delta within a 256-row tile) fall back to
+schedule 0."""
 
 import numpy as np
 import pytest
@@ -39,6 +40,27 @@ def irregular(n, nextra, seed):
     return m
 
 
+def scattered(n, seed):
+    """A chain plus one long-range face per cell pair of a random matching:
+    at most three faces per row, but offsets scattered over the whole mesh."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(n)
+    pairs = {(i, i + 1) for i in range(n - 1)}
+    for k in range(0, n - 1, 2):
+        a, b = sorted((int(perm[k]), int(perm[k + 1])))
+        if b - a > 1:
+            pairs.add((a, b))
+    pairs = sorted(pairs)
+    low = np.array([p[0] for p in pairs], dtype=np.int32)
+    upp = np.array([p[1] for p in pairs], dtype=np.int32)
+    own, los, loss = mg.derived_addressing(n, low, upp)
+    upper = -rng.uniform(0.5, 1.5, len(pairs))
+    lower = -rng.uniform(0.5, 1.5, len(pairs))
+    diag = -mg.row_offdiag_sums(n, low, upp, upper, lower) + 0.5
+    return mg.LduMatrix(n, low, upp, own, los, loss, diag, upper, lower,
+                        rng.standard_normal(n), (n,), "scattered")
+
+
 def spmv_all_schedules(ctx, m, x, lower):
     out = []
     for s in (0, 1, 2):
@@ -49,22 +71,24 @@ def spmv_all_schedules(ctx, m, x, lower):
     return out
 
 
-@pytest.mark.parametrize("case", ["poisson", "convdiff", "wide", "irregular"])
+@pytest.mark.parametrize("case", ["poisson", "convdiff", "wide", "irregular", "scattered"])
 def test_spmv_schedules_bitwise_equal(ctx, case):
     if case == "poisson":
         m = mg.poisson(33, 21, 17, seed=5)
     elif case == "convdiff":
         m = mg.convection_diffusion(29, 18, 13, seed=6)
-    elif case == "wide":   # z offset 200*170 = 34000 > int16: layout not possible
+    elif case == "wide":   # z offset 200*170 = 34000: an int16 delta from the tile base
         m = mg.convection_diffusion(200, 170, 3, seed=7)
-    else:
+    elif case == "irregular":
         m = irregular(3000, 2500, 8)
+    else:
+        m = scattered(200000, 9)
     print(f"{case}: n={m.n} faces={m.nfaces}", flush=True)
     ctx.set_topology(m)
     x = np.random.default_rng(9).standard_normal(m.n).astype(np.float32)
     for lower in (None, m.lower):
         ys = spmv_all_schedules(ctx, m, x, lower)
-        assert ctx.active_schedule() == (2 if case in ("poisson", "convdiff") else 0)
+        assert ctx.active_schedule() == (0 if case in ("irregular", "scattered") else 2)
         np.testing.assert_array_equal(ys[2], ys[0])
         np.testing.assert_array_equal(ys[1], ys[0])
         # and against an FP64 product, to the FP32 rounding bound
@@ -111,3 +135,48 @@ def test_batched_slots_identical_to_face_loops(ctx):
         b.close()
     for a, c in zip(outs[0], outs[1]):
         np.testing.assert_array_equal(c, a)
+
+
+@pytest.mark.parametrize("sched", [0, 1, 2])
+@pytest.mark.parametrize("sym", [True, False])
+def test_spmv_fp32_bit_exact_against_oracle(ctx, orc, sched, sym):
+    """Without FMA contraction every schedule rounds each row term exactly like
+    the oracle's FP32 row gather, so arbitrary float inputs match bit for bit."""
+    m = mg.convection_diffusion(23, 19, 11, seed=40)
+    if sym:
+        m.lower = None
+    rng = np.random.default_rng(41)
+    x = rng.standard_normal(m.n).astype(np.float32)
+    ctx.set_topology(m)
+    ctx.set_schedule(spmvSchedule=sched)
+    ctx.set_matrix(m.diag, m.upper, m.lower)
+    np.testing.assert_array_equal(ctx.amul(x, H.FP32), orc.spmv_f(m, x))
+
+
+@pytest.mark.parametrize("bicg", [False, True])
+def test_inner_solve_against_oracle_bits(ctx, orc, bicg):
+    """The inner FP32 iterations against the oracle's.  The SpMV and the
+    Chebyshev and BiCGStab vector updates round exactly as the oracle's do;
+    the scalars are double sums rounded once to float, whose summation order
+    differs from the oracle's, which could move a scalar by one float ulp.  On
+    these inputs no scalar moves, and the corrections agree bit for bit."""
+    m = mg.convection_diffusion(20, 16, 12, seed=42) if bicg else mg.poisson(20, 16, 12, seed=43)
+    ctx.set_topology(m)
+    ctx.set_schedule(spmvSchedule=2)
+    lower = m.lower if bicg else None
+    ctx.set_coeffs(m.diag, m.upper, lower)
+    r = m.source
+    target = 1e-4 * np.abs(r).sum()
+    if bicg:
+        st = ctx.bicgstab(r, target, 200)
+        so, do = orc.inner_bicgstab_f(m, r, target, 200)
+    else:
+        st = ctx.pcg(r, target, 200, order=8, beta=2.5)
+        so, do = orc.inner_pcg_f(m, r, target, 200, cheb_order=8, cheb_beta=2.5)
+    d = ctx.delta()
+    same = np.mean(d == do)
+    ulps = np.abs(d.view(np.int32).astype(np.int64) - do.view(np.int32).astype(np.int64)).max()
+    print(f"{'bicg' if bicg else 'pcg'}: it gpu {st.nIterations} oracle {so.nIterations}, "
+          f"identical {100 * same:.1f} %, max ulp distance {ulps}")
+    assert st.nIterations == so.nIterations
+    np.testing.assert_array_equal(d, do)
