@@ -37,11 +37,17 @@ import re
 def _bytes(name: str, n: int, nf: int, asym: bool, slots: bool = False) -> int | None:
     """Algorithmic bytes per launch of the kernels on the solve path."""
     b = _ldu_bytes(name, n, nf, asym)
-    if (b is None or not slots or "<float" not in name or "k_pcg_update" in name
-            or "k_bicg_update" in name):
+    if b is None or not slots or "k_pcg_update" in name or "k_bicg_update" in name:
         return b
-    face32 = 20 if asym else 16
-    return b - (8 * n + 8 + face32 * nf) + 36 * n
+    if "<float" in name:
+        face, coef = (20 if asym else 16), 4
+    elif "<double" in name:
+        face, coef = (28 if asym else 24), 8
+    else:
+        return b
+    # row starts and per-face arrays out; six coefficient slots and three
+    # packed offset words per row in
+    return b - (8 * n + 8 + face * nf) + (6 * coef + 12) * n
 
 
 def _ldu_bytes(name: str, n: int, nf: int, asym: bool) -> int | None:
