@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--cold-reps", type=int, default=10,
                     help="SpMV launches timed after a MALL-flushing copy")
+    ap.add_argument("--adjoint-precision", choices=["fp32", "fp64"], default="fp32",
+                    help="adjoint128: transpose solve refined (FP32 inner) or plain FP64")
     ap.add_argument("--fd-faces", type=int, default=0,
                     help="adjoint128: finite-difference check on this many faces and cells")
     args = ap.parse_args()
@@ -461,14 +463,18 @@ def run_adjoint(args, H, mg, desc):
     log(ctx.description)
     ctx.set_topology(m)
     ctl = H.default_controls(maxRestarts=8, maxIter=20000)
+    # g = x - x_t has a non-zero sum: on the pinned Poisson the reference's
+    # zero-mean deflation stalls on such a right-hand side, so the refined
+    # transpose solve projects its inner right-hand side along u = A 1
+    # (deflate = 3); --adjoint-precision fp64 solves it in plain FP64 instead
+    ctl_adj = H.default_controls(maxRestarts=8, maxIter=20000, deflate=3)
+    adj_prec = H.FP64 if args.adjoint_precision == "fp64" else H.FP32
 
     def fwd_bwd():
         x = np.zeros(m.n)
         pf = ctx.solve(H.PCG, m.diag, m.upper, None, x, m.source, ctl)
-        # g = x - x_t has a non-zero sum: on the pinned Poisson the refined
-        # transpose solve's constant-mode deflation stalls on such a right-hand
-        # side, so the transpose system is solved in plain FP64 on the GPU
-        pb = ctx.adjoint(H.PCG, m.diag, m.upper, None, x, x - xt, ctl, precision=H.FP64)
+        pb = ctx.adjoint(H.PCG, m.diag, m.upper, None, x, x - xt,
+                         ctl_adj if adj_prec == H.FP32 else ctl, precision=adj_prec)
         return x, pf, pb
 
     for _ in range(args.warmup):
@@ -478,7 +484,14 @@ def run_adjoint(args, H, mg, desc):
         x, pf, (pb, lam, du, _dl, dd, db) = fwd_bwd()
     ms = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
     log(f"adjoint: forward+backward {ms:.1f} ms (host arrays), forward it={pf.nIterations} "
-        f"res={pf.finalResidual:.2e}, adjoint it={pb.nIterations} res={pb.finalResidual:.2e}")
+        f"res={pf.finalResidual:.2e}, adjoint ({args.adjoint_precision}) it={pb.nIterations} "
+        f"res={pb.finalResidual:.2e}")
+    # the same gradients from a plain-FP64 transpose solve at the same tolerance
+    p64, lam64, du64, _dl64, dd64, db64 = ctx.adjoint(H.PCG, m.diag, m.upper, None, x, x - xt,
+                                                       ctl, precision=H.FP64)
+    grad_diff = max(float(np.abs(a - b).max() / np.abs(b).max())
+                    for a, b in ((du, du64), (dd, dd64), (db, db64)))
+    log(f"adjoint gradients vs FP64 transpose solve: max relative difference {grad_diff:.2e}")
     fd = None
     if args.fd_faces > 0:
         rng = np.random.default_rng(1004)
@@ -524,7 +537,10 @@ def run_adjoint(args, H, mg, desc):
     out = _common(args, round(ms, 3), "ms", ms, desc, False)
     out["config"].update({"cells": m.n, "faces": m.nfaces, "tolerance": 1e-6})
     out["adjoint"] = {"forward_iterations": pf.nIterations, "adjoint_iterations": pb.nIterations,
-                      "adjoint_precision": "fp64", "adjoint_residual": pb.finalResidual,
+                      "adjoint_precision": ("fp32 refined, deflate 3 (pinned projection)"
+                                            if adj_prec == H.FP32 else "fp64"),
+                      "adjoint_residual": pb.finalResidual,
+                      "gradients_vs_fp64_adjoint_max_rel_diff": grad_diff,
                       "adjoint_converged": bool(pb.converged),
                       "fd_check": fd}
     out["roofline"] = None

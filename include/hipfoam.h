@@ -62,7 +62,12 @@ typedef struct
     double relTol;             /* default 0                               */
     int32_t maxIter;           /* default 1000 (caps total inner iters)   */
     int32_t minIter;           /* default 0                               */
-    int32_t deflate;           /* 0 off, 1 on, 2 auto (default)           */
+    int32_t deflate;           /* 0 off, 1 on, 2 auto (default), 3 pinned:
+                                  auto, with the inner right-hand side
+                                  projected along u = A 1 instead of the
+                                  constant (no reference counterpart; lets
+                                  right-hand sides with sum != 0 on pinned
+                                  Neumann matrices converge)               */
     int32_t chebyshevOrder;    /* default 8 (PCG); 1 = Jacobi             */
     double chebyshevRatio;     /* default 30                              */
     int32_t maxRestarts;       /* default 8                               */

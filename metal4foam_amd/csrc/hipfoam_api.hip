@@ -690,11 +690,12 @@ int run_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int 
 }
 
 // Inner correction solve in reference-backend semantics
-// (metalFoamBackend::pcg / ::bicgstab): r64 -> FP32 residual (minus *mean
-// when given), delta0 = 0, device loop, best-iterate restore.
+// (metalFoamBackend::pcg / ::bicgstab): r64 -> FP32 residual (minus *mean,
+// times proj when given, when mean is given), delta0 = 0, device loop,
+// best-iterate restore.
 int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* mean,
-                double target, int maxIter, int ips, int order, double beta, double ratio,
-                hipfoam_stats* stats) {
+                const double* proj, double target, int maxIter, int ips, int order,
+                double beta, double ratio, hipfoam_stats* stats) {
   const double tw0 = now_seconds();
   hipfoam_stats st{};
   if (!c->good || c->n == 0) {
@@ -706,7 +707,7 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
   Work<float>& w = c->w32;
   const int n = c->n;
   const dim3 g1(blocks_for(n, 256)), b1(256);
-  hipLaunchKernelGGL((k_inner_init<float>), g1, b1, 0, c->stream, n, r64, mean, w.r.p,
+  hipLaunchKernelGGL((k_inner_init<float>), g1, b1, 0, c->stream, n, r64, mean, proj, w.r.p,
                      kind == KIND_BICG ? w.rA0.p : nullptr, w.delta.p, w.p0.p, w.p1.p,
                      kind == KIND_BICG ? w.AyA.p : nullptr);
   CKL();
@@ -841,7 +842,7 @@ int initial_residual(hipfoam_context* c, const double* psiCur, const double* src
   const dim3 g(row_grid(c)), bl(bs);
   hipLaunchKernelGGL(k_rowsum, g, bl, 0, c->stream, A, psiCur, c->u64.p, c->ost.p, c->partials.p,
                      c->pstride);
-  launch_fin<3>(c, g.x, false, 0, FinRowsum{c->ost.p, c->n});
+  launch_fin<4>(c, g.x, false, 0, FinRowsum{c->ost.p, c->n});
   hipLaunchKernelGGL(k_initial_residual, g, bl, 0, c->stream, A, psiCur, src, c->u64.p,
                      c->r64.p, c->ost.p, c->partials.p, c->pstride);
   launch_fin<3>(c, g.x, false, 0, FinInitialResidual{c->ost.p, c->n});
@@ -878,6 +879,8 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
   const bool useBiCG = solver == HIPFOAM_PBICGSTAB;
   const int n = c->n;
   hipfoam_performance pf{};
+  if (ctl->deflate < 0 || ctl->deflate > 3)
+    return fail(c, HIPFOAM_ERR_ARG, "deflate must be 0 (off), 1 (on), 2 (auto) or 3 (pinned)");
   rc = ensure_vectors(c);
   if (rc) return rc;
   // lower is used only by PBiCGStab on asymmetric matrices (metalFoamSolver.C:214-219)
@@ -898,6 +901,9 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
   if (ctl->minIter > 0 ||
       !of_converged_h(pf.finalResidual, pf.initialResidual, ctl->tolerance, ctl->relTol)) {
     // deflation (metalFoamSolver.C:170-190)
+    // deflate 3: auto detection, inner right-hand side projected along u
+    // instead of the constant (no reference counterpart)
+    const bool pinnedProj = ctl->deflate == 3;
     bool deflate = false;
     if (!useBiCG && ctl->deflate != 0) {
       if (ctl->deflate == 1)
@@ -939,7 +945,8 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
       hipfoam_stats st{};
       // zero-mean projection of the inner RHS (metalFoamSolver.C:278-287)
       rc = inner_solve(c, useBiCG ? KIND_BICG : KIND_PCG, c->r64.p,
-                       deflate ? &c->ost.p->rmean : nullptr, target, ctl->maxIter - totalIter,
+                       deflate ? (pinnedProj ? &c->ost.p->rproj : &c->ost.p->rmean) : nullptr,
+                       deflate && pinnedProj ? c->u64.p : nullptr, target, ctl->maxIter - totalIter,
                        ctl->iterationsPerSync, chebOrder, chebBeta, chebRatio, &st);
       if (rc) return rc;
       totalIter += st.nIterations;
@@ -1055,8 +1062,9 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
     pf.chebBeta = beta;
     hipLaunchKernelGGL(k_recip, g1, b1, 0, c->stream, n, c->diag64.p, c->rD64.p);
     hipLaunchKernelGGL((k_inner_init<double>), g1, b1, 0, c->stream, n, c->r64.p,
-                       (const double*)nullptr, w.r.p, useBiCG ? w.rA0.p : nullptr,
-                       (double*)nullptr, w.p0.p, w.p1.p, useBiCG ? w.AyA.p : nullptr);
+                       (const double*)nullptr, (const double*)nullptr, w.r.p,
+                       useBiCG ? w.rA0.p : nullptr, (double*)nullptr, w.p0.p, w.p1.p,
+                       useBiCG ? w.AyA.p : nullptr);
     hipLaunchKernelGGL((k_copy<double>), g1, b1, 0, c->stream, n, c->psi0.p, w.delta.p);
     CKL();
     KState s{};
@@ -1424,7 +1432,7 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
                       hipMemcpyHostToDevice, c->stream));
   }
   c->pstride = std::max(1, blocks_for(nCells, 64));
-  CK(c->partials.ensure((size_t)c->pstride * 3));
+  CK(c->partials.ensure((size_t)c->pstride * 4));
   c->w32.release();
   c->w64.release();
   int rc = ensure_vectors(c);
@@ -1464,7 +1472,8 @@ static int32_t inner_entry(hipfoam_context* c, Kind kind, const double* r, doubl
   if (order < 1 || order > kMaxCheb + 1)
     return fail(c, HIPFOAM_ERR_ARG, "chebOrder must be in [1, 33]");
   CK(hipMemcpyAsync(c->r64.p, r, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
-  rc = inner_solve(c, kind, c->r64.p, nullptr, target, maxIter, ips, order, beta, ratio, stats);
+  rc = inner_solve(c, kind, c->r64.p, nullptr, nullptr, target, maxIter, ips, order, beta, ratio,
+                   stats);
   if (rc) return rc;
   CK(hipStreamSynchronize(c->stream));
   return HIPFOAM_OK;

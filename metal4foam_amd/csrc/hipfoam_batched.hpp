@@ -540,14 +540,15 @@ __global__ __launch_bounds__(256) void kb_any_running(const KState* stBase, int 
 // FP64 refinement steps, per segment
 // ---------------------------------------------------------------------------
 
-// u = A 1 (once per system, written for rhs 0 of the segment only), Σ psi, pinned count, u.u
+// u = A 1 (once per system, written for rhs 0 of the segment only), Σ psi, pinned count,
+// u.u, Σ u
 __global__ __launch_bounds__(256) void kb_rowsum(BatchMap B, LduOp<double> A,
                                                  const double* __restrict__ psi,
                                                  double* __restrict__ u, const int32_t* active,
                                                  double* partials, int pstride) {
   const int lb = logical_block(B.xcdRemap);
   const BlockSeg g = block_seg(B, lb);
-  double v[3] = {0.0, 0.0, 0.0};
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
   const int i = g.r0 + threadIdx.x;
   if (i < g.r1 && active[g.seg]) {
     const double ui = ldu_row(A, i, 1.0, [](int) { return 1.0; });
@@ -555,8 +556,9 @@ __global__ __launch_bounds__(256) void kb_rowsum(BatchMap B, LduOp<double> A,
     v[0] = psi[g.voff + i];
     v[1] = (fabs(ui) > 1e-8 * fabs(A.diag[i])) ? 1.0 : 0.0;
     v[2] = ui * ui;
+    v[3] = ui;
   }
-  block_partials<3>(v, partials, pstride, lb);
+  block_partials<4>(v, partials, pstride, lb);
 }
 
 __global__ __launch_bounds__(256) void kb_initial_residual(BatchMap B, LduOp<double> A,
@@ -619,8 +621,10 @@ __global__ __launch_bounds__(256) void kb_prepare_inner(BatchMap B, LduOp<double
   block_partials<2, true>(v, partials, pstride, lb);
 }
 
-// rA = float(r64 - mean); delta = p0 = p1 = AyA = 0; rA0 = rA, for active segments
+// rA = float(r64 - mean) (deflated segments; with pinned, r64 - rproj u);
+// delta = p0 = p1 = AyA = 0; rA0 = rA, for active segments
 __global__ __launch_bounds__(256) void kb_inner_init(BatchMap B, const double* __restrict__ r64,
+                                                     const double* __restrict__ u, int pinned,
                                                      const OState* osBase, const int32_t* deflated,
                                                      const int32_t* active, float* __restrict__ rA,
                                                      float* __restrict__ rA0,
@@ -633,8 +637,9 @@ __global__ __launch_bounds__(256) void kb_inner_init(BatchMap B, const double* _
   const int i = g.r0 + threadIdx.x;
   if (i >= g.r1 || !active[g.seg]) return;
   const long k = g.voff + i;
-  const double m = deflated[g.seg] ? osBase[g.seg].rmean : 0.0;
-  const float v = (float)(deflated[g.seg] ? r64[k] - m : r64[k]);
+  const bool defl = deflated[g.seg] != 0;
+  const double m = defl ? (pinned ? osBase[g.seg].rproj : osBase[g.seg].rmean) : 0.0;
+  const float v = (float)(defl ? (pinned ? r64[k] - m * u[i] : r64[k] - m) : r64[k]);
   rA[k] = v;
   if (rA0) rA0[k] = v;
   delta[k] = 0.f;

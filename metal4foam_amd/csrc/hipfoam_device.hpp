@@ -68,6 +68,8 @@ struct alignas(128) OState {
   double sumPsi, xbar, nPinned, uu;  // rowsum pass
   double normFactor, sumMagR, sumR, initRes;
   double rmean;  // mean of the current residual
+  double sumU;   // sum of u = A 1
+  double rproj;  // sum r / sum u: weight of u in the pinned-cell projection
   double dmean;  // mean of the inner correction
   double ur, c, finalRes;
   double bail, betaDefl, betaExact;
@@ -672,17 +674,21 @@ __global__ __launch_bounds__(1024) void k_bicg_update_rho(int n, int xcdRemap,
 // inner-solve initialisation
 // ---------------------------------------------------------------------------
 
-// rA = T(r64 - mean); delta = p0 = p1 = AyA = 0; rA0 = rA (BiCGStab).
+// rA = T(r64 - m v) with m = *mean and v = proj (or 1 without proj): the
+// zero-mean projection of the deflated inner right-hand side, or with
+// proj = u = A 1 the projection along the pinned rows; delta = p0 = p1 =
+// AyA = 0; rA0 = rA (BiCGStab).
 template <typename T>
 __global__ __launch_bounds__(1024) void k_inner_init(int n, const double* __restrict__ r64,
-                                                     const double* mean, T* __restrict__ rA,
-                                                     T* __restrict__ rA0, T* __restrict__ delta,
-                                                     T* __restrict__ p0, T* __restrict__ p1,
-                                                     T* __restrict__ AyA) {
+                                                     const double* mean,
+                                                     const double* __restrict__ proj,
+                                                     T* __restrict__ rA, T* __restrict__ rA0,
+                                                     T* __restrict__ delta, T* __restrict__ p0,
+                                                     T* __restrict__ p1, T* __restrict__ AyA) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double m = mean ? *mean : 0.0;
-  const T v = (T)(mean ? r64[i] - m : r64[i]);
+  const T v = (T)(mean ? (proj ? r64[i] - m * proj[i] : r64[i] - m) : r64[i]);
   rA[i] = v;
   if (rA0) rA0[i] = v;
   if (delta) delta[i] = (T)0;
@@ -702,15 +708,16 @@ __global__ __launch_bounds__(1024) void k_rowsum(LduOp<double> A, const double* 
   const int lb = logical_block(A.xcdRemap);
   int r0, r1;
   block_rows(A.n, lb, r0, r1);
-  double v[3] = {0.0, 0.0, 0.0};
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
   for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const double ui = ldu_row(A, i, 1.0, [](int) { return 1.0; });
     u[i] = ui;
     v[0] += psi[i];
     v[1] += (fabs(ui) > 1e-8 * fabs(A.diag[i])) ? 1.0 : 0.0;
     v[2] += ui * ui;
+    v[3] += ui;
   }
-  block_partials<3>(v, partials, pstride, lb);
+  block_partials<4>(v, partials, pstride, lb);
 }
 
 // wA = A psi; r = b - wA; partials: sum|r|, normFactor sum, sum r.
@@ -1127,8 +1134,13 @@ struct FinRowsum {
     os->xbar = t[0] / (double)n;
     os->nPinned = t[1];
     os->uu = t[2];
+    os->sumU = t[3];
   }
 };
+
+__device__ __forceinline__ double pinned_weight(const OState* os) {
+  return os->sumU != 0.0 ? os->sumR / os->sumU : 0.0;
+}
 
 struct FinInitialResidual {
   OState* os;
@@ -1140,6 +1152,7 @@ struct FinInitialResidual {
     os->finalRes = os->initRes;
     os->sumR = t[2];
     os->rmean = t[2] / (double)n;
+    os->rproj = pinned_weight(os);
   }
 };
 
@@ -1172,6 +1185,7 @@ struct FinRefine {
     os->finalRes = t[1] / os->normFactor;
     os->sumR = t[2];
     os->rmean = t[2] / (double)n;
+    os->rproj = pinned_weight(os);
     os->c = (os->uu > kVSmall) ? t[0] / os->uu : 0.0;
   }
 };
@@ -1184,6 +1198,7 @@ struct FinShift {
     os->finalRes = t[0] / os->normFactor;
     os->sumR = t[1];
     os->rmean = t[1] / (double)n;
+    os->rproj = pinned_weight(os);
   }
 };
 

@@ -153,7 +153,8 @@ Foam::solverPerformance Foam::hipFoamSolver::solveImpl
     ctl.maxIter = maxIter_;
     ctl.minIter = minIter_;
     const word deflate = hipDict_.getOrDefault<word>("deflate", "auto");
-    ctl.deflate = (deflate == "off" ? 0 : deflate == "on" ? 1 : 2);
+    ctl.deflate =
+        deflate == "off" ? 0 : deflate == "on" ? 1 : deflate == "pinned" ? 3 : 2;
     ctl.chebyshevOrder = hipDict_.getOrDefault<label>("chebyshevOrder", 8);
     ctl.chebyshevRatio = hipDict_.getOrDefault<scalar>("chebyshevRatio", 30.0);
     ctl.maxRestarts = hipDict_.getOrDefault<label>("maxRestarts", 8);

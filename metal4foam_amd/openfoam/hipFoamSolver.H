@@ -14,7 +14,10 @@ Description
     metalFoamSolver.C:77-437), whose FP64 steps ran on the CPU.
 
     Optional controls, in a "hip" sub-dictionary of the solver entry:
-        deflate            auto | on | off   (auto)
+        deflate            auto | on | off | pinned   (auto; pinned: auto
+                           detection, inner right-hand side projected along
+                           A 1, for sources with a non-zero sum on a matrix
+                           with pinned reference cells)
         chebyshevOrder     8                 (hipPCG; 1 = Jacobi)
         chebyshevRatio     30
         maxRestarts        8

@@ -239,3 +239,18 @@ def test_plugin_min_and_max_iter_controls(driver, tmp_path):
                               minIter=3)
     assert r.returncode == 0, r.stdout + r.stderr
     assert res[0]["iters"] >= 3 and res[0]["converged"]
+
+
+@pytest.mark.gpu
+def test_plugin_deflate_pinned_keyword(driver, tmp_path):
+    """hip { deflate pinned; } selects the projection along A 1: a pinned
+    Poisson whose source has a non-zero sum converges in one restart."""
+    m = mg.poisson(16, 14, 12, seed=45, lognormal=True, zero_mean_rhs=False, rhs="normal")
+    r, res, _, arrays = run_driver(driver, tmp_path, m, np.zeros(m.n), "hipPCG",
+                                   tolerance=1e-9, hip=dict(deflate="pinned", profile="true",
+                                                            maxRestarts=8))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert res[0]["converged"]
+    assert "restarts=1" in r.stdout or "restarts=0" in r.stdout
+    x = np.linalg.solve(mg.dense(m), m.source)
+    assert np.abs(arrays[0] - x).max() <= 1e-6 * np.abs(x).max()
