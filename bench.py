@@ -25,6 +25,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from metal4foam_amd.profsum import slot_matrix_bytes  # noqa: E402
+
 METRIC = ("time-to-1e-6 FP64 residual & SpMV HBM GB/s (%roofline), "
           "refined vs plain FP64")
 HBM_PEAK_GBS = 8000.0
@@ -80,12 +82,13 @@ def cheb_step_bytes(n, nf, symmetric):
     return 36 * n + 8 + face * nf
 
 
-def to_slot_bytes(ldu_bytes, n, nf, symmetric):
-    """The same kernel on the row-slot layout: the two row-start arrays and
-    the per-face arrays are replaced by six FP32 coefficient slots and three
-    packed int16 offset words per row (36 B/row, every array read once)."""
+def to_slot_bytes(ldu_bytes, n, nf, symmetric, patterned_rows):
+    """The same FP32 kernel on the row-slot layout: the two row-start arrays
+    and the per-face arrays are replaced by the row-slot matrix (coefficient
+    planes, slot bytes, offsets of explicit tiles, tile descriptors)."""
     face = 16 if symmetric else 20
-    return ldu_bytes - (8 * n + 8 + face * nf) + 36 * n
+    return (ldu_bytes - (8 * n + 8 + face * nf)
+            + slot_matrix_bytes(n, not symmetric, 4, patterned_rows))
 
 
 def cpu_baseline(m, solver, sample_iters):
@@ -234,6 +237,7 @@ def main():
     ctx.set_kernel_timing(False, 0)
     layout = ctx.active_schedule()
     slots = layout == 2
+    patterned, _ = ctx.slot_layout()
     if solver == "pcg":
         ref_kbytes = cheb_step_bytes(m.n, m.nfaces, sym)
         kname = "k_cheb_step<float, false, false> (interior Chebyshev step fused with its SpMV)"
@@ -242,7 +246,7 @@ def main():
         # and k_bicg_sz_spmv (reads r, rD, AyA; writes sA, zA, tA)
         ref_kbytes = spmv_bytes(m.n, m.nfaces, sym, extra_vectors=5)
         kname = "k_bicg_py_spmv / k_bicg_sz_spmv (BiCGStab SpMVs fused with their vector updates)"
-    kbytes = to_slot_bytes(ref_kbytes, m.n, m.nfaces, sym) if slots else ref_kbytes
+    kbytes = to_slot_bytes(ref_kbytes, m.n, m.nfaces, sym, patterned) if slots else ref_kbytes
     kavg = secs / max(nl, 1)
     kgbs = kbytes / kavg / 1e9 if nl else 0.0
     ref_kgbs = ref_kbytes / kavg / 1e9 if nl else 0.0
@@ -259,7 +263,7 @@ def main():
     ns, ssecs = ctx.kernel_timing()
     ctx.set_kernel_timing(False, 0)
     ref_sbytes = spmv_bytes(m.n, m.nfaces, sym)
-    sbytes = to_slot_bytes(ref_sbytes, m.n, m.nfaces, sym) if slots else ref_sbytes
+    sbytes = to_slot_bytes(ref_sbytes, m.n, m.nfaces, sym, patterned) if slots else ref_sbytes
     savg = ssecs / max(ns, 1)
     sgbs = sbytes / savg / 1e9 if ns else 0.0
     ref_sgbs = ref_sbytes / savg / 1e9 if ns else 0.0
@@ -293,7 +297,10 @@ def main():
     if os.path.exists(tfile):
         tj = json.load(open(tfile))
         # counters measured on the layout this run uses only
-        if tj.get("layout", "LDU face") == ("row-slot" if slots else "LDU face"):
+        same = tj.get("layout", "LDU face") == ("row-slot" if slots else "LDU face")
+        if same and slots:
+            same = tj.get("patterned_rows") == patterned
+        if same:
             traffic = tj.get("bytes_per_launch")
 
     cpu = None
@@ -334,6 +341,7 @@ def main():
             "traffic": traffic,
             "kernel": kname,
             "layout": "row-slot" if slots else "LDU face",
+            "patterned_rows": patterned if slots else None,
             "bytes_per_launch": kbytes,
             "reference_format_bytes": ref_kbytes,
             "reference_format_GB_s": round(ref_kgbs, 1),

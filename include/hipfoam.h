@@ -324,6 +324,14 @@ int32_t hipfoam_get_schedule(const hipfoam_context* ctx, hipfoam_schedule* s);
  * not allow them.  No reference counterpart (the reference has one kernel). */
 int32_t hipfoam_active_schedule(const hipfoam_context* ctx,
                                 int32_t* spmvSchedule);
+/* Row-slot layout of the current mesh (schedule 2): *patternedRows = rows
+ * in tiles whose faces all sit at the tile's per-slot column offsets, which
+ * need 1 byte of addressing per row instead of 13; *symmetricPlanes = the
+ * number of coefficient planes a symmetric matrix stores (3: each face's
+ * coefficient once, in its owner row; an asymmetric matrix stores 6).  Both
+ * 0 when the mesh does not allow row slots.  No reference counterpart. */
+int32_t hipfoam_slot_layout(const hipfoam_context* ctx, int64_t* patternedRows,
+                            int32_t* symmetricPlanes);
 int32_t hipfoam_set_schedule(hipfoam_context* ctx, const hipfoam_schedule* s);
 /* Times the candidate SpMV schedules on the staged operator and keeps the
  * fastest; *best (may be NULL) receives it. */

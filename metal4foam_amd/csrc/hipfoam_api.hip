@@ -26,6 +26,7 @@
 #include "../../include/hipfoam.h"
 #include "hipfoam_batched.hpp"
 #include "hipfoam_device.hpp"
+#include "hipfoam_slots.hpp"
 
 using namespace hipfoam;
 
@@ -136,9 +137,10 @@ struct hipfoam_context {
   int n = 0, nf = 0;
   int maxOwn = 0, maxNbr = 0;  // longest owner / neighbour face run of a row
   bool slotOK = false;  // row-slot layout possible (see build_slots)
-  bool slotBased = false;  // some tile base is not zero
+  int64_t slotPatterned = 0;  // rows in patterned tiles
   Buf<uint32_t> slotOff;
-  Buf<int32_t> slotBase;
+  Buf<int32_t> slotTile;
+  Buf<uint8_t> slotRow;
   Buf<float> slotCoef32;
   Buf<double> slotCoef64;
   const int32_t* lastLow = nullptr;
@@ -178,156 +180,6 @@ struct hipfoam_context {
 
 namespace {
 
-// Row-slot layout of a mesh, built on the host once per topology.  Each row's
-// owner faces go, in face order, into increasing slots among 0..2, and its
-// neighbour faces, in losort order, into increasing slots among 3..5 (a row
-// with fewer faces leaves slots empty, in any position, so the face order and
-// hence the summation order is kept).  Every kSlotTile-row tile has one column
-// offset base per slot; a face is stored as the int16 delta of its offset
-// (column - row) from its slot's base.
-//
-// Candidate bases of a tile side, tried in turn until every row of the tile
-// fits: the median of each position over the rows with all three faces of
-// that side; then the centres of the clusters of the tile's offsets (values
-// within an int16 window), spread over the three slots in increasing order.
-// A row takes the placement of its faces that keeps its largest |delta|
-// smallest.  False when a row has more than kShortRow faces on a side or no
-// candidate fits a tile.  words: slotOff[p*n + i] of LduOp (deltas of slots
-// 2p and 2p+1, kSlotEmpty for an empty slot; based is false, and an empty
-// slot holds 0, when every base is zero).
-bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
-                 const int32_t* losortStart, const int32_t* losort, const int32_t* lowAddr,
-                 std::vector<int32_t>& base, std::vector<uint32_t>& words, bool& based) {
-  constexpr int K = kShortRow, S = 2 * kShortRow;
-  constexpr int64_t kHalf = 32767;
-  const int nt = (n + kSlotTile - 1) / kSlotTile;
-  base.assign((size_t)nt * S, 0);
-  words.assign((size_t)K * n, 0);
-  std::vector<int16_t> delta((size_t)S * kSlotTile);
-  // offsets of row i on one side (owner: side 0, neighbour: side 1), ascending
-  auto offsets = [&](int i, int side, int64_t* off) -> int {
-    if (side == 0) {
-      const int f0 = ownStart[i], cnt = ownStart[i + 1] - f0;
-      for (int q = 0; q < cnt && q < K; ++q) off[q] = (int64_t)uppAddr[f0 + q] - i;
-      return cnt;
-    }
-    const int k0 = losortStart[i], cnt = losortStart[i + 1] - k0;
-    for (int q = 0; q < cnt && q < K; ++q) off[q] = (int64_t)lowAddr[losort[k0 + q]] - i;
-    return cnt;
-  };
-  // best increasing placement of cnt offsets on bases bs; false if it does not fit
-  auto place = [&](const int64_t* off, int cnt, const int64_t* bs, int* pos) -> bool {
-    int64_t bestErr = INT64_MAX;
-    for (int mask = 0; mask < (1 << K); ++mask) {
-      if (__builtin_popcount(mask) != cnt) continue;
-      int ps[K], m = 0;
-      for (int q = 0; q < K; ++q)
-        if (mask >> q & 1) ps[m++] = q;
-      int64_t err = 0;
-      for (int j = 0; j < cnt; ++j) {
-        const int64_t e = off[j] - bs[ps[j]];
-        err = std::max(err, e < 0 ? -e : e);
-      }
-      if (err < bestErr) {
-        bestErr = err;
-        for (int j = 0; j < cnt; ++j) pos[j] = ps[j];
-      }
-    }
-    return cnt == 0 || bestErr <= kHalf;
-  };
-  std::vector<int64_t> full[K], all;
-  for (int q = 0; q < K; ++q) full[q].reserve(kSlotTile);
-  all.reserve((size_t)K * kSlotTile);
-  // places every row of the tile side on bases bs; false if one does not fit
-  auto try_bases = [&](int i0, int i1, int side, const int64_t* bs) -> bool {
-    for (int i = i0; i < i1; ++i) {
-      int64_t off[K];
-      int pos[K];
-      const int cnt = offsets(i, side, off);
-      if (!place(off, cnt, bs, pos)) return false;
-      int16_t* d = &delta[(size_t)(i - i0) * S + side * K];
-      for (int q = 0; q < K; ++q) d[q] = (int16_t)kSlotEmpty;
-      for (int j = 0; j < cnt; ++j) d[pos[j]] = (int16_t)(off[j] - bs[pos[j]]);
-    }
-    return true;
-  };
-  for (int t = 0; t < nt; ++t) {
-    const int i0 = t * kSlotTile, i1 = std::min(n, i0 + kSlotTile);
-    for (int side = 0; side < 2; ++side) {
-      for (int q = 0; q < K; ++q) full[q].clear();
-      all.clear();
-      for (int i = i0; i < i1; ++i) {
-        int64_t off[K];
-        const int cnt = offsets(i, side, off);
-        if (cnt > K) return false;
-        for (int q = 0; q < cnt; ++q) {
-          all.push_back(off[q]);
-          if (cnt == K) full[q].push_back(off[q]);
-        }
-      }
-      int64_t chosen[K] = {0, 0, 0};
-      // zero bases when every offset of the tile side fits an int16 delta as it is
-      bool ok = try_bases(i0, i1, side, chosen);
-      if (!ok && !full[0].empty()) {
-        for (int q = 0; q < K; ++q) {
-          std::nth_element(full[q].begin(), full[q].begin() + full[q].size() / 2, full[q].end());
-          chosen[q] = full[q][full[q].size() / 2];
-        }
-        ok = try_bases(i0, i1, side, chosen);
-      }
-      if (!ok) {
-        std::sort(all.begin(), all.end());
-        int64_t centre[K + 1];
-        int nc = 0;
-        for (size_t a = 0; a < all.size() && nc <= K;) {
-          size_t e = a;
-          while (e + 1 < all.size() && all[e + 1] - all[a] <= 2 * kHalf) ++e;
-          centre[nc++] = all[a] + (all[e] - all[a]) / 2;
-          a = e + 1;
-        }
-        if (nc == 0) centre[nc++] = 0;
-        int64_t cand[2][K];
-        int ncand = 0;
-        if (nc == 1) {
-          for (int q = 0; q < K; ++q) cand[0][q] = centre[0];
-          ncand = 1;
-        } else if (nc == 2) {
-          const int64_t a2[2][K] = {{centre[0], centre[1], centre[1]},
-                                    {centre[0], centre[0], centre[1]}};
-          std::copy(&a2[0][0], &a2[0][0] + 2 * K, &cand[0][0]);
-          ncand = 2;
-        } else if (nc == 3) {
-          for (int q = 0; q < K; ++q) cand[0][q] = centre[q];
-          ncand = 1;
-        }
-        for (int c = 0; c < ncand && !ok; ++c) {
-          ok = try_bases(i0, i1, side, cand[c]);
-          if (ok) std::copy(cand[c], cand[c] + K, chosen);
-        }
-      }
-      if (!ok) return false;
-      for (int q = 0; q < K; ++q) {
-        if (chosen[q] < INT32_MIN || chosen[q] > INT32_MAX) return false;
-        base[(size_t)t * S + side * K + q] = (int32_t)chosen[q];
-      }
-    }
-    for (int i = i0; i < i1; ++i)
-      for (int p = 0; p < K; ++p) {
-        const int16_t* d = &delta[(size_t)(i - i0) * S];
-        words[(size_t)p * n + i] =
-            (uint32_t)(uint16_t)d[2 * p] | ((uint32_t)(uint16_t)d[2 * p + 1] << 16);
-      }
-  }
-  // every base zero: offsets are their own deltas and 0 marks an empty slot
-  based = std::any_of(base.begin(), base.end(), [](int32_t v) { return v != 0; });
-  if (!based)
-    for (auto& w : words) {
-      const uint32_t lo = (w & 0xffffu) == 0x8000u ? 0u : (w & 0xffffu);
-      const uint32_t hi = (w >> 16) == 0x8000u ? 0u : (w >> 16);
-      w = lo | (hi << 16);
-    }
-  return true;
-}
 
 // Workgroups of a row-kernel launch: one per tile of blockSize rows, capped at
 // gridLimit (each workgroup then walks a contiguous run of tiles).
@@ -412,7 +264,9 @@ LduOp<T> make_op(hipfoam_context* c, const T* diag, const T* upper, const T* low
   op.slots = slotCoef && c->sched.spmvSchedule == 2 && c->slotOK;
   op.slotCoef = op.slots ? slotCoef : nullptr;
   op.slotOff = op.slots ? c->slotOff.p : nullptr;
-  op.slotBase = op.slots && c->slotBased ? c->slotBase.p : nullptr;
+  op.slotTile = op.slots ? c->slotTile.p : nullptr;
+  op.slotRow = op.slots ? c->slotRow.p : nullptr;
+  op.slotSym = op.slots && lowerF == upper;
   if (op.direct) {
     op.nbrAddr = c->lowAddr.p;
     op.lowerL = lowerF;
@@ -606,7 +460,8 @@ void enqueue_chunk(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD
 template <typename T>
 int get_graph(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
               int chunk, hipGraphExec_t* out) {
-  GraphKey key{(int)kind, (int)sizeof(T), order, chunk, A.direct + 2 * A.slots, A.xcdRemap,
+  GraphKey key{(int)kind, (int)sizeof(T), order, chunk, A.direct + 2 * A.slots + 4 * A.slotSym,
+               A.xcdRemap,
                c->sched.blockSize, row_grid(c), A.shortRows, c->gen};
   auto it = c->graphs.find(key);
   if (it != c->graphs.end()) {
@@ -803,12 +658,15 @@ int stage_matrix(hipfoam_context* c, const double* diag, const double* upper,
   if (c->slotOK && n > 0) {
     CK(c->slotCoef32.ensure((size_t)2 * kShortRow * n));
     CK(c->slotCoef64.ensure((size_t)2 * kShortRow * n));
+    const int sym = lower == nullptr;
     hipLaunchKernelGGL((k_build_slot_coeffs<float>), dim3(blocks_for(n, 256)), dim3(256), 0,
                        c->stream, n, c->ownStart.p, c->losortStart.p, c->slotOff.p,
-                       (int)c->slotBased, c->upper32.p, c->lowerL32.p, c->slotCoef32.p);
+                       c->slotTile.p, c->slotRow.p, sym, c->upper32.p, c->lowerL32.p,
+                       c->slotCoef32.p);
     hipLaunchKernelGGL((k_build_slot_coeffs<double>), dim3(blocks_for(n, 256)), dim3(256), 0,
                        c->stream, n, c->ownStart.p, c->losortStart.p, c->slotOff.p,
-                       (int)c->slotBased, c->upper64.p, c->lowerL64.p, c->slotCoef64.p);
+                       c->slotTile.p, c->slotRow.p, sym, c->upper64.p, c->lowerL64.p,
+                       c->slotCoef64.p);
     CKL();
   }
   c->have64 = true;
@@ -1394,10 +1252,9 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
     c->maxOwn = std::max(c->maxOwn, ownStart[i + 1] - ownStart[i]);
     c->maxNbr = std::max(c->maxNbr, losortStart[i + 1] - losortStart[i]);
   }
-  std::vector<int32_t> bases;
-  std::vector<uint32_t> words;
-  c->slotOK = build_slots(nCells, ownStart, uppAddr, losortStart, losort, lowAddr, bases,
-                          words, c->slotBased);
+  SlotLayout lay;
+  c->slotOK = build_slots(nCells, ownStart, uppAddr, losortStart, losort, lowAddr, lay);
+  c->slotPatterned = c->slotOK ? lay.patternedRows : 0;
   c->lastLow = lowAddr;
   c->lastUpp = uppAddr;
   c->have32 = c->have64 = false;
@@ -1424,12 +1281,17 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
     CKL();
   }
   if (c->slotOK && nCells > 0) {
-    CK(c->slotOff.ensure((size_t)kShortRow * nCells));
-    CK(c->slotBase.ensure(bases.size()));
-    CK(hipMemcpyAsync(c->slotBase.p, bases.data(), sizeof(int32_t) * bases.size(),
+    CK(c->slotOff.ensure(lay.words.size()));
+    CK(c->slotTile.ensure(lay.tile.size()));
+    CK(c->slotRow.ensure(lay.row.size()));
+    CK(hipMemcpyAsync(c->slotTile.p, lay.tile.data(), sizeof(int32_t) * lay.tile.size(),
                       hipMemcpyHostToDevice, c->stream));
-    CK(hipMemcpyAsync(c->slotOff.p, words.data(), sizeof(uint32_t) * words.size(),
+    CK(hipMemcpyAsync(c->slotOff.p, lay.words.data(), sizeof(uint32_t) * lay.words.size(),
                       hipMemcpyHostToDevice, c->stream));
+    CK(hipMemcpyAsync(c->slotRow.p, lay.row.data(), lay.row.size(), hipMemcpyHostToDevice,
+                      c->stream));
+    // the host layout must outlive the asynchronous copies
+    CK(hipStreamSynchronize(c->stream));
   }
   c->pstride = std::max(1, blocks_for(nCells, 64));
   CK(c->partials.ensure((size_t)c->pstride * 4));
@@ -1711,6 +1573,14 @@ int32_t hipfoam_active_schedule(const hipfoam_context* c, int32_t* spmvSchedule)
   if (!c || !spmvSchedule) return HIPFOAM_ERR_ARG;
   const int s = c->sched.spmvSchedule;
   *spmvSchedule = (s == 2 && !c->slotOK) ? 0 : s;
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_slot_layout(const hipfoam_context* c, int64_t* patternedRows,
+                            int32_t* symmetricPlanes) {
+  if (!c || !patternedRows || !symmetricPlanes) return HIPFOAM_ERR_ARG;
+  *patternedRows = c->slotOK ? c->slotPatterned : 0;
+  *symmetricPlanes = c->slotOK ? kShortRow : 0;
   return HIPFOAM_OK;
 }
 

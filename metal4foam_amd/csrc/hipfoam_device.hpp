@@ -29,6 +29,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "hipfoam_slots.hpp"
 
 namespace hipfoam {
 
@@ -88,14 +89,24 @@ struct LduOp {
   const T* __restrict__ lowerL;  // staged: lower[losort[k]]; direct: lower[f]
   // Row-slot layout (slots != 0): row i's owner faces in slots 0..2 and its
   // neighbour faces in slots 3..5, in the order the face loops visit them.
-  // slotCoef[q*n + i] is the coefficient of slot q.  The column of slot q is
-  // i + slotBase[(i / kSlotTile)*6 + q] + delta, with the int16 deltas of
-  // slots 2p and 2p+1 packed in slotOff[p*n + i] and kSlotEmpty marking an
-  // empty slot; without slotBase (every offset fits int16) the delta is the
-  // offset itself and 0 marks an empty slot.
+  // Each kSlotTile-row tile has a descriptor slotTile[t*kSlotDesc + ...]: one
+  // column offset base per slot (words 0..5) and flags (word 6).
+  //  - Patterned tile (flags & kSlotPatterned): every face of the tile sits
+  //    exactly at its slot's base, so the column of slot q is i + base[q];
+  //    slotRow[i] holds the row's presence mask (bit q: slot q holds a face).
+  //  - Explicit tile: the column of slot q is i + base[q] + delta, the int16
+  //    deltas of slots 2p and 2p+1 packed in slotOff[p*n + i], kSlotEmpty
+  //    marking an empty slot.
+  // slotCoef[q*n + i] is the coefficient of slot q.  A symmetric operator
+  // (slotSym) stores the owner slots only: the coefficient of neighbour slot
+  // q is the owner-side coefficient of the same face, slotCoef[s*n + col],
+  // where s is the face's owner slot in row col: bits 2(q-3)..2(q-3)+1 of
+  // flags >> 8 in a patterned tile, of slotRow[i] in an explicit tile.
   const T* __restrict__ slotCoef;
   const uint32_t* __restrict__ slotOff;
-  const int32_t* __restrict__ slotBase;
+  const int32_t* __restrict__ slotTile;
+  const uint8_t* __restrict__ slotRow;
+  int32_t slotSym;
   int32_t n;
   int32_t direct;
   int32_t xcdRemap;
@@ -103,11 +114,6 @@ struct LduOp {
   int32_t slots;
 };
 
-// Rows of hexahedral meshes have at most 3 owner and 3 neighbour faces.
-constexpr int kShortRow = 3;
-// Rows per offset base of the row-slot layout, and the empty-slot delta.
-constexpr int kSlotTile = 256;
-constexpr int kSlotEmpty = -32768;
 
 // ---------------------------------------------------------------------------
 // launch geometry
@@ -246,35 +252,39 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
 // added in exactly the order of the loop form, so both give identical bits.
 // One row of the row-slot layout held in registers: loaded once, applied to
 // any number of vectors (the right-hand sides of a batch share the matrix).
+// slot_decode with the descriptor read by scalar loads when the wavefront's
+// rows share one tile.
+__device__ __forceinline__ void slot_offsets(const uint32_t* __restrict__ slotOff,
+                                             const int32_t* __restrict__ slotTile,
+                                             const uint8_t* __restrict__ slotRow, size_t n,
+                                             int i, int (&o)[2 * kShortRow], unsigned& own) {
+  const unsigned rb = slotRow[i];
+  const int t = i / kSlotTile;
+  const int t0 = __builtin_amdgcn_readfirstlane(t);
+  if (__all(t == t0))
+    slot_decode(slotTile + (size_t)t0 * kSlotDesc, rb, slotOff, n, i, o, own);
+  else
+    slot_decode(slotTile + (size_t)t * kSlotDesc, rb, slotOff, n, i, o, own);
+}
+
 template <typename T>
 struct RowSlots {
   T c[2 * kShortRow];
   int o[2 * kShortRow];
   __device__ __forceinline__ void load(const LduOp<T>& A, int i) {
     const size_t n = (size_t)A.n;
-    // every load issued before any decoding branch
-    uint32_t w[kShortRow];
+    // the row's own coefficients first: independent coalesced loads
 #pragma unroll
-    for (int p = 0; p < kShortRow; ++p) w[p] = A.slotOff[p * n + i];
+    for (int q = 0; q < 2 * kShortRow; ++q)
+      if (q < kShortRow || !A.slotSym) c[q] = A.slotCoef[q * n + i];
+    unsigned own;
+    slot_offsets(A.slotOff, A.slotTile, A.slotRow, n, i, o, own);
+    if (A.slotSym) {
+      // a neighbour face's coefficient is stored once, in its owner's row
 #pragma unroll
-    for (int q = 0; q < 2 * kShortRow; ++q) c[q] = A.slotCoef[q * n + i];
-    if (!A.slotBase) {
-      // every offset is its own int16 delta; 0 marks an empty slot
-#pragma unroll
-      for (int p = 0; p < kShortRow; ++p) {
-        o[2 * p] = (int)(int16_t)(w[p] & 0xffffu);
-        o[2 * p + 1] = (int)(int16_t)(w[p] >> 16);
-      }
-    } else {
-      // the tile's bases, read once per wavefront when its rows share a tile
-      const int t = i / kSlotTile;
-      const int t0 = __builtin_amdgcn_readfirstlane(t);
-      const int32_t* base = A.slotBase + (__all(t == t0) ? t0 : t) * (2 * kShortRow);
-#pragma unroll
-      for (int p = 0; p < kShortRow; ++p) {
-        const int d0 = (int)(int16_t)(w[p] & 0xffffu), d1 = (int)(int16_t)(w[p] >> 16);
-        o[2 * p] = d0 == kSlotEmpty ? 0 : base[2 * p] + d0;
-        o[2 * p + 1] = d1 == kSlotEmpty ? 0 : base[2 * p + 1] + d1;
+      for (int q = kShortRow; q < 2 * kShortRow; ++q) {
+        const unsigned s = (own >> (2 * (q - kShortRow))) & 3u;
+        c[q] = o[q] != 0 ? A.slotCoef[s * n + (size_t)(i + o[q])] : T(0);
       }
     }
   }
@@ -888,24 +898,25 @@ __global__ __launch_bounds__(1024) void k_gather_addr(int nf, const int32_t* __r
 
 // Row-slot coefficients (layout of LduOp::slotCoef) from the face-ordered
 // upper and losort-ordered lower coefficients: a side's faces fill, in order,
-// the non-empty slots of the row's slotOff words; empty slots hold 0.
+// the row's non-empty slots; empty slots hold 0.  sym: the owner slots only.
 template <typename T>
 __global__ __launch_bounds__(256) void k_build_slot_coeffs(
     int n, const int32_t* __restrict__ ownStart, const int32_t* __restrict__ losortStart,
-    const uint32_t* __restrict__ slotOff, int based, const T* __restrict__ upper,
+    const uint32_t* __restrict__ slotOff, const int32_t* __restrict__ slotTile,
+    const uint8_t* __restrict__ slotRow, int sym, const T* __restrict__ upper,
     const T* __restrict__ lowerL, T* __restrict__ slotCoef) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const size_t nn = (size_t)n;
-  const int empty = based ? kSlotEmpty : 0;
+  int o[2 * kShortRow];
+  unsigned own;
+  slot_offsets(slotOff, slotTile, slotRow, nn, i, o, own);
   int f = ownStart[i], k = losortStart[i];
 #pragma unroll
   for (int q = 0; q < 2 * kShortRow; ++q) {
-    const uint32_t w = slotOff[(q >> 1) * nn + i];
-    const int d = (int)(int16_t)((q & 1) ? (w >> 16) : (w & 0xffffu));
     T v = T(0);
-    if (d != empty) v = q < kShortRow ? upper[f++] : lowerL[k++];
-    slotCoef[q * nn + i] = v;
+    if (o[q] != 0) v = q < kShortRow ? upper[f++] : lowerL[k++];
+    if (q < kShortRow || !sym) slotCoef[q * nn + i] = v;
   }
 }
 

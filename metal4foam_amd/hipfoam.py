@@ -266,6 +266,14 @@ class Context:
         self._check(self.lib.hipfoam_active_schedule(self.h, C.byref(v)), "active_schedule")
         return v.value
 
+    def slot_layout(self) -> tuple:
+        """(rows in patterned tiles, coefficient planes of a symmetric matrix)
+        of the current mesh's row-slot layout; (0, 0) without row slots."""
+        rows, planes = C.c_int64(), C.c_int32()
+        self._check(self.lib.hipfoam_slot_layout(self.h, C.byref(rows), C.byref(planes)),
+                    "slot_layout")
+        return rows.value, planes.value
+
     def set_schedule(self, **kw):
         s = self.get_schedule()
         for k, v in kw.items():

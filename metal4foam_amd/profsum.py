@@ -20,9 +20,11 @@ counter reads for k_copy<double>, a copy of known size (8 B read + 8 B
 written per element).  Counter values below 5 % of a kernel's largest
 dispatch are the stopped launches and are left out of its mean.
 
---layout slots prices the FP32 SpMV kernels on the row-slot layout (six
-FP32 coefficients and three packed int16 offset words per row instead of the
-row-start arrays and per-face arrays).  --traffic-json writes the calibrated
+--layout slots prices the SpMV kernels on the row-slot layout instead of the
+row-start arrays and per-face arrays: six coefficient planes (three for a
+symmetric matrix), one slot byte per row, three packed int16 offset words per
+row outside patterned tiles (--patterned-rows, as hipfoam_slot_layout
+reports) and one 32-byte descriptor per 256-row tile.  --traffic-json writes the calibrated
 HBM bytes per dispatch of one kernel (--traffic-kernel) as JSON.
 """
 
@@ -34,7 +36,19 @@ import csv
 import re
 
 
-def _bytes(name: str, n: int, nf: int, asym: bool, slots: bool = False) -> int | None:
+def slot_matrix_bytes(n: int, asym: bool, coef: int, patterned_rows: int) -> int:
+    """Matrix bytes one pass over the row-slot layout reads: the coefficient
+    planes (six, or three for a symmetric matrix whose neighbour-side
+    coefficients are gathered from their owner rows), one slot byte per row,
+    three packed int16 offset words per row of an explicit tile and one
+    32-byte descriptor per 256-row tile."""
+    planes = 6 if asym else 3
+    tiles = (n + 255) // 256
+    return planes * coef * n + n + 12 * (n - patterned_rows) + 32 * tiles
+
+
+def _bytes(name: str, n: int, nf: int, asym: bool, slots: bool = False,
+           patterned_rows: int = 0) -> int | None:
     """Algorithmic bytes per launch of the kernels on the solve path."""
     b = _ldu_bytes(name, n, nf, asym)
     if b is None or not slots or "k_pcg_update" in name or "k_bicg_update" in name:
@@ -45,9 +59,8 @@ def _bytes(name: str, n: int, nf: int, asym: bool, slots: bool = False) -> int |
         face, coef = (28 if asym else 24), 8
     else:
         return b
-    # row starts and per-face arrays out; six coefficient slots and three
-    # packed offset words per row in
-    return b - (8 * n + 8 + face * nf) + (6 * coef + 12) * n
+    # row starts and per-face arrays out, the row-slot matrix in
+    return b - (8 * n + 8 + face * nf) + slot_matrix_bytes(n, asym, coef, patterned_rows)
 
 
 def _ldu_bytes(name: str, n: int, nf: int, asym: bool) -> int | None:
@@ -97,6 +110,7 @@ def main():
     ap.add_argument("--command", action="append", default=[])
     ap.add_argument("--peak-gbs", type=float, default=8000.0)
     ap.add_argument("--layout", choices=["faces", "slots"], default="faces")
+    ap.add_argument("--patterned-rows", type=int, default=0)
     ap.add_argument("--traffic-json")
     ap.add_argument("--traffic-kernel", default="k_cheb_step<float, false, false>")
     ap.add_argument("--workload", default="")
@@ -134,8 +148,10 @@ def main():
     for c in a.command:
         print(f"    {c}")
     print()
+    lay = (f"row-slot layout, {a.patterned_rows:,} rows in patterned tiles" if slots
+           else "LDU face layout")
     print(f"Workload: {a.cells:,} cells, {a.faces:,} faces, "
-          f"{'asymmetric' if a.asym else 'symmetric'}. Durations: rocprofv3 kernel trace, "
+          f"{'asymmetric' if a.asym else 'symmetric'}, {lay}. Durations: rocprofv3 kernel trace, "
           f"launches of at least {a.noop_us} us (shorter launches returned at entry after the "
           "device-side stop and are listed apart).\n")
     if fscale:
@@ -149,7 +165,7 @@ def main():
         real = [x for x in v if x >= a.noop_us]
         stopped = len(v) - len(real)
         avg = mean(real) if real else mean(v)
-        b = _bytes(name, a.cells, a.faces, a.asym, slots)
+        b = _bytes(name, a.cells, a.faces, a.asym, slots, a.patterned_rows)
         gbs = b / (avg * 1e3) if (b and real) else None
         tr = None
         if fscale and name in fetch and name in write:
@@ -172,13 +188,15 @@ def main():
                 "kernel": a.traffic_kernel,
                 "workload": a.workload,
                 "layout": "row-slot" if slots else "LDU face",
+                "patterned_rows": a.patterned_rows if slots else None,
                 "fetch_kib_per_dispatch": round(f_kib, 1),
                 "write_kib_per_dispatch": round(w_kib, 1),
                 "calibration": {"kernel": "k_copy<double>", "bytes_read": 8 * a.cells,
                                 "bytes_written": 8 * a.cells, "fetch_scale": round(fscale, 4),
                                 "write_scale": round(wscale, 4)},
                 "bytes_per_launch": int(round((f_kib * fscale + w_kib * wscale) * 1024.0)),
-                "algorithmic_bytes_per_launch": _bytes(k, a.cells, a.faces, a.asym, slots),
+                "algorithmic_bytes_per_launch": _bytes(k, a.cells, a.faces, a.asym, slots,
+                                                       a.patterned_rows),
                 "commands": a.command,
             }
             with open(a.traffic_json, "w") as fh:

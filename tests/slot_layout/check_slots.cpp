@@ -1,0 +1,101 @@
+// Checks the row-slot layout builder on the host: every row's slots, decoded
+// with the same slot_decode the kernels use, must give the columns of its
+// owner faces (face order) and neighbour faces (losort order), and, for the
+// symmetric form, each neighbour slot's owner slot must hold the same face.
+//
+// Input (binary, int32): nMeshes, then per mesh n, nFaces, ownStart[n+1],
+// losortStart[n+1], upperAddr[nFaces], lowerAddr[nFaces], losort[nFaces].
+// Output: one line per mesh, "slots=<0|1> patterned=<rows> explicit=<rows>
+// errors=<count>".
+
+#include <cstdio>
+#include <vector>
+
+#include "hipfoam_slots.hpp"
+
+using namespace hipfoam;
+
+static bool read_ints(FILE* f, std::vector<int32_t>& v, size_t count) {
+  v.resize(count);
+  return count == 0 || fread(v.data(), sizeof(int32_t), count, f) == count;
+}
+
+int main(int argc, char** argv) {
+  if (argc != 2) {
+    fprintf(stderr, "usage: check_slots <addressing.bin>\n");
+    return 2;
+  }
+  FILE* f = fopen(argv[1], "rb");
+  if (!f) {
+    fprintf(stderr, "cannot open %s\n", argv[1]);
+    return 2;
+  }
+  int32_t nm = 0;
+  if (fread(&nm, sizeof(int32_t), 1, f) != 1) return 2;
+  constexpr int K = kShortRow, S = 2 * kShortRow;
+  for (int m = 0; m < nm; ++m) {
+    int32_t hdr[2];
+    if (fread(hdr, sizeof(int32_t), 2, f) != 2) return 2;
+    const int n = hdr[0], nf = hdr[1];
+    std::vector<int32_t> own, los, upp, low, lso;
+    if (!read_ints(f, own, n + 1) || !read_ints(f, los, n + 1) || !read_ints(f, upp, nf) ||
+        !read_ints(f, low, nf) || !read_ints(f, lso, nf))
+      return 2;
+    SlotLayout L;
+    const bool ok = build_slots(n, own.data(), upp.data(), los.data(), lso.data(), low.data(), L);
+    if (!ok) {
+      printf("slots=0 patterned=0 explicit=0 errors=0\n");
+      continue;
+    }
+    long errors = 0;
+    int64_t patterned = 0;
+    auto decode = [&](int i, int (&o)[S], unsigned& ow) {
+      const int32_t* d = &L.tile[(size_t)(i / kSlotTile) * kSlotDesc];
+      slot_decode(d, L.row[i], L.words.data(), (size_t)n, i, o, ow);
+    };
+    for (int i = 0; i < n; ++i) {
+      int o[S];
+      unsigned ow;
+      decode(i, o, ow);
+      if (L.tile[(size_t)(i / kSlotTile) * kSlotDesc + 6] & kSlotPatterned) ++patterned;
+      // owner slots: the row's owner faces in face order
+      int k = own[i];
+      for (int q = 0; q < K; ++q)
+        if (o[q] != 0 && (k >= own[i + 1] || i + o[q] != upp[k++])) ++errors;
+      if (k != own[i + 1]) ++errors;
+      // neighbour slots: the losort faces in order, each found again in the
+      // owner slot its packed index names
+      k = los[i];
+      for (int q = K; q < S; ++q) {
+        if (o[q] == 0) continue;
+        if (k >= los[i + 1]) {
+          ++errors;
+          continue;
+        }
+        const int face = lso[k++];
+        const int col = i + o[q];
+        if (col != low[face] || col < 0 || col >= n) {
+          ++errors;
+          continue;
+        }
+        const unsigned s = (ow >> (2 * (q - K))) & 3u;
+        int oc[S];
+        unsigned owc;
+        decode(col, oc, owc);
+        if (s >= (unsigned)K || oc[s] == 0 || col + oc[s] != i) {
+          ++errors;
+          continue;
+        }
+        // the face of owner slot s of row col: its rank among col's filled owner slots
+        int rank = 0;
+        for (int p = 0; p < (int)s; ++p) rank += oc[p] != 0;
+        if (own[col] + rank != face) ++errors;
+      }
+      if (k != los[i + 1]) ++errors;
+    }
+    printf("slots=1 patterned=%lld explicit=%lld errors=%ld\n", (long long)L.patternedRows,
+           (long long)(n - L.patternedRows), errors + (patterned != L.patternedRows));
+  }
+  fclose(f);
+  return 0;
+}

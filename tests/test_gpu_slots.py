@@ -61,6 +61,40 @@ def scattered(n, seed):
                         rng.standard_normal(n), (n,), "scattered")
 
 
+def blocks(seed, gen=mg.convection_diffusion):
+    """Three structured blocks on one block-diagonal mesh: tiles that straddle
+    two blocks see two offset patterns and keep per-row offsets (explicit
+    tiles); the others are patterned."""
+    parts = [gen(20, 20, 20, seed=seed), gen(23, 23, 23, seed=seed + 1),
+             gen(17, 19, 21, seed=seed + 2)]
+    n0, low, upp, d, up, lo, src = 0, [], [], [], [], [], []
+    for p in parts:
+        low.append(p.lower_addr + n0)
+        upp.append(p.upper_addr + n0)
+        d.append(p.diag)
+        up.append(p.upper)
+        lo.append(p.lower_or_upper())
+        src.append(p.source)
+        n0 += p.n
+    low = np.concatenate(low).astype(np.int32)
+    upp = np.concatenate(upp).astype(np.int32)
+    own, los, loss = mg.derived_addressing(n0, low, upp)
+    return mg.LduMatrix(n0, low, upp, own, los, loss, np.concatenate(d), np.concatenate(up),
+                        np.concatenate(lo), np.concatenate(src), (n0,), "blocks")
+
+
+def face_order_shuffled(seed):
+    """A structured block whose faces of one owner are not in neighbour
+    order: every tile keeps per-row offsets."""
+    m = mg.convection_diffusion(15, 13, 11, seed=seed)
+    rng = np.random.default_rng(seed)
+    order = np.lexsort((rng.random(m.nfaces), m.lower_addr))
+    low, upp = m.lower_addr[order].astype(np.int32), m.upper_addr[order].astype(np.int32)
+    own, los, loss = mg.derived_addressing(m.n, low, upp)
+    return mg.LduMatrix(m.n, low, upp, own, los, loss, m.diag, m.upper[order],
+                        m.lower[order], m.source, m.shape, "face_order")
+
+
 def spmv_all_schedules(ctx, m, x, lower):
     out = []
     for s in (0, 1, 2):
@@ -71,7 +105,8 @@ def spmv_all_schedules(ctx, m, x, lower):
     return out
 
 
-@pytest.mark.parametrize("case", ["poisson", "convdiff", "wide", "irregular", "scattered"])
+@pytest.mark.parametrize("case", ["poisson", "convdiff", "wide", "irregular", "scattered",
+                                  "blocks", "face_order"])
 def test_spmv_schedules_bitwise_equal(ctx, case):
     if case == "poisson":
         m = mg.poisson(33, 21, 17, seed=5)
@@ -81,10 +116,23 @@ def test_spmv_schedules_bitwise_equal(ctx, case):
         m = mg.convection_diffusion(200, 170, 3, seed=7)
     elif case == "irregular":
         m = irregular(3000, 2500, 8)
+    elif case == "blocks":
+        m = blocks(30)
+    elif case == "face_order":
+        m = face_order_shuffled(31)
     else:
         m = scattered(200000, 9)
     print(f"{case}: n={m.n} faces={m.nfaces}", flush=True)
     ctx.set_topology(m)
+    patterned, planes = ctx.slot_layout()
+    if case in ("poisson", "convdiff", "wide"):
+        assert (patterned, planes) == (m.n, 3)
+    elif case == "blocks":
+        assert 0 < m.n - patterned < m.n // 10
+    elif case == "face_order":
+        assert patterned == 0 and planes == 3
+    else:
+        assert (patterned, planes) == (0, 0)
     x = np.random.default_rng(9).standard_normal(m.n).astype(np.float32)
     for lower in (None, m.lower):
         ys = spmv_all_schedules(ctx, m, x, lower)
@@ -101,10 +149,14 @@ def test_spmv_schedules_bitwise_equal(ctx, case):
             assert np.all(np.abs(ys[2] - ref) <= 8 * 2.0 ** -24 * absrow + 1e-30)
 
 
-@pytest.mark.parametrize("solver", [H.PCG, H.PBICGSTAB])
-def test_refined_solve_identical_across_schedules(ctx, solver):
-    m = mg.poisson(28, 24, 20, seed=11) if solver == H.PCG else \
-        mg.convection_diffusion(26, 22, 18, seed=12)
+@pytest.mark.parametrize("solver,mesh", [(H.PCG, "box"), (H.PBICGSTAB, "box"),
+                                         (H.PCG, "blocks"), (H.PBICGSTAB, "blocks")])
+def test_refined_solve_identical_across_schedules(ctx, solver, mesh):
+    if mesh == "blocks":
+        m = blocks(13, mg.poisson) if solver == H.PCG else blocks(14)
+    else:
+        m = mg.poisson(28, 24, 20, seed=11) if solver == H.PCG else \
+            mg.convection_diffusion(26, 22, 18, seed=12)
     lower = None if solver == H.PCG else m.lower
     ctx.set_topology(m)
     results = []

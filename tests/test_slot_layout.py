@@ -1,0 +1,154 @@
+"""Row-slot layout builder (host code of the HIP library), checked on the CPU.
+
+tests/slot_layout/check_slots.cpp runs the library's own layout builder and
+its per-row decode (the function the kernels call) over each mesh and
+counts rows whose decoded slots differ from the face loops: owner faces in
+face order, neighbour faces in losort order, and each neighbour slot's packed
+owner slot pointing at the same face.  Structured blocks must come out fully
+patterned (1 byte of addressing per row); meshes with rows of more than three
+faces a side must be refused (the library then runs the face loops)."""
+
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from metal4foam_amd import meshgen as mg
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(os.path.dirname(HERE), "metal4foam_amd", "csrc")
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    cc = hipcc()
+    if cc is None:
+        pytest.skip("hipcc not found")
+    out = str(tmp_path_factory.mktemp("slots") / "check_slots")
+    subprocess.run([cc, "--offload-host-only", "-O2", "-std=c++17", "-I", CSRC,
+                    os.path.join(HERE, "slot_layout", "check_slots.cpp"), "-o", out],
+                   check=True)
+    return out
+
+
+def from_pairs(n, pairs):
+    pairs = sorted({(min(a, b), max(a, b)) for a, b in pairs if a != b})
+    low = np.array([p[0] for p in pairs], dtype=np.int32)
+    upp = np.array([p[1] for p in pairs], dtype=np.int32)
+    own, los, loss = mg.derived_addressing(n, low, upp)
+    return n, low, upp, own, los, loss
+
+
+def block(nx, ny, nz):
+    return mg.structured_addressing(nx, ny, nz)
+
+
+def concat(blocks):
+    n0, lows, upps = 0, [], []
+    for n, low, upp, *_ in blocks:
+        lows.append(low + n0)
+        upps.append(upp + n0)
+        n0 += n
+    low = np.concatenate(lows).astype(np.int32)
+    upp = np.concatenate(upps).astype(np.int32)
+    own, los, loss = mg.derived_addressing(n0, low, upp)
+    return n0, low, upp, own, los, loss
+
+
+def renumbered(nx, ny, nz, seed):
+    n, low, upp, *_ = block(nx, ny, nz)
+    perm = np.random.default_rng(seed).permutation(n)
+    return from_pairs(n, zip(perm[low].tolist(), perm[upp].tolist()))
+
+
+def irregular(n, seed):
+    rng = np.random.default_rng(seed)
+    pairs = [(i, i + 1) for i in range(n - 1)]
+    pairs += [tuple(rng.choice(n, 2, replace=False)) for _ in range(n)]
+    return from_pairs(n, pairs)
+
+
+def scattered(n, seed):
+    perm = np.random.default_rng(seed).permutation(n)
+    pairs = [(i, i + 1) for i in range(n - 1)]
+    pairs += [(int(perm[k]), int(perm[k + 1])) for k in range(0, n - 1, 2)]
+    return from_pairs(n, pairs)
+
+
+def face_order_shuffled(nx, ny, nz, seed):
+    """A structured block whose faces of one owner are not in neighbour order."""
+    n, low, upp, *_ = block(nx, ny, nz)
+    rng = np.random.default_rng(seed)
+    order = np.lexsort((rng.random(low.shape[0]), low))
+    low, upp = low[order].astype(np.int32), upp[order].astype(np.int32)
+    own, los, loss = mg.derived_addressing(n, low, upp)
+    return n, low, upp, own, los, loss
+
+
+CASES = {
+    "box": (lambda: block(33, 21, 17), "all"),
+    "plane": (lambda: block(100, 100, 1), "all"),
+    "wide": (lambda: block(200, 170, 3), "all"),
+    "thin": (lambda: block(7, 5, 300), "all"),
+    "line": (lambda: block(1000, 1, 1), "all"),
+    "blocks": (lambda: concat([block(20, 20, 20), block(23, 23, 23), block(17, 19, 21)]),
+               "mixed"),
+    "face_order": (lambda: face_order_shuffled(9, 8, 7, 3), "any"),
+    "renumbered": (lambda: renumbered(12, 11, 10, 4), "any"),
+    "scattered": (lambda: scattered(3000, 5), "any"),
+    "irregular": (lambda: irregular(2000, 6), "refused"),
+    "one_cell": (lambda: block(1, 1, 1), "all"),
+}
+
+
+def run_checker(checker, tmp_path, meshes):
+    path = tmp_path / "addr.bin"
+    with open(path, "wb") as f:
+        np.array([len(meshes)], dtype=np.int32).tofile(f)
+        for n, low, upp, own, los, loss in meshes:
+            np.array([n, low.shape[0]], dtype=np.int32).tofile(f)
+            for a in (own, loss, upp, low, los):
+                np.ascontiguousarray(a, dtype=np.int32).tofile(f)
+    out = subprocess.run([checker, str(path)], check=True, capture_output=True, text=True)
+    rows = []
+    for line in out.stdout.strip().splitlines():
+        rows.append({k: int(v) for k, v in (kv.split("=") for kv in line.split())})
+    return rows
+
+
+def test_slot_layout_matches_face_loops(checker, tmp_path):
+    names = list(CASES)
+    meshes = [CASES[k][0]() for k in names]
+    results = run_checker(checker, tmp_path, meshes)
+    assert len(results) == len(names)
+    for name, mesh, res in zip(names, meshes, results):
+        expect = CASES[name][1]
+        n = mesh[0]
+        print(name, n, res)
+        assert res["errors"] == 0, name
+        if expect == "refused":
+            assert res["slots"] == 0, name
+            continue
+        if expect != "any":
+            assert res["slots"] == 1, name
+        if expect == "all":
+            assert res["patterned"] == n, name
+        if expect == "mixed":
+            assert 0 < res["explicit"] < n // 10, name
+            assert res["patterned"] + res["explicit"] == n
+
+
+def test_slot_layout_empty_mesh(checker, tmp_path):
+    z = np.zeros(0, dtype=np.int32)
+    res = run_checker(checker, tmp_path, [(0, z, z, np.zeros(1, np.int32), z,
+                                           np.zeros(1, np.int32))])
+    assert res == [{"slots": 1, "patterned": 0, "explicit": 0, "errors": 0}]
