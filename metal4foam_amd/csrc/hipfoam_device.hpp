@@ -270,21 +270,50 @@ __device__ __forceinline__ void slot_offsets(const uint32_t* __restrict__ slotOf
 template <typename T>
 struct RowSlots {
   T c[2 * kShortRow];
-  int o[2 * kShortRow];
+  int col[2 * kShortRow];  // a valid row index for every slot (the row itself if empty)
+  unsigned m;              // bit q: slot q holds a face
   __device__ __forceinline__ void load(const LduOp<T>& A, int i) {
     const size_t n = (size_t)A.n;
     // the row's own coefficients first: independent coalesced loads
 #pragma unroll
     for (int q = 0; q < 2 * kShortRow; ++q)
       if (q < kShortRow || !A.slotSym) c[q] = A.slotCoef[q * n + i];
+    const unsigned rb = A.slotRow[i];
+    const int t = i / kSlotTile;
+    const int t0 = __builtin_amdgcn_readfirstlane(t);
+    if (__all(t == t0))
+      decode(A, A.slotTile + (size_t)t0 * kSlotDesc, rb, i);
+    else
+      decode(A, A.slotTile + (size_t)t * kSlotDesc, rb, i);
+  }
+  __device__ __forceinline__ void decode(const LduOp<T>& A, const int32_t* d, unsigned rb,
+                                         int i) {
+    const size_t n = (size_t)A.n;
+    const int flags = d[6];
     unsigned own;
-    slot_offsets(A.slotOff, A.slotTile, A.slotRow, n, i, o, own);
+    if (flags & kSlotPatterned) {
+      // columns from the tile bases alone: the gathers need not wait for the
+      // slot byte, which only selects the terms
+#pragma unroll
+      for (int q = 0; q < 2 * kShortRow; ++q) col[q] = min(max(i + d[q], 0), A.n - 1);
+      m = rb;
+      own = (unsigned)flags >> 8;
+    } else {
+      int o[2 * kShortRow];
+      slot_decode(d, rb, A.slotOff, n, i, o, own);
+      m = 0;
+#pragma unroll
+      for (int q = 0; q < 2 * kShortRow; ++q) {
+        col[q] = i + o[q];
+        m |= (o[q] != 0 ? 1u : 0u) << q;
+      }
+    }
     if (A.slotSym) {
       // a neighbour face's coefficient is stored once, in its owner's row
 #pragma unroll
       for (int q = kShortRow; q < 2 * kShortRow; ++q) {
-        const unsigned s = (own >> (2 * (q - kShortRow))) & 3u;
-        c[q] = o[q] != 0 ? A.slotCoef[s * n + (size_t)(i + o[q])] : T(0);
+        const unsigned sq = (own >> (2 * (q - kShortRow))) & 3u;
+        c[q] = A.slotCoef[sq * n + (size_t)col[q]];
       }
     }
   }
@@ -295,11 +324,10 @@ struct RowSlots {
     T s = diag[i] * xi;
     T xq[2 * kShortRow];
 #pragma unroll
-    for (int q = 0; q < 2 * kShortRow; ++q)
-      if (o[q] != 0) xq[q] = xf(i + o[q]);
+    for (int q = 0; q < 2 * kShortRow; ++q) xq[q] = xf(col[q]);
 #pragma unroll
     for (int q = 0; q < 2 * kShortRow; ++q)
-      if (o[q] != 0) s += c[q] * xq[q];
+      if ((m >> q) & 1u) s += c[q] * xq[q];
     return s;
   }
 };
