@@ -151,9 +151,9 @@ def main():
     lay = (f"row-slot layout, {a.patterned_rows:,} rows in patterned tiles" if slots
            else "LDU face layout")
     print(f"Workload: {a.cells:,} cells, {a.faces:,} faces, "
-          f"{'asymmetric' if a.asym else 'symmetric'}, {lay}. Durations: rocprofv3 kernel trace, "
-          f"launches of at least {a.noop_us} us (shorter launches returned at entry after the "
-          "device-side stop and are listed apart).\n")
+          f"{'asymmetric' if a.asym else 'symmetric'}, {lay}. Durations: rocprofv3 kernel trace; "
+          f"for row kernels, launches of at least {a.noop_us} us (shorter launches returned at "
+          "entry after the device-side stop and are listed apart).\n")
     if fscale:
         print(f"HBM counter calibration on k_copy<double> ({8 * a.cells:,} B read, "
               f"{8 * a.cells:,} B written): FETCH_SIZE x {fscale:.3f}, WRITE_SIZE x {wscale:.3f}.\n")
@@ -162,10 +162,11 @@ def main():
     print("|---|---:|---:|---:|---:|---:|---:|---:|")
     rows = sorted(durs.items(), key=lambda kv: -sum(kv[1]))[: a.top]
     for name, v in rows:
-        real = [x for x in v if x >= a.noop_us]
+        b = _bytes(name, a.cells, a.faces, a.asym, slots, a.patterned_rows)
+        # the cut applies to row kernels; one-workgroup kernels are short anyway
+        real = [x for x in v if x >= a.noop_us] if b is not None else list(v)
         stopped = len(v) - len(real)
         avg = mean(real) if real else mean(v)
-        b = _bytes(name, a.cells, a.faces, a.asym, slots, a.patterned_rows)
         gbs = b / (avg * 1e3) if (b and real) else None
         tr = None
         if fscale and name in fetch and name in write:

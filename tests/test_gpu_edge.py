@@ -104,6 +104,8 @@ def test_256_cubed_poisson(ctx):
         ys.append(ctx.amul(x, H.FP32))
     ctx.set_schedule(**keep)
     assert ctx.active_schedule() == 2
+    # z offset 65,536: every tile still patterned, on bases beyond int16
+    assert ctx.slot_layout() == (m.n, 3)
     np.testing.assert_array_equal(ys[1], ys[0])
     psi = np.zeros(m.n)
     perf = ctx.solve(H.PCG, m.diag, m.upper, None, psi, m.source,
