@@ -660,11 +660,11 @@ int stage_matrix(hipfoam_context* c, const double* diag, const double* upper,
     CK(c->slotCoef64.ensure((size_t)2 * kShortRow * n));
     const int sym = lower == nullptr;
     hipLaunchKernelGGL((k_build_slot_coeffs<float>), dim3(blocks_for(n, 256)), dim3(256), 0,
-                       c->stream, n, c->ownStart.p, c->losortStart.p, c->slotOff.p,
+                       c->stream, n, nullptr, c->ownStart.p, c->losortStart.p, c->slotOff.p,
                        c->slotTile.p, c->slotRow.p, sym, c->upper32.p, c->lowerL32.p,
                        c->slotCoef32.p);
     hipLaunchKernelGGL((k_build_slot_coeffs<double>), dim3(blocks_for(n, 256)), dim3(256), 0,
-                       c->stream, n, c->ownStart.p, c->losortStart.p, c->slotOff.p,
+                       c->stream, n, nullptr, c->ownStart.p, c->losortStart.p, c->slotOff.p,
                        c->slotTile.p, c->slotRow.p, sym, c->upper64.p, c->lowerL64.p,
                        c->slotCoef64.p);
     CKL();

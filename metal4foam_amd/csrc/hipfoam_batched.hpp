@@ -35,13 +35,14 @@ struct BatchMap {
 };
 
 struct BlockSeg {
-  int r0, r1, rhs, seg;
+  int t, r0, r1, rhs, seg;
   long voff;  // rhs * N: offset of this RHS in the vectors
 };
 
 __device__ __forceinline__ BlockSeg block_seg(const BatchMap& B, int lb) {
   BlockSeg s;
   const int t = lb / B.nrhs;
+  s.t = t;
   s.rhs = lb - t * B.nrhs;
   s.r0 = B.tileRow0[t];
   s.r1 = B.tileRow0[t + 1];
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(256) void kb_cheb_step(BatchMap B, LduOp<float> A,
   const int i = u.r0 + threadIdx.x;
   const bool in = i < u.r1;
   RowSlots<float> rs;
-  if (in && A.slots) rs.load(A, i);
+  if (in && A.slots) rs.load(A, i, u.t);
 #pragma unroll 1
   for (int q = 0; q < u.nr; ++q) {
     if (!((run >> q) & 1)) continue;
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(256) void kb_pcg_p_spmv(BatchMap B, LduOp<float> A,
   const int i = u.r0 + threadIdx.x;
   const bool in = i < u.r1;
   RowSlots<float> rs;
-  if (in && A.slots) rs.load(A, i);
+  if (in && A.slots) rs.load(A, i, u.t);
 #pragma unroll 1
   for (int q = 0; q < u.nr; ++q) {
     if (!((run >> q) & 1)) continue;
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(256) void kb_bicg_py_spmv(BatchMap B, LduOp<float> 
   const int i = u.r0 + threadIdx.x;
   const bool in = i < u.r1;
   RowSlots<float> rs;
-  if (in && A.slots) rs.load(A, i);
+  if (in && A.slots) rs.load(A, i, u.t);
 #pragma unroll 1
   for (int q = 0; q < u.nr; ++q) {
     if (!((run >> q) & 1)) continue;
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(256) void kb_bicg_sz_spmv(BatchMap B, LduOp<float> 
   const int i = u.r0 + threadIdx.x;
   const bool in = i < u.r1;
   RowSlots<float> rs;
-  if (in && A.slots) rs.load(A, i);
+  if (in && A.slots) rs.load(A, i, u.t);
 #pragma unroll 1
   for (int q = 0; q < u.nr; ++q) {
     if (!((run >> q) & 1)) continue;
@@ -551,7 +552,7 @@ __global__ __launch_bounds__(256) void kb_rowsum(BatchMap B, LduOp<double> A,
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   const int i = g.r0 + threadIdx.x;
   if (i < g.r1 && active[g.seg]) {
-    const double ui = ldu_row(A, i, 1.0, [](int) { return 1.0; });
+    const double ui = ldu_row_t(A, i, g.t, 1.0, [](int) { return 1.0; });
     if (g.rhs == 0) u[i] = ui;
     v[0] = psi[g.voff + i];
     v[1] = (fabs(ui) > 1e-8 * fabs(A.diag[i])) ? 1.0 : 0.0;
@@ -576,7 +577,7 @@ __global__ __launch_bounds__(256) void kb_initial_residual(BatchMap B, LduOp<dou
   if (i < g.r1 && active[g.seg]) {
     const double xbar = osBase[g.seg].xbar;
     const double* ps = psi + g.voff;
-    const double wA = ldu_row(A, i, ps[i], [&](int j) { return ps[j]; });
+    const double wA = ldu_row_t(A, i, g.t, ps[i], [&](int j) { return ps[j]; });
     const double ri = b[g.voff + i] - wA;
     r[g.voff + i] = ri;
     const double t = u[i] * xbar;
@@ -678,7 +679,7 @@ __global__ __launch_bounds__(256) void kb_refine_residual(
       const float* dl = delta + g.voff;
       auto pf = [&](int j) { return po[j] + ((double)dl[j] - dm); };
       const double pi = pf(i);
-      const double wA = ldu_row(A, i, pi, pf);
+      const double wA = ldu_row_t(A, i, g.t, pi, pf);
       psiNew[k] = pi;
       const double ri = b[k] - wA;
       r[k] = ri;

@@ -252,34 +252,19 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
 // added in exactly the order of the loop form, so both give identical bits.
 // One row of the row-slot layout held in registers: loaded once, applied to
 // any number of vectors (the right-hand sides of a batch share the matrix).
-// slot_decode with the descriptor read by scalar loads when the wavefront's
-// rows share one tile.
-__device__ __forceinline__ void slot_offsets(const uint32_t* __restrict__ slotOff,
-                                             const int32_t* __restrict__ slotTile,
-                                             const uint8_t* __restrict__ slotRow, size_t n,
-                                             int i, int (&o)[2 * kShortRow], unsigned& own) {
-  const unsigned rb = slotRow[i];
-  const int t = i / kSlotTile;
-  const int t0 = __builtin_amdgcn_readfirstlane(t);
-  if (__all(t == t0))
-    slot_decode(slotTile + (size_t)t0 * kSlotDesc, rb, slotOff, n, i, o, own);
-  else
-    slot_decode(slotTile + (size_t)t * kSlotDesc, rb, slotOff, n, i, o, own);
-}
-
 template <typename T>
 struct RowSlots {
   T c[2 * kShortRow];
   int col[2 * kShortRow];  // a valid row index for every slot (the row itself if empty)
   unsigned m;              // bit q: slot q holds a face
-  __device__ __forceinline__ void load(const LduOp<T>& A, int i) {
+  // row i of slot tile t (i / kSlotTile, or the batch's work tile)
+  __device__ __forceinline__ void load(const LduOp<T>& A, int i, int t) {
     const size_t n = (size_t)A.n;
     // the row's own coefficients first: independent coalesced loads
 #pragma unroll
     for (int q = 0; q < 2 * kShortRow; ++q)
       if (q < kShortRow || !A.slotSym) c[q] = A.slotCoef[q * n + i];
     const unsigned rb = A.slotRow[i];
-    const int t = i / kSlotTile;
     const int t0 = __builtin_amdgcn_readfirstlane(t);
     if (__all(t == t0))
       decode(A, A.slotTile + (size_t)t0 * kSlotDesc, rb, i);
@@ -332,13 +317,14 @@ struct RowSlots {
   }
 };
 
+// Row i of A x; t is row i's slot tile (ldu_row below: i / kSlotTile).
 template <typename T, typename XF>
-__device__ __forceinline__ T ldu_row(const LduOp<T>& A, int i, T xi, XF xf) {
+__device__ __forceinline__ T ldu_row_t(const LduOp<T>& A, int i, int t, T xi, XF xf) {
   if (A.slots) {
     // coefficients and offsets are independent coalesced loads (no row-start
     // indirection)
     RowSlots<T> rs;
-    rs.load(A, i);
+    rs.load(A, i, t);
     return rs.apply(A.diag, i, xi, xf);
   }
   const int f0 = A.ownStart[i], f1 = A.ownStart[i + 1];
@@ -382,6 +368,11 @@ __device__ __forceinline__ T ldu_row(const LduOp<T>& A, int i, T xi, XF xf) {
     }
   }
   return s;
+}
+
+template <typename T, typename XF>
+__device__ __forceinline__ T ldu_row(const LduOp<T>& A, int i, T xi, XF xf) {
+  return ldu_row_t(A, i, i / kSlotTile, xi, xf);
 }
 
 // ---------------------------------------------------------------------------
@@ -927,18 +918,28 @@ __global__ __launch_bounds__(1024) void k_gather_addr(int nf, const int32_t* __r
 // Row-slot coefficients (layout of LduOp::slotCoef) from the face-ordered
 // upper and losort-ordered lower coefficients: a side's faces fill, in order,
 // the row's non-empty slots; empty slots hold 0.  sym: the owner slots only.
+// tileStart: the tiles' first rows (tile b = workgroup b, at most kSlotTile
+// rows each), or nullptr for kSlotTile-row tiles.
 template <typename T>
 __global__ __launch_bounds__(256) void k_build_slot_coeffs(
-    int n, const int32_t* __restrict__ ownStart, const int32_t* __restrict__ losortStart,
-    const uint32_t* __restrict__ slotOff, const int32_t* __restrict__ slotTile,
-    const uint8_t* __restrict__ slotRow, int sym, const T* __restrict__ upper,
-    const T* __restrict__ lowerL, T* __restrict__ slotCoef) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+    int n, const int32_t* __restrict__ tileStart, const int32_t* __restrict__ ownStart,
+    const int32_t* __restrict__ losortStart, const uint32_t* __restrict__ slotOff,
+    const int32_t* __restrict__ slotTile, const uint8_t* __restrict__ slotRow, int sym,
+    const T* __restrict__ upper, const T* __restrict__ lowerL, T* __restrict__ slotCoef) {
+  int i, t;
+  if (tileStart) {
+    t = blockIdx.x;
+    i = tileStart[t] + threadIdx.x;
+    if (i >= tileStart[t + 1]) return;
+  } else {
+    i = blockIdx.x * blockDim.x + threadIdx.x;
+    t = i / kSlotTile;
+    if (i >= n) return;
+  }
   const size_t nn = (size_t)n;
   int o[2 * kShortRow];
   unsigned own;
-  slot_offsets(slotOff, slotTile, slotRow, nn, i, o, own);
+  slot_decode(slotTile + (size_t)t * kSlotDesc, slotRow[i], slotOff, nn, i, o, own);
   int f = ownStart[i], k = losortStart[i];
 #pragma unroll
   for (int q = 0; q < 2 * kShortRow; ++q) {

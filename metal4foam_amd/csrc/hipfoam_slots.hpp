@@ -74,12 +74,25 @@ struct SlotLayout {
 // whose deltas are all zero, and whose neighbour faces of each slot all sit
 // in the same owner slot of their owner rows, is patterned.  False when a row
 // has more than kShortRow faces on a side or no candidate fits a tile.
+// Tiles are kSlotTile-row runs from row 0, or, with tileStart, the ntiles
+// runs [tileStart[t], tileStart[t + 1]) of at most kSlotTile rows (a batch's
+// work tiles, which never straddle two systems).
 inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
                  const int32_t* losortStart, const int32_t* losort, const int32_t* lowAddr,
-                 SlotLayout& L) {
+                 SlotLayout& L, const int32_t* tileStart = nullptr, int ntiles = 0) {
   constexpr int K = kShortRow, S = 2 * kShortRow;
   constexpr int64_t kHalf = 32767;
-  const int nt = (n + kSlotTile - 1) / kSlotTile;
+  const int nt = tileStart ? ntiles : (n + kSlotTile - 1) / kSlotTile;
+  // rows [i0, i1) of tile t
+  auto tile_rows = [&](int t, int& i0, int& i1) {
+    if (tileStart) {
+      i0 = tileStart[t];
+      i1 = tileStart[t + 1];
+    } else {
+      i0 = t * kSlotTile;
+      i1 = std::min(n, i0 + kSlotTile);
+    }
+  };
   L.tile.assign((size_t)nt * kSlotDesc, 0);
   L.words.assign((size_t)K * n, 0);
   L.row.assign((size_t)n, 0);
@@ -139,7 +152,8 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
   };
   std::vector<uint8_t> zeroTile((size_t)nt, 1);
   for (int t = 0; t < nt; ++t) {
-    const int i0 = t * kSlotTile, i1 = std::min(n, i0 + kSlotTile);
+    int i0, i1;
+    tile_rows(t, i0, i1);
     for (int side = 0; side < 2; ++side) {
       for (int q = 0; q < K; ++q) most[q].clear();
       all.clear();
@@ -239,7 +253,8 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
     packed[i] = (uint8_t)pk;
   }
   for (int t = 0; t < nt; ++t) {
-    const int i0 = t * kSlotTile, i1 = std::min(n, i0 + kSlotTile);
+    int i0, i1;
+    tile_rows(t, i0, i1);
     // patterned: zero deltas, and one owner slot per neighbour slot
     bool pat = zeroTile[t];
     int ownQ[K] = {-1, -1, -1};

@@ -3,8 +3,10 @@
 // owner faces (face order) and neighbour faces (losort order), and, for the
 // symmetric form, each neighbour slot's owner slot must hold the same face.
 //
-// Input (binary, int32): nMeshes, then per mesh n, nFaces, ownStart[n+1],
-// losortStart[n+1], upperAddr[nFaces], lowerAddr[nFaces], losort[nFaces].
+// Input (binary, int32): nMeshes, then per mesh n, nFaces, nTiles,
+// tileStart[nTiles+1] when nTiles > 0 (else 256-row tiles from row 0),
+// ownStart[n+1], losortStart[n+1], upperAddr[nFaces], lowerAddr[nFaces],
+// losort[nFaces].
 // Output: one line per mesh, "slots=<0|1> patterned=<rows> explicit=<rows>
 // errors=<count>".
 
@@ -34,15 +36,22 @@ int main(int argc, char** argv) {
   if (fread(&nm, sizeof(int32_t), 1, f) != 1) return 2;
   constexpr int K = kShortRow, S = 2 * kShortRow;
   for (int m = 0; m < nm; ++m) {
-    int32_t hdr[2];
-    if (fread(hdr, sizeof(int32_t), 2, f) != 2) return 2;
-    const int n = hdr[0], nf = hdr[1];
-    std::vector<int32_t> own, los, upp, low, lso;
-    if (!read_ints(f, own, n + 1) || !read_ints(f, los, n + 1) || !read_ints(f, upp, nf) ||
-        !read_ints(f, low, nf) || !read_ints(f, lso, nf))
+    int32_t hdr[3];
+    if (fread(hdr, sizeof(int32_t), 3, f) != 3) return 2;
+    const int n = hdr[0], nf = hdr[1], nt = hdr[2];
+    std::vector<int32_t> ts, own, los, upp, low, lso;
+    if ((nt > 0 && !read_ints(f, ts, nt + 1)) || !read_ints(f, own, n + 1) ||
+        !read_ints(f, los, n + 1) || !read_ints(f, upp, nf) || !read_ints(f, low, nf) ||
+        !read_ints(f, lso, nf))
       return 2;
+    // slot tile of every row
+    std::vector<int> tileOf(n);
+    for (int i = 0; i < n; ++i) tileOf[i] = i / kSlotTile;
+    for (int t = 0; t < nt; ++t)
+      for (int i = ts[t]; i < ts[t + 1]; ++i) tileOf[i] = t;
     SlotLayout L;
-    const bool ok = build_slots(n, own.data(), upp.data(), los.data(), lso.data(), low.data(), L);
+    const bool ok = build_slots(n, own.data(), upp.data(), los.data(), lso.data(), low.data(), L,
+                                nt > 0 ? ts.data() : nullptr, nt);
     if (!ok) {
       printf("slots=0 patterned=0 explicit=0 errors=0\n");
       continue;
@@ -50,14 +59,14 @@ int main(int argc, char** argv) {
     long errors = 0;
     int64_t patterned = 0;
     auto decode = [&](int i, int (&o)[S], unsigned& ow) {
-      const int32_t* d = &L.tile[(size_t)(i / kSlotTile) * kSlotDesc];
+      const int32_t* d = &L.tile[(size_t)tileOf[i] * kSlotDesc];
       slot_decode(d, L.row[i], L.words.data(), (size_t)n, i, o, ow);
     };
     for (int i = 0; i < n; ++i) {
       int o[S];
       unsigned ow;
       decode(i, o, ow);
-      if (L.tile[(size_t)(i / kSlotTile) * kSlotDesc + 6] & kSlotPatterned) ++patterned;
+      if (L.tile[(size_t)tileOf[i] * kSlotDesc + 6] & kSlotPatterned) ++patterned;
       // owner slots: the row's owner faces in face order
       int k = own[i];
       for (int q = 0; q < K; ++q)

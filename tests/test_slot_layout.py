@@ -64,6 +64,14 @@ def concat(blocks):
     return n0, low, upp, own, los, loss
 
 
+def work_tiles(mesh, sizes):
+    starts, r0 = [], 0
+    for s in sizes:
+        starts += list(range(r0, r0 + s, 256))
+        r0 += s
+    return (*mesh, starts + [r0])
+
+
 def renumbered(nx, ny, nz, seed):
     n, low, upp, *_ = block(nx, ny, nz)
     perm = np.random.default_rng(seed).permutation(n)
@@ -102,6 +110,11 @@ CASES = {
     "line": (lambda: block(1000, 1, 1), "all"),
     "blocks": (lambda: concat([block(20, 20, 20), block(23, 23, 23), block(17, 19, 21)]),
                "mixed"),
+    # the same mesh on a batch's work tiles (256-row runs restarting at every
+    # block): no tile straddles two blocks, so every tile is patterned
+    "blocks_work_tiles": (lambda: work_tiles(
+        concat([block(20, 20, 20), block(23, 23, 23), block(17, 19, 21)]),
+        [8000, 12167, 6783]), "all"),
     "face_order": (lambda: face_order_shuffled(9, 8, 7, 3), "any"),
     "renumbered": (lambda: renumbered(12, 11, 10, 4), "any"),
     "scattered": (lambda: scattered(3000, 5), "any"),
@@ -111,11 +124,17 @@ CASES = {
 
 
 def run_checker(checker, tmp_path, meshes):
+    """meshes: addressing tuples, each optionally followed by its tile starts."""
     path = tmp_path / "addr.bin"
     with open(path, "wb") as f:
         np.array([len(meshes)], dtype=np.int32).tofile(f)
-        for n, low, upp, own, los, loss in meshes:
-            np.array([n, low.shape[0]], dtype=np.int32).tofile(f)
+        for mesh in meshes:
+            n, low, upp, own, los, loss = mesh[:6]
+            tiles = mesh[6] if len(mesh) > 6 else None
+            nt = 0 if tiles is None else len(tiles) - 1
+            np.array([n, low.shape[0], nt], dtype=np.int32).tofile(f)
+            if tiles is not None:
+                np.asarray(tiles, dtype=np.int32).tofile(f)
             for a in (own, loss, upp, low, los):
                 np.ascontiguousarray(a, dtype=np.int32).tofile(f)
     out = subprocess.run([checker, str(path)], check=True, capture_output=True, text=True)
