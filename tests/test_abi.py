@@ -71,6 +71,11 @@ def test_schedule_validation_without_device():
     ok = _lib.Schedule(1, 512, 0, 16, 1024)
     assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(ok)) == _lib.OK
     assert ctx.get_schedule().as_dict() == ok.as_dict()
+    # no mesh: no row-slot layout
+    assert ctx.slot_layout() == (0, 0)
+    rows = C.c_int64()
+    assert ctx.lib.hipfoam_slot_layout(ctx.h, C.byref(rows), None) == _lib.ERR_ARG
+    assert ctx.lib.hipfoam_slot_layout(None, C.byref(rows), C.byref(C.c_int32())) == _lib.ERR_ARG
     ctx.close()
 
 
