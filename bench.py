@@ -92,38 +92,50 @@ def to_slot_bytes(ldu_bytes, n, nf, symmetric, patterned_rows):
 
 
 def cpu_baseline(m, solver, sample_iters):
-    """Oracle PCG(DIC) / PBiCGStab(DILU) in FP64 on one host core, timed on a
-    bounded number of iterations and scaled to the iteration count of the full
-    solve (committed fixture written by the oracle)."""
+    """Oracle PCG(DIC) / PBiCGStab(DILU) in FP64 on one host core.
+
+    sample_iters == 0: the full solve to 1e-6, timed, and its solution
+    returned so the GPU results can state their error against it.  Otherwise
+    a bounded number of iterations is timed and scaled to the iteration count
+    of the full solve (committed fixture written by the oracle); no solution.
+    Returns (baseline dict, x or None)."""
     import oracle
 
     fpath = os.path.join(ROOT, "tests", "golden", "cpu_baseline_iterations.json")
     full = json.load(open(fpath)) if os.path.exists(fpath) else {}
     key = m.name
+    max_iter = sample_iters if sample_iters > 0 else 20000
     t0 = time.perf_counter()
     if solver == "pcg":
-        _, perf = oracle.pcg(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=sample_iters)
+        x, perf = oracle.pcg(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=max_iter)
     else:
-        _, perf = oracle.pbicgstab(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=sample_iters)
+        x, perf = oracle.pbicgstab(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=max_iter)
     dt = time.perf_counter() - t0
-    iters_full = full.get(key, {}).get("iterations")
+    name = "PCG(DIC)" if solver == "pcg" else "PBiCGStab(DILU)"
     done = perf.nIterations
-    if perf.converged or iters_full is None:
-        est = dt
-        iters_full = done
+    if perf.converged:
+        est, sample = dt, (f"{name} FP64, 1 core, the full solve to 1e-6 timed "
+                           f"({done} iterations, final residual {perf.finalResidual:.3e})")
     else:
-        est = dt / done * iters_full
+        iters_full = full.get(key, {}).get("iterations") or done
+        est = dt / max(done, 1) * iters_full
+        sample = (f"{name} FP64, 1 core, {done} iterations timed ({dt:.2f} s) and scaled to "
+                  f"the {iters_full} iterations the full solve to 1e-6 takes")
+        x = None
     return {
         "value": round(1e3 * est, 3),
         "unit": "ms",
         "cores": 1,
         "kind": "port",
-        "sample": (f"{'PCG(DIC)' if solver == 'pcg' else 'PBiCGStab(DILU)'} FP64, 1 core, "
-                   f"{done} iterations timed ({dt:.2f} s) and scaled to the {iters_full} "
-                   f"iterations the full solve to 1e-6 takes"),
+        "sample": sample,
         "cpu": platform.processor() or platform.machine(),
         "nproc": os.cpu_count(),
-    }
+    }, x
+
+
+def rel_diff(a, b):
+    """max |a - b| / max |b|"""
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
 def main():
@@ -135,7 +147,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--host-steps", type=int, default=2,
                     help="solves timed with host arrays (PCIe copies included)")
-    ap.add_argument("--cpu-iters", type=int, default=200)
+    ap.add_argument("--cpu-iters", type=int, default=0,
+                    help="0: time the full CPU solve and compare solutions; N: time N "
+                         "iterations and scale")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--cold-reps", type=int, default=10,
@@ -218,6 +232,7 @@ def main():
     # ---- plain FP64 comparator ----------------------------------------------------
     fp64_ms = None
     fp64_perf = None
+    psi64 = None
     if args.fp64_steps > 0:
         step(fp64=True)
         ctx.synchronize()
@@ -226,6 +241,7 @@ def main():
             fp64_perf = step(fp64=True)
         ctx.synchronize()
         fp64_ms = 1e3 * (time.perf_counter() - t0) / args.fp64_steps
+        psi64 = d_psi.download()
         log(f"plain fp64: {fp64_ms:.3f} ms/solve, iters={fp64_perf.nIterations} "
             f"res={fp64_perf.finalResidual:.3e}")
 
@@ -304,10 +320,19 @@ def main():
             traffic = tj.get("bytes_per_launch")
 
     cpu = None
+    errors = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(m, solver, args.cpu_iters)
+        cpu, x_cpu = cpu_baseline(m, solver, args.cpu_iters)
         log(f"cpu baseline: {cpu['value']} ms ({cpu['sample']})")
+        if x_cpu is not None:
+            # both solutions meet the same 1e-6 normalised residual; their
+            # difference is what that tolerance leaves, not an error of either
+            errors = {"refined_vs_cpu_fp64_max_rel": rel_diff(psi, x_cpu)}
+            if psi64 is not None:
+                errors["plain_fp64_vs_cpu_fp64_max_rel"] = rel_diff(psi64, x_cpu)
+            log("solution vs CPU FP64 solve: " +
+                ", ".join(f"{k} {v:.3e}" for k, v in errors.items()))
 
     out = {
         "metric": METRIC,
@@ -368,6 +393,7 @@ def main():
             "deflated": perf.deflated,
             "polished": perf.polished,
             "host_arrays_ms": None if host_ms is None else round(host_ms, 3),
+            "vs_cpu_fp64": errors,
         },
         "fp64_plain": None if fp64_perf is None else {
             "ms": round(fp64_ms, 4),
