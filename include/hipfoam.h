@@ -297,13 +297,16 @@ int32_t hipfoam_synchronize(hipfoam_context* ctx);
  *   0 = losort-ordered lower coefficients (one indirection),
  *   1 = direct lower[losort[k]] / lowAddr[losort[k]] (two indirections),
  *   2 = row slots (the default): every row's <= 3 owner and <= 3 neighbour
- *       coefficients in six slot-major arrays with int16 column offsets, no
- *       row-start indirection; used when every row has <= 3 owner and
- *       <= 3 neighbour faces and every |upperAddr - lowerAddr| <= 32767,
- *       otherwise schedule 0 runs.  Same terms in the same order as 0 and 1,
- *       so all three give identical results,
+ *       coefficients in slot-major arrays (a symmetric matrix stores each
+ *       face once, on its owner side), columns from per-256-row-tile bases
+ *       (a 1-byte presence mask per row where every face sits at its base,
+ *       int16 deltas elsewhere), no row-start indirection; used when every
+ *       row has <= 3 owner and <= 3 neighbour faces and each tile's column
+ *       offsets fit the bases, otherwise schedule 0 runs.  Same terms in the
+ *       same order as 0 and 1, so all three give identical results,
  * and the iteration driver:
- *   graphChunk = iterations captured per hipGraph launch (0 = eager).
+ *   graphChunk = iterations captured per hipGraph launch (0 = eager; even;
+ *                default 4).
  * A row kernel launches min(ceil(n / blockSize), gridLimit) workgroups; each
  * walks a contiguous run of blockSize-row tiles. */
 typedef struct

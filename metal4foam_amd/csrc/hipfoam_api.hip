@@ -173,7 +173,7 @@ struct hipfoam_context {
   hipEvent_t evPoll[2] = {nullptr, nullptr};
   hipEvent_t evT0 = nullptr, evT1 = nullptr;
 
-  hipfoam_schedule sched{2, 256, 1, 8, 0, 0};
+  hipfoam_schedule sched{2, 256, 1, 4, 0, 0};
   std::map<GraphKey, hipGraphExec_t> graphs;
   KernelTimer timer;
 };
@@ -493,7 +493,7 @@ int run_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int 
              int maxIter, double* gpuSeconds) {
   const bool timing = c->timer.on;
   const bool eager = c->sched.graphChunk <= 0 || timing;
-  const int chunk = timing ? 1 : (c->sched.graphChunk > 0 ? c->sched.graphChunk : 8);
+  const int chunk = timing ? 1 : (c->sched.graphChunk > 0 ? c->sched.graphChunk : 4);
   hipGraphExec_t ex = nullptr;
   if (!eager) {
     int rc = get_graph<T>(c, kind, A, rD, order, chunk, &ex);

@@ -18,7 +18,7 @@ TIGHT = dict(tolerance=1e-12, maxRestarts=12, maxIter=20000)
 
 @pytest.fixture(autouse=True)
 def _default_schedule(ctx):
-    ctx.set_schedule(spmvSchedule=0, blockSize=256, xcdRemap=1, graphChunk=8, gridLimit=0,
+    ctx.set_schedule(spmvSchedule=0, blockSize=256, xcdRemap=1, graphChunk=4, gridLimit=0,
                      rowForm=0)
     yield
 

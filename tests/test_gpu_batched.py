@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(autouse=True)
 def _default_schedule(ctx):
-    ctx.set_schedule(spmvSchedule=0, blockSize=256, xcdRemap=1, graphChunk=8, gridLimit=0,
+    ctx.set_schedule(spmvSchedule=0, blockSize=256, xcdRemap=1, graphChunk=4, gridLimit=0,
                      rowForm=0)
     yield
 

@@ -27,7 +27,7 @@ def rand_asym(shape, seed):
 
 @pytest.fixture(autouse=True)
 def _default_schedule(ctx):
-    ctx.set_schedule(spmvSchedule=2, blockSize=256, xcdRemap=1, graphChunk=8, gridLimit=0,
+    ctx.set_schedule(spmvSchedule=2, blockSize=256, xcdRemap=1, graphChunk=4, gridLimit=0,
                      rowForm=0)
     yield
 
@@ -211,7 +211,7 @@ def test_solve_device_entry_and_graph_modes_agree_bitwise(ctx):
     ctx.set_topology(m)
     outs = []
     # graph chunks, eager launches, and the looped row form all give the same bits
-    for chunk, form in ((8, 0), (0, 0), (2, 0), (8, 1)):
+    for chunk, form in ((4, 0), (0, 0), (2, 0), (8, 0), (4, 1)):
         ctx.set_schedule(graphChunk=chunk, rowForm=form)
         psi = np.zeros(m.n)
         perf = ctx.solve(H.PCG, m.diag, m.upper, None, psi, m.source)
@@ -219,7 +219,7 @@ def test_solve_device_entry_and_graph_modes_agree_bitwise(ctx):
     for psi, it in outs[1:]:
         np.testing.assert_array_equal(psi, outs[0][0])
         assert it == outs[0][1]
-    ctx.set_schedule(graphChunk=8)
+    ctx.set_schedule(graphChunk=4)
     D = H.DeviceArray
     d, u, p, b = (D.from_host(ctx, m.diag), D.from_host(ctx, m.upper),
                   D.from_host(ctx, np.zeros(m.n)), D.from_host(ctx, m.source))

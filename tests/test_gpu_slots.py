@@ -98,7 +98,7 @@ def face_order_shuffled(seed):
 def spmv_all_schedules(ctx, m, x, lower):
     out = []
     for s in (0, 1, 2):
-        ctx.set_schedule(spmvSchedule=s, blockSize=256, xcdRemap=1, graphChunk=8,
+        ctx.set_schedule(spmvSchedule=s, blockSize=256, xcdRemap=1, graphChunk=4,
                          gridLimit=0, rowForm=0)
         ctx.set_matrix(m.diag, m.upper, lower)
         out.append(ctx.amul(x, H.FP32))
@@ -161,7 +161,7 @@ def test_refined_solve_identical_across_schedules(ctx, solver, mesh):
     ctx.set_topology(m)
     results = []
     for s in (0, 2):
-        ctx.set_schedule(spmvSchedule=s, blockSize=256, xcdRemap=1, graphChunk=8,
+        ctx.set_schedule(spmvSchedule=s, blockSize=256, xcdRemap=1, graphChunk=4,
                          gridLimit=0, rowForm=0)
         psi = np.zeros(m.n)
         perf = ctx.solve(solver, m.diag, m.upper, lower, psi, m.source,
@@ -177,7 +177,7 @@ def test_batched_slots_identical_to_face_loops(ctx):
                mg.poisson(9, 9, 9, seed=22, nrhs=2, rhs="normal")]
     outs = []
     for s in (0, 2):
-        ctx.set_schedule(spmvSchedule=s, blockSize=256, xcdRemap=1, graphChunk=8,
+        ctx.set_schedule(spmvSchedule=s, blockSize=256, xcdRemap=1, graphChunk=4,
                          gridLimit=0, rowForm=0)
         b = H.Batch(ctx, H.PCG, systems, 2)
         for k, m in enumerate(systems):
