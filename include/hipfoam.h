@@ -313,7 +313,13 @@ typedef struct
 {
     int32_t spmvSchedule;
     int32_t blockSize;    /* 64 .. 1024, multiple of 64 */
-    int32_t xcdRemap;     /* 1 = contiguous row range per XCD */
+    int32_t xcdRemap;     /* 0 = hardware order (workgroup b on XCD b mod 8),
+                             1 = contiguous row range per XCD,
+                             2 = auto (default): 0 when the mesh's largest
+                             stride is a multiple of 8 workgroups and its
+                             second stride at most 1/8 of a workgroup's
+                             rows, else 1; batches use 1.  Results do not
+                             depend on it */
     int32_t graphChunk;
     int32_t gridLimit;    /* max workgroups per row kernel; 0 = 8 per CU */
     int32_t rowForm;      /* 0 = unrolled predicated rows when every row has

@@ -138,6 +138,7 @@ struct hipfoam_context {
   int maxOwn = 0, maxNbr = 0;  // longest owner / neighbour face run of a row
   bool slotOK = false;  // row-slot layout possible (see build_slots)
   int64_t slotPatterned = 0;  // rows in patterned tiles
+  int64_t strideMajor = 0, strideMinor = 0;  // dominant offsets (SlotLayout)
   Buf<uint32_t> slotOff;
   Buf<int32_t> slotTile;
   Buf<uint8_t> slotRow;
@@ -173,7 +174,7 @@ struct hipfoam_context {
   hipEvent_t evPoll[2] = {nullptr, nullptr};
   hipEvent_t evT0 = nullptr, evT1 = nullptr;
 
-  hipfoam_schedule sched{2, 256, 1, 4, 0, 0};
+  hipfoam_schedule sched{2, 256, 2, 4, 0, 0};
   std::map<GraphKey, hipGraphExec_t> graphs;
   KernelTimer timer;
 };
@@ -188,6 +189,25 @@ inline int row_grid(const hipfoam_context* c) {
   const int tiles = std::max(1, (c->n + bs - 1) / bs);
   const int cap = c->sched.gridLimit > 0 ? c->sched.gridLimit : 8 * std::max(c->numCUs, 1);
   return std::min(tiles, cap);
+}
+
+// Block-to-XCD mapping of the row kernels.  Hardware dispatch places
+// workgroup b on XCD b mod 8; xcdRemap 1 gives each XCD one contiguous row
+// range instead.  Auto (2): keep the hardware order when it already keeps
+// gathers on one XCD's L2, i.e. when the mesh's largest stride is a
+// multiple of 8 workgroups and its second stride is at most an eighth of a
+// workgroup's rows (128^3: strides 16384 and 128 at 1024 rows per
+// workgroup); else contiguous ranges.  The mapping moves no arithmetic:
+// partials are indexed by logical block.
+inline int effective_remap(const hipfoam_context* c) {
+  if (c->sched.xcdRemap != 2) return c->sched.xcdRemap;
+  const int64_t bs = c->sched.blockSize;
+  const int64_t tiles = std::max<int64_t>(1, (c->n + bs - 1) / bs);
+  const int64_t g = row_grid(c);
+  const int64_t rows = (tiles + g - 1) / g * bs;  // rows per workgroup
+  const bool local = c->strideMajor > 0 && c->strideMajor % (8 * rows) == 0 &&
+                     c->strideMinor * 8 <= rows;
+  return local ? 0 : 1;
 }
 
 template <int NV, bool LASTMAX, typename F>
@@ -258,7 +278,7 @@ LduOp<T> make_op(hipfoam_context* c, const T* diag, const T* upper, const T* low
   op.diag = diag;
   op.upper = upper;
   op.n = c->n;
-  op.xcdRemap = c->sched.xcdRemap;
+  op.xcdRemap = effective_remap(c);
   op.shortRows = c->maxOwn <= kShortRow && c->maxNbr <= kShortRow && c->sched.rowForm == 0;
   op.direct = c->sched.spmvSchedule == 1;
   op.slots = slotCoef && c->sched.spmvSchedule == 2 && c->slotOK;
@@ -1255,6 +1275,8 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
   SlotLayout lay;
   c->slotOK = build_slots(nCells, ownStart, uppAddr, losortStart, losort, lowAddr, lay);
   c->slotPatterned = c->slotOK ? lay.patternedRows : 0;
+  c->strideMajor = c->slotOK ? lay.strideMajor : 0;
+  c->strideMinor = c->slotOK ? lay.strideMinor : 0;
   c->lastLow = lowAddr;
   c->lastUpp = uppAddr;
   c->have32 = c->have64 = false;
@@ -1599,8 +1621,9 @@ int32_t hipfoam_set_schedule(hipfoam_context* c, const hipfoam_schedule* s) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
   }
+  if (s->xcdRemap < 0 || s->xcdRemap > 2)
+    return fail(c, HIPFOAM_ERR_ARG, "xcdRemap must be 0, 1 or 2");
   c->sched = *s;
-  c->sched.xcdRemap = s->xcdRemap ? 1 : 0;
   clear_graphs(c);
   return HIPFOAM_OK;
 }

@@ -9,6 +9,8 @@
 #include <array>
 #include <climits>
 #include <cstdint>
+#include <map>
+#include <utility>
 #include <vector>
 
 namespace hipfoam {
@@ -55,6 +57,9 @@ struct SlotLayout {
   std::vector<uint32_t> words;  // kShortRow * n: int16 delta pairs
   std::vector<uint8_t> row;     // n: presence mask or packed owner slots
   int64_t patternedRows = 0;    // rows in patterned tiles
+  // the most common largest and second-largest owner offsets of the
+  // patterned tiles (the z and y strides of a structured block), 0 if none
+  int64_t strideMajor = 0, strideMinor = 0;
 };
 
 // Each row's owner faces go, in face order, into increasing slots among 0..2,
@@ -97,6 +102,7 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
   L.words.assign((size_t)K * n, 0);
   L.row.assign((size_t)n, 0);
   L.patternedRows = 0;
+  L.strideMajor = L.strideMinor = 0;
   std::vector<int16_t> delta((size_t)S * n, (int16_t)kSlotEmpty);
   // offsets of row i on one side (owner: side 0, neighbour: side 1), in face order
   auto offsets = [&](int i, int side, int64_t* off) -> int {
@@ -240,6 +246,7 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
       if (delta[(size_t)i * S + q] != kSlotEmpty) ownSlot[f++] = (int8_t)q;
   }
   std::vector<uint8_t> packed((size_t)n, 0);
+  std::map<std::pair<int64_t, int64_t>, int64_t> strides;
   for (int i = 0; i < n; ++i) {
     int k = losortStart[i];
     unsigned pk = 0;
@@ -270,6 +277,13 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
       for (int q = 0; q < K; ++q) pk |= (unsigned)std::max(ownQ[q], 0) << (2 * q);
       L.tile[(size_t)t * kSlotDesc + 6] = kSlotPatterned | (int32_t)(pk << 8);
       L.patternedRows += i1 - i0;
+      // the tile's two largest owner offsets
+      int64_t o[K];
+      for (int q = 0; q < K; ++q) o[q] = L.tile[(size_t)t * kSlotDesc + q];
+      std::sort(o, o + K);
+      const std::pair<int64_t, int64_t> key(std::max<int64_t>(o[K - 1], 0),
+                                            std::max<int64_t>(o[K - 2], 0));
+      ++strides[key];
     }
     for (int i = i0; i < i1; ++i) {
       const int16_t* d = &delta[(size_t)i * S];
@@ -282,6 +296,13 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
             (uint32_t)(uint16_t)d[2 * p] | ((uint32_t)(uint16_t)d[2 * p + 1] << 16);
     }
   }
+  int64_t bestCount = 0;
+  for (const auto& kv : strides)
+    if (kv.second > bestCount) {
+      bestCount = kv.second;
+      L.strideMajor = kv.first.first;
+      L.strideMinor = kv.first.second;
+    }
   return true;
 }
 

@@ -8,7 +8,7 @@
 // ownStart[n+1], losortStart[n+1], upperAddr[nFaces], lowerAddr[nFaces],
 // losort[nFaces].
 // Output: one line per mesh, "slots=<0|1> patterned=<rows> explicit=<rows>
-// errors=<count>".
+// errors=<count> major=<offset> minor=<offset>" (the dominant strides).
 
 #include <cstdio>
 #include <vector>
@@ -53,7 +53,7 @@ int main(int argc, char** argv) {
     const bool ok = build_slots(n, own.data(), upp.data(), los.data(), lso.data(), low.data(), L,
                                 nt > 0 ? ts.data() : nullptr, nt);
     if (!ok) {
-      printf("slots=0 patterned=0 explicit=0 errors=0\n");
+      printf("slots=0 patterned=0 explicit=0 errors=0 major=0 minor=0\n");
       continue;
     }
     long errors = 0;
@@ -102,8 +102,10 @@ int main(int argc, char** argv) {
       }
       if (k != los[i + 1]) ++errors;
     }
-    printf("slots=1 patterned=%lld explicit=%lld errors=%ld\n", (long long)L.patternedRows,
-           (long long)(n - L.patternedRows), errors + (patterned != L.patternedRows));
+    printf("slots=1 patterned=%lld explicit=%lld errors=%ld major=%lld minor=%lld\n",
+           (long long)L.patternedRows, (long long)(n - L.patternedRows),
+           errors + (patterned != L.patternedRows), (long long)L.strideMajor,
+           (long long)L.strideMinor);
   }
   fclose(f);
   return 0;

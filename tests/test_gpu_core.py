@@ -210,16 +210,18 @@ def test_solve_device_entry_and_graph_modes_agree_bitwise(ctx):
     m = mg.poisson(24, 20, 16, seed=7, lognormal=True)
     ctx.set_topology(m)
     outs = []
-    # graph chunks, eager launches, and the looped row form all give the same bits
-    for chunk, form in ((4, 0), (0, 0), (2, 0), (8, 0), (4, 1)):
-        ctx.set_schedule(graphChunk=chunk, rowForm=form)
+    # graph chunks, eager launches, the looped row form and every block-to-XCD
+    # mapping all give the same bits
+    for chunk, form, remap in ((4, 0, 1), (0, 0, 1), (2, 0, 1), (8, 0, 1), (4, 1, 1),
+                               (4, 0, 0), (4, 0, 2)):
+        ctx.set_schedule(graphChunk=chunk, rowForm=form, xcdRemap=remap)
         psi = np.zeros(m.n)
         perf = ctx.solve(H.PCG, m.diag, m.upper, None, psi, m.source)
         outs.append((psi, perf.nIterations))
     for psi, it in outs[1:]:
         np.testing.assert_array_equal(psi, outs[0][0])
         assert it == outs[0][1]
-    ctx.set_schedule(graphChunk=4)
+    ctx.set_schedule(graphChunk=4, xcdRemap=1)
     D = H.DeviceArray
     d, u, p, b = (D.from_host(ctx, m.diag), D.from_host(ctx, m.upper),
                   D.from_host(ctx, np.zeros(m.n)), D.from_host(ctx, m.source))

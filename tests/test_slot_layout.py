@@ -123,6 +123,12 @@ CASES = {
 }
 
 
+# dominant (largest, second-largest) owner offsets: the z and y strides
+STRIDES = {"box": (33 * 21, 33), "plane": (100, 1), "wide": (200 * 170, 200),
+           "thin": (35, 7), "line": (1, 0), "one_cell": (0, 0),
+           "blocks_work_tiles": (23 * 23, 23)}
+
+
 def run_checker(checker, tmp_path, meshes):
     """meshes: addressing tuples, each optionally followed by its tile starts."""
     path = tmp_path / "addr.bin"
@@ -161,6 +167,8 @@ def test_slot_layout_matches_face_loops(checker, tmp_path):
             assert res["slots"] == 1, name
         if expect == "all":
             assert res["patterned"] == n, name
+        if name in STRIDES:
+            assert (res["major"], res["minor"]) == STRIDES[name], name
         if expect == "mixed":
             assert 0 < res["explicit"] < n // 10, name
             assert res["patterned"] + res["explicit"] == n
@@ -170,4 +178,5 @@ def test_slot_layout_empty_mesh(checker, tmp_path):
     z = np.zeros(0, dtype=np.int32)
     res = run_checker(checker, tmp_path, [(0, z, z, np.zeros(1, np.int32), z,
                                            np.zeros(1, np.int32))])
-    assert res == [{"slots": 1, "patterned": 0, "explicit": 0, "errors": 0}]
+    assert res == [{"slots": 1, "patterned": 0, "explicit": 0, "errors": 0, "major": 0,
+                    "minor": 0}]
