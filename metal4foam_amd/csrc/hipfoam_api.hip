@@ -165,8 +165,10 @@ struct hipfoam_context {
   Work<double> w64;
 
   Buf<double> partials;
+  Buf<double> partials2;  // partials of the kernels that merge a second stage
   int pstride = 0;
   Buf<KState> kst;
+  Buf<KState> kst2;  // the Krylov state between the two merging kernels
   Buf<OState> ost;
   KState* hk = nullptr;  // pinned
   OState* ho = nullptr;  // pinned
@@ -381,6 +383,17 @@ std::pair<hipEvent_t, hipEvent_t> timer_slot(hipfoam_context* c) {
 // one Krylov iteration, enqueued on the context stream
 // -------------------------------------------------------------------------
 
+// Second stages merged into the next kernel (Merge) need 256-thread
+// workgroups: the fold then has k_finalize's tree, so merged and separate
+// stages give the same bits.  Every workgroup of the merging kernel reads all
+// partials, so merging pays only on small grids: 10k cells (40 workgroups)
+// 2.57 -> 2.44 ms; 128^3 (2048) unchanged; 96^3 BiCGStab (1728, three
+// values per partial) 3.19 -> 3.49 ms.
+constexpr int kMergeMaxBlocks = 512;
+inline bool merge_stages(const hipfoam_context* c) {
+  return c->sched.blockSize == 256 && row_grid(c) <= kMergeMaxBlocks;
+}
+
 template <typename T>
 void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order,
                       int parity, bool capturing) {
@@ -390,6 +403,9 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
   KState* st = c->kst.p;
   double* P = c->partials.p;
   const int ps = c->pstride;
+  // merged: beta's stage runs in k_pcg_p_spmv (state kst -> kst2, partials
+  // P -> partials2) and alpha's in k_pcg_update (kst2 -> kst, partials2 -> P)
+  const bool merge = merge_stages(c);
   if (order > 1) {
     const T* zin = nullptr;
     for (int s = 1; s < order; ++s) {
@@ -407,21 +423,34 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
       else
         LAUNCH_CLASSED(0, capturing, (k_cheb_step<T, false, false>), grid, block, A, w.r.p, rD,
                        zin, w.w.p, zout, st, P, ps, s);
-      if (last) launch_fin<1>(c, grid.x, true, kRunning, FinBeta<T>{st});
+      if (last && !merge) launch_fin<1>(c, grid.x, true, kRunning, FinBeta<T>{st});
       zin = zout;
     }
   } else {
     hipLaunchKernelGGL((k_jacobi_dot<T>), grid, block, 0, c->stream, c->n, A.xcdRemap, w.r.p,
                        rD, w.wA.p, st, P, ps);
-    launch_fin<1>(c, grid.x, true, kRunning, FinBeta<T>{st});
+    if (!merge) launch_fin<1>(c, grid.x, true, kRunning, FinBeta<T>{st});
   }
   T* pold = parity ? w.p1.p : w.p0.p;
   T* pnew = parity ? w.p0.p : w.p1.p;
-  LAUNCH_CLASSED(1, capturing, (k_pcg_p_spmv<T>), grid, block, A, w.wA.p, pold, pnew, w.Ap.p,
-                 st, P, ps);
-  launch_fin<1>(c, grid.x, true, kRunning, FinAlpha<T>{st});
-  hipLaunchKernelGGL((k_pcg_update<T>), grid, block, 0, c->stream, c->n, A.xcdRemap,
-                     w.delta.p, w.deltaBest.p, w.r.p, pnew, w.Ap.p, st, P, ps);
+  if (merge) {
+    KState* st2 = c->kst2.p;
+    double* P2 = c->partials2.p;
+    LAUNCH_CLASSED(1, capturing, (k_pcg_p_spmv<T, FinBeta<T>, true>), grid, block, A, w.wA.p,
+                   pold, pnew, w.Ap.p, st, P2, ps, Merge{st2, P, (int)grid.x},
+                   FinBeta<T>{nullptr});
+    hipLaunchKernelGGL((k_pcg_update<T, FinAlpha<T>, true>), grid, block, 0, c->stream, c->n,
+                       A.xcdRemap, w.delta.p, w.deltaBest.p, w.r.p, pnew, w.Ap.p, st2, P, ps,
+                       Merge{st, P2, (int)grid.x}, FinAlpha<T>{nullptr});
+  } else {
+    LAUNCH_CLASSED(1, capturing, (k_pcg_p_spmv<T, FinBeta<T>, false>), grid, block, A, w.wA.p,
+                   pold, pnew, w.Ap.p, st, P, ps, Merge{nullptr, nullptr, 0},
+                   FinBeta<T>{nullptr});
+    launch_fin<1>(c, grid.x, true, kRunning, FinAlpha<T>{st});
+    hipLaunchKernelGGL((k_pcg_update<T, FinAlpha<T>, false>), grid, block, 0, c->stream, c->n,
+                       A.xcdRemap, w.delta.p, w.deltaBest.p, w.r.p, pnew, w.Ap.p, st, P, ps,
+                       Merge{nullptr, nullptr, 0}, FinAlpha<T>{nullptr});
+  }
   launch_fin<1>(c, grid.x, true, kRunning, FinPcgRes<T>{st});
 }
 
@@ -440,14 +469,33 @@ void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int p
   T* pnew = parity ? w.p0.p : w.p1.p;
   T* AyAold = parity ? w.AyA1.p : w.AyA.p;
   T* AyAnew = parity ? w.AyA.p : w.AyA1.p;
+  // merged: alpha's stage runs in k_bicg_sz_spmv (state kst -> kst2,
+  // partials P -> partials2) and the |sA| test and omega in
+  // k_bicg_update_rho (kst2 -> kst, partials2 -> P)
+  const bool merge = merge_stages(c);
+  KState* st2 = merge ? c->kst2.p : st;
+  double* P2 = merge ? c->partials2.p : P;
+  const Merge none{nullptr, nullptr, 0};
   LAUNCH_CLASSED(2, capturing, (k_bicg_py_spmv<T>), grid, block, A, w.r.p, rD, pold, pnew,
                  AyAold, AyAnew, w.yA.p, w.rA0.p, st, P, ps);
-  launch_fin<1>(c, nb, true, kRunning, FinBicgAlpha<T>{st});
-  LAUNCH_CLASSED(2, capturing, (k_bicg_sz_spmv<T>), grid, block, A, w.r.p, rD, AyAnew, w.sA.p,
-                 w.zA.p, w.tA.p, st, P, ps);
-  launch_fin<3>(c, nb, true, kRunning, FinBicgSOmega<T>{st});
-  hipLaunchKernelGGL((k_bicg_update_rho<T>), grid, block, 0, c->stream, n, rm, w.delta.p,
-                     w.deltaBest.p, w.yA.p, w.zA.p, w.r.p, w.sA.p, w.tA.p, w.rA0.p, st, P, ps);
+  if (merge) {
+    LAUNCH_CLASSED(2, capturing, (k_bicg_sz_spmv<T, FinBicgAlpha<T>, true>), grid, block, A,
+                   w.r.p, rD, AyAnew, w.sA.p, w.zA.p, w.tA.p, st, P2, ps, Merge{st2, P, nb},
+                   FinBicgAlpha<T>{nullptr});
+    hipLaunchKernelGGL((k_bicg_update_rho<T, FinBicgSOmega<T>, true>), grid, block, 0,
+                       c->stream, n, rm, w.delta.p, w.deltaBest.p, w.yA.p, w.zA.p, w.r.p,
+                       w.sA.p, w.tA.p, w.rA0.p, st2, P, ps, Merge{st, P2, nb},
+                       FinBicgSOmega<T>{nullptr});
+  } else {
+    launch_fin<1>(c, nb, true, kRunning, FinBicgAlpha<T>{st});
+    LAUNCH_CLASSED(2, capturing, (k_bicg_sz_spmv<T, FinBicgAlpha<T>, false>), grid, block, A,
+                   w.r.p, rD, AyAnew, w.sA.p, w.zA.p, w.tA.p, st, P, ps, none,
+                   FinBicgAlpha<T>{nullptr});
+    launch_fin<3>(c, nb, true, kRunning, FinBicgSOmega<T>{st});
+    hipLaunchKernelGGL((k_bicg_update_rho<T, FinBicgSOmega<T>, false>), grid, block, 0,
+                       c->stream, n, rm, w.delta.p, w.deltaBest.p, w.yA.p, w.zA.p, w.r.p,
+                       w.sA.p, w.tA.p, w.rA0.p, st, P, ps, none, FinBicgSOmega<T>{nullptr});
+  }
   launch_fin<2>(c, nb, true, kRunningOrEarly, FinBicgUpdateRho<T>{st});
 }
 
@@ -1184,7 +1232,8 @@ int32_t hipfoam_create(int32_t device, hipfoam_context** out) {
   if (hipHostMalloc(reinterpret_cast<void**>(&c->hk), sizeof(KState)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&c->ho), sizeof(OState)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&c->hstatus), 4 * sizeof(int32_t)) != hipSuccess ||
-      c->kst.ensure(1) != hipSuccess || c->ost.ensure(1) != hipSuccess ||
+      c->kst.ensure(1) != hipSuccess || c->kst2.ensure(1) != hipSuccess ||
+      c->ost.ensure(1) != hipSuccess ||
       hipEventCreate(&c->evPoll[0]) != hipSuccess || hipEventCreate(&c->evPoll[1]) != hipSuccess ||
       hipEventCreate(&c->evT0) != hipSuccess || hipEventCreate(&c->evT1) != hipSuccess) {
     c->desc = "hipfoam: device allocation failed";
@@ -1192,6 +1241,7 @@ int32_t hipfoam_create(int32_t device, hipfoam_context** out) {
     return HIPFOAM_ERR_DEVICE;
   }
   (void)hipMemset(c->kst.p, 0, sizeof(KState));
+  (void)hipMemset(c->kst2.p, 0, sizeof(KState));
   (void)hipMemset(c->ost.p, 0, sizeof(OState));
   (void)hipDeviceSynchronize();
   char buf[512];
@@ -1317,6 +1367,7 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
   }
   c->pstride = std::max(1, blocks_for(nCells, 64));
   CK(c->partials.ensure((size_t)c->pstride * 4));
+  CK(c->partials2.ensure((size_t)c->pstride * 4));
   c->w32.release();
   c->w64.release();
   int rc = ensure_vectors(c);

@@ -240,6 +240,43 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
   if (threadIdx.x == 0) f(t);
 }
 
+// A reduction's second stage merged into the next kernel: the state it
+// writes and the previous kernel's partials (nblk workgroups).
+struct Merge {
+  KState* stOut;
+  const double* pin;
+  int nblk;
+};
+
+// Every workgroup reads the Krylov state stIn and the partials M.pin, folds
+// them exactly as k_finalize does (256-thread workgroups give the same fold
+// tree) and applies the finaliser f, when the state is running, to its own
+// LDS copy ks; workgroup 0 writes that copy to M.stOut.  No workgroup writes
+// memory another one reads (stIn, M.pin and this kernel's partials are three
+// different buffers), so the workgroups need no ordering.  Returns the state
+// the kernel's rows use.
+template <int NV, typename F>
+__device__ __forceinline__ const KState* merged_finalize(const KState* stIn, const Merge& M,
+                                                         int pstride, const F& f, KState* ks) {
+  constexpr int W = (int)(sizeof(KState) / sizeof(uint32_t));
+  static_assert(sizeof(KState) % sizeof(uint32_t) == 0 && W <= 256, "KState word copy");
+  const int k = threadIdx.x;
+  uint32_t* lds = reinterpret_cast<uint32_t*>(ks);
+  const uint32_t word = k < W ? reinterpret_cast<const uint32_t*>(stIn)[k] : 0u;
+  double t[NV];
+  fold_partials<NV, false>(M.pin, pstride, M.nblk, t);
+  if (k < W) lds[k] = word;
+  __syncthreads();
+  if (k == 0 && ks->status == ST_RUNNING) {
+    F g = f;
+    g.st = ks;
+    g(t);
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && k < W) reinterpret_cast<uint32_t*>(M.stOut)[k] = lds[k];
+  return ks;
+}
+
 // ---------------------------------------------------------------------------
 // the LDU row
 // ---------------------------------------------------------------------------
@@ -513,12 +550,19 @@ __global__ __launch_bounds__(1024) void k_jacobi_dot(int n, int xcdRemap, const 
 // metalFoamKernels.metal:62-109, 236-247); p ping-pongs (pold -> pnew) so
 // neighbour gathers read the previous p.  Finalises wApA and alpha
 // (reduce_scalars mode 1) or, in OpenFOAM mode, the singularity check.
-template <typename T>
+// MERGED: beta's finaliser (FinBeta of the Chebyshev or Jacobi dot) runs
+// here (merged_finalize).
+template <typename T, typename F, bool MERGED>
 __global__ __launch_bounds__(1024) void k_pcg_p_spmv(LduOp<T> A, const T* __restrict__ wA,
                                                      const T* __restrict__ pold,
                                                      T* __restrict__ pnew, T* __restrict__ Ap,
-                                                     KState* st, double* partials,
-                                                     int pstride) {
+                                                     const KState* stIn, double* partials,
+                                                     int pstride, Merge M, F fin) {
+  const KState* st = stIn;
+  if constexpr (MERGED) {
+    __shared__ KState ks;
+    st = merged_finalize<1>(stIn, M, pstride, fin, &ks);
+  }
   if (st->status != ST_RUNNING) return;
   const int lb = logical_block(A.xcdRemap);
   int r0, r1;
@@ -539,12 +583,19 @@ __global__ __launch_bounds__(1024) void k_pcg_p_spmv(LduOp<T> A, const T* __rest
 // delta += alpha p; r -= alpha Ap; partial sum|r| (pcg_update_sol_res,
 // metalFoamKernels.metal:250-276).  Before the update, copies delta to
 // deltaBest when the previous sync batch improved the residual.
-template <typename T>
+// MERGED: alpha's finaliser (FinAlpha of p.Ap) runs here.
+template <typename T, typename F, bool MERGED>
 __global__ __launch_bounds__(1024) void k_pcg_update(int n, int xcdRemap, T* __restrict__ delta,
                                                      T* __restrict__ deltaBest,
                                                      T* __restrict__ r, const T* __restrict__ p,
-                                                     const T* __restrict__ Ap, KState* st,
-                                                     double* partials, int pstride) {
+                                                     const T* __restrict__ Ap, const KState* stIn,
+                                                     double* partials, int pstride, Merge M,
+                                                     F fin) {
+  const KState* st = stIn;
+  if constexpr (MERGED) {
+    __shared__ KState ks;
+    st = merged_finalize<1>(stIn, M, pstride, fin, &ks);
+  }
   if (st->status != ST_RUNNING) return;
   const int lb = logical_block(xcdRemap);
   int r0, r1;
@@ -631,13 +682,20 @@ __global__ __launch_bounds__(1024) void k_bicg_py_spmv(LduOp<T> A, const T* __re
   block_partials<1>(v, partials, pstride, lb);
 }
 
-template <typename T>
+// MERGED: alpha's finaliser (FinBicgAlpha of rA0.AyA) runs here.
+template <typename T, typename F, bool MERGED>
 __global__ __launch_bounds__(1024) void k_bicg_sz_spmv(LduOp<T> A, const T* __restrict__ r,
                                                        const T* __restrict__ rD,
                                                        const T* __restrict__ AyA,
                                                        T* __restrict__ sA, T* __restrict__ zA,
-                                                       T* __restrict__ tA, KState* st,
-                                                       double* partials, int pstride) {
+                                                       T* __restrict__ tA, const KState* stIn,
+                                                       double* partials, int pstride, Merge M,
+                                                       F fin) {
+  const KState* st = stIn;
+  if constexpr (MERGED) {
+    __shared__ KState ks;
+    st = merged_finalize<1>(stIn, M, pstride, fin, &ks);
+  }
   if (st->status != ST_RUNNING) return;
   const int lb = logical_block(A.xcdRemap);
   int r0, r1;
@@ -663,7 +721,8 @@ __global__ __launch_bounds__(1024) void k_bicg_sz_spmv(LduOp<T> A, const T* __re
 
 // After an |sA| exit (status EARLY) only delta += alpha yA is applied
 // (bicg_early_update, metalFoamKernels.metal:415-426).
-template <typename T>
+// MERGED: the |sA| test and omega (FinBicgSOmega) run here.
+template <typename T, typename F, bool MERGED>
 __global__ __launch_bounds__(1024) void k_bicg_update_rho(int n, int xcdRemap,
                                                           T* __restrict__ delta,
                                                           T* __restrict__ deltaBest,
@@ -672,8 +731,14 @@ __global__ __launch_bounds__(1024) void k_bicg_update_rho(int n, int xcdRemap,
                                                           T* __restrict__ r,
                                                           const T* __restrict__ sA,
                                                           const T* __restrict__ tA,
-                                                          const T* __restrict__ rA0, KState* st,
-                                                          double* partials, int pstride) {
+                                                          const T* __restrict__ rA0,
+                                                          const KState* stIn, double* partials,
+                                                          int pstride, Merge M, F fin) {
+  const KState* st = stIn;
+  if constexpr (MERGED) {
+    __shared__ KState ks;
+    st = merged_finalize<3>(stIn, M, pstride, fin, &ks);
+  }
   const int status = st->status;
   if (status != ST_RUNNING && status != ST_EARLY) return;
   const int lb = logical_block(xcdRemap);
