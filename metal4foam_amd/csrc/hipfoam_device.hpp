@@ -413,6 +413,117 @@ __device__ __forceinline__ T ldu_row(const LduOp<T>& A, int i, T xi, XF xf) {
 }
 
 // ---------------------------------------------------------------------------
+// row loop of the single-system kernels
+// ---------------------------------------------------------------------------
+
+// Loads through the constant address space at a wave-uniform address compile
+// to scalar loads.  The tile descriptors are written once per mesh and only
+// read by the kernels.
+#define HF_SCALAR_AS __attribute__((address_space(4)))
+
+// A slot tile's descriptor in scalar registers.
+struct TileDesc {
+  int base[2 * kShortRow];
+  int flags;
+};
+
+__device__ __forceinline__ TileDesc tile_desc(const int32_t* slotTile, int t) {
+  const HF_SCALAR_AS int32_t* d =
+      (const HF_SCALAR_AS int32_t*)(slotTile + (size_t)t * kSlotDesc);
+  TileDesc s;
+#pragma unroll
+  for (int q = 0; q < 2 * kShortRow; ++q) s.base[q] = d[q];
+  s.flags = d[6];
+  return s;
+}
+
+// Row i of A x in a slot tile with descriptor d (PAT: a patterned tile).
+// Straight-line code: the row's coefficient, slot-byte, diagonal and x loads
+// are independent of one another, so they issue together; empty slots are
+// masked by selects.  The terms and their order are those of ldu_row_t.
+template <typename T, bool PAT, typename XF>
+__device__ __forceinline__ T slot_row(const LduOp<T>& A, const TileDesc& d, int i, T xi, XF xf) {
+  constexpr int K = kShortRow, S = 2 * kShortRow;
+  const size_t n = (size_t)A.n;
+  const T* __restrict__ C = A.slotCoef;
+  const bool sym = A.slotSym != 0;
+  T c[S];
+  int col[S];
+  unsigned m;
+#pragma unroll
+  for (int q = 0; q < K; ++q) c[q] = C[q * n + i];
+  const unsigned rb = A.slotRow[i];
+  if (PAT) {
+    // columns from the tile bases alone (an absent slot is clamped to a valid
+    // row); the neighbour faces' owner slots are the tile's (flags >> 8)
+    const unsigned own = (unsigned)d.flags >> 8;
+#pragma unroll
+    for (int q = 0; q < S; ++q) col[q] = min(max(i + d.base[q], 0), A.n - 1);
+#pragma unroll
+    for (int q = K; q < S; ++q) {
+      const int plane = sym ? (int)((own >> (2 * (q - K))) & 3u) : q;
+      c[q] = C[(size_t)plane * n + (sym ? col[q] : i)];
+    }
+    m = rb;
+  } else {
+    uint32_t w[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) w[p] = A.slotOff[p * n + i];
+    m = 0;
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const int d0 = (int)(int16_t)(w[p] & 0xffffu), d1 = (int)(int16_t)(w[p] >> 16);
+      const int o0 = d0 == kSlotEmpty ? 0 : d.base[2 * p] + d0;
+      const int o1 = d1 == kSlotEmpty ? 0 : d.base[2 * p + 1] + d1;
+      col[2 * p] = i + o0;
+      col[2 * p + 1] = i + o1;
+      m |= (o0 != 0 ? 1u : 0u) << (2 * p);
+      m |= (o1 != 0 ? 1u : 0u) << (2 * p + 1);
+    }
+#pragma unroll
+    for (int q = K; q < S; ++q) {
+      const size_t plane = sym ? ((rb >> (2 * (q - K))) & 3u) : (unsigned)q;
+      c[q] = C[plane * n + (sym ? col[q] : i)];
+    }
+  }
+  T s = A.diag[i] * xi;
+  T xq[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q) xq[q] = xf(col[q]);
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    const T t = s + c[q] * xq[q];
+    s = ((m >> q) & 1u) ? t : s;
+  }
+  return s;
+}
+
+// Calls fn(i, row) for every row i in [r0, r1), where row(xi, xf) returns row
+// i of A x for xi = x[i] and xf(j) = x[j].  With row slots the rows go in
+// passes of blockDim rows: a wavefront's 64 rows of a pass lie in one slot
+// tile (blockDim and the slot tile are multiples of 64, r0 a multiple of
+// blockDim), whose descriptor is read once per pass into scalar registers.
+template <typename T, typename Fn>
+__device__ __forceinline__ void for_rows(const LduOp<T>& A, int r0, int r1, Fn fn) {
+  if (A.slots) {
+    for (int base = r0; base < r1; base += blockDim.x) {
+      const int w0 = __builtin_amdgcn_readfirstlane(base + (int)(threadIdx.x & ~63u));
+      if (w0 >= r1) continue;
+      const TileDesc d = tile_desc(A.slotTile, w0 / kSlotTile);
+      const int i = base + (int)threadIdx.x;
+      if (i >= r1) continue;
+      if (d.flags & kSlotPatterned)
+        fn(i, [&](T xi, auto xf) { return slot_row<T, true>(A, d, i, xi, xf); });
+      else
+        fn(i, [&](T xi, auto xf) { return slot_row<T, false>(A, d, i, xi, xf); });
+    }
+  } else {
+    for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x)
+      fn(i, [&](T xi, auto xf) { return ldu_row_t(A, i, i / kSlotTile, xi, xf); });
+  }
+}
+
+// ---------------------------------------------------------------------------
 // scalar finalisation (runs in thread 0 of the last workgroup)
 // ---------------------------------------------------------------------------
 
@@ -503,23 +614,25 @@ __global__ __launch_bounds__(1024) void k_cheb_step(LduOp<T> A, const T* __restr
   const T c3 = (T)st->chebC3[step - 1];
   const T c1 = (T)st->chebC1;
   double dot = 0.0;
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+  for_rows(A, r0, r1, [&](int i, auto row) {
+    // the row's own operands load before the gathers, in one batch
+    const T ri = r[i], rDi = rD[i];
     T zi_new, wi;
     if (FIRST) {
-      const T z0 = c1 * rD[i] * r[i];
-      const T v = ldu_row(A, i, z0, [&](int j) { return c1 * rD[j] * r[j]; });
-      wi = c2 * z0 + c3 * rD[i] * (r[i] - v);
+      const T z0 = c1 * rDi * ri;
+      const T v = row(z0, [&](int j) { return c1 * rD[j] * r[j]; });
+      wi = c2 * z0 + c3 * rDi * (ri - v);
       zi_new = z0 + wi;
     } else {
-      const T zi = zin[i];
-      const T v = ldu_row(A, i, zi, [&](int j) { return zin[j]; });
-      wi = c2 * w[i] + c3 * rD[i] * (r[i] - v);
+      const T zi = zin[i], wold = w[i];
+      const T v = row(zi, [&](int j) { return zin[j]; });
+      wi = c2 * wold + c3 * rDi * (ri - v);
       zi_new = zi + wi;
     }
     if (!LAST) w[i] = wi;
     zout[i] = zi_new;
-    if (LAST) dot += (double)(zi_new * r[i]);
-  }
+    if (LAST) dot += (double)(zi_new * ri);
+  });
   if (LAST) {
     double v[1] = {dot};
     block_partials<1>(v, partials, pstride, lb);
@@ -569,13 +682,13 @@ __global__ __launch_bounds__(1024) void k_pcg_p_spmv(LduOp<T> A, const T* __rest
   block_rows(A.n, lb, r0, r1);
   const T beta = (T)st->beta;
   double dot = 0.0;
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+  for_rows(A, r0, r1, [&](int i, auto row) {
     const T pi = wA[i] + beta * pold[i];
-    const T y = ldu_row(A, i, pi, [&](int j) { return wA[j] + beta * pold[j]; });
+    const T y = row(pi, [&](int j) { return wA[j] + beta * pold[j]; });
     pnew[i] = pi;
     Ap[i] = y;
     dot += (double)(pi * y);
-  }
+  });
   double v[1] = {dot};
   block_partials<1>(v, partials, pstride, lb);
 }
@@ -666,18 +779,19 @@ __global__ __launch_bounds__(1024) void k_bicg_py_spmv(LduOp<T> A, const T* __re
   block_rows(A.n, lb, r0, r1);
   const T beta = (T)st->beta, omega = (T)st->omega;
   double d = 0.0;
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+  for_rows(A, r0, r1, [&](int i, auto row) {
+    const T a0 = rA0[i];
     const T pi = r[i] + beta * (pold[i] - omega * AyAold[i]);
     const T yi = rD[i] * pi;
-    const T y = ldu_row(A, i, yi, [&](int j) {
+    const T y = row(yi, [&](int j) {
       const T pj = r[j] + beta * (pold[j] - omega * AyAold[j]);
       return rD[j] * pj;
     });
     pnew[i] = pi;
     yA[i] = yi;
     AyAnew[i] = y;
-    d += (double)(rA0[i] * y);
-  }
+    d += (double)(a0 * y);
+  });
   double v[1] = {d};
   block_partials<1>(v, partials, pstride, lb);
 }
@@ -702,10 +816,10 @@ __global__ __launch_bounds__(1024) void k_bicg_sz_spmv(LduOp<T> A, const T* __re
   block_rows(A.n, lb, r0, r1);
   const T alpha = (T)st->alpha;
   double d[3] = {0.0, 0.0, 0.0};
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+  for_rows(A, r0, r1, [&](int i, auto row) {
     const T si = r[i] - alpha * AyA[i];
     const T zi = rD[i] * si;
-    const T t = ldu_row(A, i, zi, [&](int j) {
+    const T t = row(zi, [&](int j) {
       const T sj = r[j] - alpha * AyA[j];
       return rD[j] * sj;
     });
@@ -715,7 +829,7 @@ __global__ __launch_bounds__(1024) void k_bicg_sz_spmv(LduOp<T> A, const T* __re
     d[0] += (double)fabs(si);
     d[1] += (double)(si * t);
     d[2] += (double)(t * t);
-  }
+  });
   block_partials<3>(d, partials, pstride, lb);
 }
 
@@ -803,14 +917,15 @@ __global__ __launch_bounds__(1024) void k_rowsum(LduOp<double> A, const double* 
   int r0, r1;
   block_rows(A.n, lb, r0, r1);
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-    const double ui = ldu_row(A, i, 1.0, [](int) { return 1.0; });
+  for_rows(A, r0, r1, [&](int i, auto row) {
+    const double pi = psi[i], di = A.diag[i];
+    const double ui = row(1.0, [](int) { return 1.0; });
     u[i] = ui;
-    v[0] += psi[i];
-    v[1] += (fabs(ui) > 1e-8 * fabs(A.diag[i])) ? 1.0 : 0.0;
+    v[0] += pi;
+    v[1] += (fabs(ui) > 1e-8 * fabs(di)) ? 1.0 : 0.0;
     v[2] += ui * ui;
     v[3] += ui;
-  }
+  });
   block_partials<4>(v, partials, pstride, lb);
 }
 
@@ -827,15 +942,15 @@ __global__ __launch_bounds__(1024) void k_initial_residual(LduOp<double> A,
   block_rows(A.n, lb, r0, r1);
   const double xbar = os->xbar;
   double v[3] = {0.0, 0.0, 0.0};
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-    const double wA = ldu_row(A, i, psi[i], [&](int j) { return psi[j]; });
-    const double ri = b[i] - wA;
+  for_rows(A, r0, r1, [&](int i, auto row) {
+    const double bi = b[i], t = u[i] * xbar;
+    const double wA = row(psi[i], [&](int j) { return psi[j]; });
+    const double ri = bi - wA;
     r[i] = ri;
-    const double t = u[i] * xbar;
     v[0] += fabs(ri);
-    v[1] += fabs(wA - t) + fabs(b[i] - t);
+    v[1] += fabs(wA - t) + fabs(bi - t);
     v[2] += ri;
-  }
+  });
   block_partials<3>(v, partials, pstride, lb);
 }
 
@@ -920,17 +1035,18 @@ __global__ __launch_bounds__(1024) void k_refine_residual(
   block_rows(A.n, lb, r0, r1);
   const double dm = useMean ? os->dmean : 0.0;
   double v[3] = {0.0, 0.0, 0.0};
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-    auto pf = [&](int j) { return psiOld[j] + ((double)delta[j] - dm); };
+  auto pf = [&](int j) { return psiOld[j] + ((double)delta[j] - dm); };
+  for_rows(A, r0, r1, [&](int i, auto row) {
+    const double bi = b[i], ui = u[i];
     const double pi = pf(i);
-    const double wA = ldu_row(A, i, pi, pf);
+    const double wA = row(pi, pf);
     psiNew[i] = pi;
-    const double ri = b[i] - wA;
+    const double ri = bi - wA;
     r[i] = ri;
-    v[0] += u[i] * ri;
+    v[0] += ui * ri;
     v[1] += fabs(ri);
     v[2] += ri;
-  }
+  });
   block_partials<3>(v, partials, pstride, lb);
 }
 
@@ -961,8 +1077,7 @@ __global__ __launch_bounds__(1024) void k_spmv(LduOp<T> A, const T* __restrict__
   const int lb = logical_block(A.xcdRemap);
   int r0, r1;
   block_rows(A.n, lb, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x)
-    y[i] = ldu_row(A, i, x[i], [&](int j) { return x[j]; });
+  for_rows(A, r0, r1, [&](int i, auto row) { y[i] = row(x[i], [&](int j) { return x[j]; }); });
 }
 
 // reciprocal diagonal
