@@ -79,11 +79,12 @@ struct Buf {
 // Krylov workspace of one precision.
 template <typename T>
 struct Work {
-  Buf<T> r, p0, p1, wA, Ap, delta, deltaBest, z0, z1, w, rA0, yA, AyA, AyA1, sA, zA, tA;
+  // r1: the residual's second buffer (Chebyshev PCG: r ping-pongs, k_cheb_first)
+  Buf<T> r, r1, p0, p1, wA, Ap, delta, deltaBest, z0, z1, w, rA0, yA, AyA, AyA1, sA, zA, tA;
   size_t n = 0;
   hipError_t ensure(size_t nn) {
     if (n == nn && r.p) return hipSuccess;
-    Buf<T>* all[] = {&r, &p0, &p1, &wA, &Ap, &delta, &deltaBest, &z0,
+    Buf<T>* all[] = {&r, &r1, &p0, &p1, &wA, &Ap, &delta, &deltaBest, &z0,
                      &z1, &w, &rA0, &yA, &AyA, &AyA1, &sA, &zA, &tA};
     for (auto* b : all) {
       b->release();
@@ -94,7 +95,7 @@ struct Work {
     return hipSuccess;
   }
   void release() {
-    Buf<T>* all[] = {&r, &p0, &p1, &wA, &Ap, &delta, &deltaBest, &z0,
+    Buf<T>* all[] = {&r, &r1, &p0, &p1, &wA, &Ap, &delta, &deltaBest, &z0,
                      &z1, &w, &rA0, &yA, &AyA, &AyA1, &sA, &zA, &tA};
     for (auto* b : all) b->release();
     n = 0;
@@ -394,64 +395,102 @@ inline bool merge_stages(const hipfoam_context* c) {
   return c->sched.blockSize == 256 && row_grid(c) <= kMergeMaxBlocks;
 }
 
+// One PCG iteration.  With Chebyshev (order > 1):
+//   k_cheb_first   previous iteration's update (delta, r), first step
+//   k_cheb_step    steps 2 .. order-1 (the last also reduces rho)
+//   FinResBeta     previous iteration's checks, beta
+//   k_pcg_p_spmv   p = wA + beta p, A p, p.Ap
+//   FinAlpha       alpha (the update is left pending)
+// so an iteration is order + 2 launches; r and p ping-pong with the
+// iteration parity.  On small grids FinResBeta runs inside k_pcg_p_spmv and
+// FinAlpha inside the next k_cheb_first (state kst -> kst2 -> kst, partials
+// P -> partials2 -> P).  With Jacobi (order 1) the update is a kernel of its
+// own (k_pcg_update) followed by FinPcgRes.
 template <typename T>
 void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order,
                       int parity, bool capturing) {
   Work<T>& w = work<T>(c);
   const int bs = c->sched.blockSize;
   const dim3 grid(row_grid(c)), block(bs);
+  const int nb = grid.x;
   KState* st = c->kst.p;
   double* P = c->partials.p;
   const int ps = c->pstride;
-  // merged: beta's stage runs in k_pcg_p_spmv (state kst -> kst2, partials
-  // P -> partials2) and alpha's in k_pcg_update (kst2 -> kst, partials2 -> P)
   const bool merge = merge_stages(c);
-  if (order > 1) {
-    const T* zin = nullptr;
-    for (int s = 1; s < order; ++s) {
-      const bool first = (s == 1), last = (s == order - 1);
-      T* zout = last ? w.wA.p : ((s & 1) ? w.z0.p : w.z1.p);
-      if (first && last)
-        LAUNCH_CLASSED(3, capturing, (k_cheb_step<T, true, true>), grid, block, A, w.r.p, rD, zin,
-                       w.w.p, zout, st, P, ps, s);
-      else if (first)
-        LAUNCH_CLASSED(3, capturing, (k_cheb_step<T, true, false>), grid, block, A, w.r.p, rD,
-                       zin, w.w.p, zout, st, P, ps, s);
-      else if (last)
-        LAUNCH_CLASSED(4, capturing, (k_cheb_step<T, false, true>), grid, block, A, w.r.p, rD,
-                       zin, w.w.p, zout, st, P, ps, s);
-      else
-        LAUNCH_CLASSED(0, capturing, (k_cheb_step<T, false, false>), grid, block, A, w.r.p, rD,
-                       zin, w.w.p, zout, st, P, ps, s);
-      if (last && !merge) launch_fin<1>(c, grid.x, true, kRunning, FinBeta<T>{st});
-      zin = zout;
-    }
-  } else {
-    hipLaunchKernelGGL((k_jacobi_dot<T>), grid, block, 0, c->stream, c->n, A.xcdRemap, w.r.p,
-                       rD, w.wA.p, st, P, ps);
-    if (!merge) launch_fin<1>(c, grid.x, true, kRunning, FinBeta<T>{st});
-  }
+  const Merge none{nullptr, nullptr, 0};
   T* pold = parity ? w.p1.p : w.p0.p;
   T* pnew = parity ? w.p0.p : w.p1.p;
+  if (order > 1) {
+    const T* rin = parity ? w.r1.p : w.r.p;
+    T* rout = parity ? w.r.p : w.r1.p;
+    KState* st2 = c->kst2.p;
+    double* P2 = c->partials2.p;
+    T* zout = order == 2 ? w.wA.p : w.z0.p;
+    if (merge) {
+      if (order == 2)
+        LAUNCH_CLASSED(3, capturing, (k_cheb_first<T, true, FinAlpha<T>, true>), grid, block, A,
+                       rin, rout, rD, pold, w.Ap.p, w.delta.p, w.deltaBest.p, w.w.p, zout, st2,
+                       P, ps, Merge{st, P2, nb}, FinAlpha<T>{nullptr});
+      else
+        LAUNCH_CLASSED(3, capturing, (k_cheb_first<T, false, FinAlpha<T>, true>), grid, block,
+                       A, rin, rout, rD, pold, w.Ap.p, w.delta.p, w.deltaBest.p, w.w.p, zout,
+                       st2, P, ps, Merge{st, P2, nb}, FinAlpha<T>{nullptr});
+    } else {
+      if (order == 2)
+        LAUNCH_CLASSED(3, capturing, (k_cheb_first<T, true, FinAlpha<T>, false>), grid, block,
+                       A, rin, rout, rD, pold, w.Ap.p, w.delta.p, w.deltaBest.p, w.w.p, zout,
+                       st, P, ps, none, FinAlpha<T>{nullptr});
+      else
+        LAUNCH_CLASSED(3, capturing, (k_cheb_first<T, false, FinAlpha<T>, false>), grid, block,
+                       A, rin, rout, rD, pold, w.Ap.p, w.delta.p, w.deltaBest.p, w.w.p, zout,
+                       st, P, ps, none, FinAlpha<T>{nullptr});
+    }
+    const T* zin = zout;
+    for (int s = 2; s < order; ++s) {
+      const bool last = (s == order - 1);
+      T* zo = last ? w.wA.p : ((s & 1) ? w.z0.p : w.z1.p);
+      if (last)
+        LAUNCH_CLASSED(4, capturing, (k_cheb_step<T, false, true>), grid, block, A, rout, rD,
+                       zin, w.w.p, zo, st, P, ps, s);
+      else
+        LAUNCH_CLASSED(0, capturing, (k_cheb_step<T, false, false>), grid, block, A, rout, rD,
+                       zin, w.w.p, zo, st, P, ps, s);
+      zin = zo;
+    }
+    if (merge) {
+      LAUNCH_CLASSED(1, capturing, (k_pcg_p_spmv<T, FinResBeta<T>, true, 2>), grid, block, A,
+                     w.wA.p, pold, pnew, w.Ap.p, st, P2, ps, Merge{st2, P, nb},
+                     FinResBeta<T>{nullptr});
+    } else {
+      launch_fin<2>(c, nb, true, kRunning, FinResBeta<T>{st});
+      LAUNCH_CLASSED(1, capturing, (k_pcg_p_spmv<T, FinResBeta<T>, false, 2>), grid, block, A,
+                     w.wA.p, pold, pnew, w.Ap.p, st, P, ps, none, FinResBeta<T>{nullptr});
+      launch_fin<1>(c, nb, true, kRunning, FinAlpha<T>{st});
+    }
+    return;
+  }
+  // merged: beta's stage runs in k_pcg_p_spmv (state kst -> kst2, partials
+  // P -> partials2) and alpha's in k_pcg_update (kst2 -> kst, partials2 -> P)
+  hipLaunchKernelGGL((k_jacobi_dot<T>), grid, block, 0, c->stream, c->n, A.xcdRemap, w.r.p, rD,
+                     w.wA.p, st, P, ps);
+  if (!merge) launch_fin<1>(c, nb, true, kRunning, FinBeta<T>{st});
   if (merge) {
     KState* st2 = c->kst2.p;
     double* P2 = c->partials2.p;
     LAUNCH_CLASSED(1, capturing, (k_pcg_p_spmv<T, FinBeta<T>, true>), grid, block, A, w.wA.p,
-                   pold, pnew, w.Ap.p, st, P2, ps, Merge{st2, P, (int)grid.x},
-                   FinBeta<T>{nullptr});
+                   pold, pnew, w.Ap.p, st, P2, ps, Merge{st2, P, nb}, FinBeta<T>{nullptr});
     hipLaunchKernelGGL((k_pcg_update<T, FinAlpha<T>, true>), grid, block, 0, c->stream, c->n,
                        A.xcdRemap, w.delta.p, w.deltaBest.p, w.r.p, pnew, w.Ap.p, st2, P, ps,
-                       Merge{st, P2, (int)grid.x}, FinAlpha<T>{nullptr});
+                       Merge{st, P2, nb}, FinAlpha<T>{nullptr});
   } else {
     LAUNCH_CLASSED(1, capturing, (k_pcg_p_spmv<T, FinBeta<T>, false>), grid, block, A, w.wA.p,
-                   pold, pnew, w.Ap.p, st, P, ps, Merge{nullptr, nullptr, 0},
-                   FinBeta<T>{nullptr});
-    launch_fin<1>(c, grid.x, true, kRunning, FinAlpha<T>{st});
+                   pold, pnew, w.Ap.p, st, P, ps, none, FinBeta<T>{nullptr});
+    launch_fin<1>(c, nb, true, kRunning, FinAlpha<T>{st});
     hipLaunchKernelGGL((k_pcg_update<T, FinAlpha<T>, false>), grid, block, 0, c->stream, c->n,
                        A.xcdRemap, w.delta.p, w.deltaBest.p, w.r.p, pnew, w.Ap.p, st, P, ps,
-                       Merge{nullptr, nullptr, 0}, FinAlpha<T>{nullptr});
+                       none, FinAlpha<T>{nullptr});
   }
-  launch_fin<1>(c, grid.x, true, kRunning, FinPcgRes<T>{st});
+  launch_fin<1>(c, nb, true, kRunning, FinPcgRes<T>{st});
 }
 
 template <typename T>
@@ -645,7 +684,9 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
   s.status = maxIter > 0 ? ST_RUNNING : ST_MAXITER;
   if (kind == KIND_PCG && order > 1) cheb_constants(s, order, beta, ratio, true);
   *c->hk = s;
+  // both state buffers: merged stages read the first iteration's state from kst2
   CK(hipMemcpyAsync(c->kst.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+  CK(hipMemcpyAsync(c->kst2.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
   double gpu = 0;
   if (s.status == ST_RUNNING) {
     const LduOp<float> A = op32(c);
@@ -1005,6 +1046,7 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
     if (order > 1) cheb_constants(s, order, beta, ctl->chebyshevRatio, false);
     *c->hk = s;
     CK(hipMemcpyAsync(c->kst.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+    CK(hipMemcpyAsync(c->kst2.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
     const LduOp<double> A = op64(c);
     double gpu = 0;
     const double tw = now_seconds();
@@ -1030,7 +1072,10 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
       kind == hipMemcpyHostToDevice ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
   CK(hipMemcpyAsync(psi, result, sizeof(double) * n, back, c->stream));
   if (residual) {
-    const double* rr = (result == c->psi0.p) ? c->r64.p : w.r.p;
+    // Chebyshev PCG: the residual of the last update went to the buffer of
+    // parity it (k_cheb_first of iteration it wrote it)
+    const double* rlast = (!useBiCG && order > 1 && (c->hk->it & 1) == 0) ? w.r1.p : w.r.p;
+    const double* rr = (result == c->psi0.p) ? c->r64.p : rlast;
     CK(hipMemcpyAsync(residual, rr, sizeof(double) * n, back, c->stream));
   }
   CK(hipStreamSynchronize(c->stream));

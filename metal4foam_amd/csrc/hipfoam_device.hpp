@@ -61,7 +61,11 @@ struct alignas(128) KState {
   double normFactor, initRes, tol, relTol;            // OpenFOAM mode
   double chebC1, chebC2[kMaxCheb], chebC3[kMaxCheb];
   int32_t it, maxIter, minIter, ips, batchSize, batchLeft, noProgress, status;
-  int32_t copyBest, mode, pad0, pad1;
+  int32_t copyBest, mode;
+  // PCG with Chebyshev: alphaDue = a p.Ap reduction follows the last beta;
+  // pend = delta and r await the update with the last alpha (applied by the
+  // next iteration's first Chebyshev step, k_cheb_first)
+  int32_t pend, alphaDue;
 };
 
 // Outer (FP64 refinement) scalars.
@@ -588,6 +592,7 @@ __device__ __forceinline__ void finalize_beta(KState* st, double t) {
     st->rho = t;
     st->beta = first ? 0.0 : t / st->rhoOld;
   }
+  st->alphaDue = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -639,6 +644,70 @@ __global__ __launch_bounds__(1024) void k_cheb_step(LduOp<T> A, const T* __restr
   }
 }
 
+// First Chebyshev step of a PCG iteration fused with the previous
+// iteration's update (pcg_update_sol_res + cheb_init + ldu_spmv + cheb_step,
+// metalFoamKernels.metal:112-194, 250-276).  When the state has an update
+// pending (pend): deltaBest = delta if the last check improved the residual,
+// delta += alpha p, and the updated residual rn = r - alpha Ap is written to
+// rout; neighbour gathers form rn on the fly from r and Ap, so r ping-pongs
+// (rin -> rout) between iterations.  Partial sum|rn| goes to row 1 of the
+// partials (FinResBeta checks it with the next beta).  Without a pending
+// update (first iteration) rout = rin.  Then z_0 = c1 rD rn and the first
+// step as in k_cheb_step; LAST also reduces rho = dot(wA, rn) into row 0.
+// MERGED: alpha's finaliser (FinAlpha of p.Ap) runs here.
+template <typename T, bool LAST, typename F, bool MERGED>
+__global__ __launch_bounds__(1024) void k_cheb_first(
+    LduOp<T> A, const T* __restrict__ rin, T* __restrict__ rout, const T* __restrict__ rD,
+    const T* __restrict__ p, const T* __restrict__ Ap, T* __restrict__ delta,
+    T* __restrict__ deltaBest, T* __restrict__ w, T* __restrict__ zout, const KState* stIn,
+    double* partials, int pstride, Merge M, F fin) {
+  const KState* st = stIn;
+  if constexpr (MERGED) {
+    __shared__ KState ks;
+    st = merged_finalize<1>(stIn, M, pstride, fin, &ks);
+  }
+  if (st->status != ST_RUNNING) return;
+  const int lb = logical_block(A.xcdRemap);
+  int r0, r1;
+  block_rows(A.n, lb, r0, r1);
+  const T c2 = (T)st->chebC2[0];
+  const T c3 = (T)st->chebC3[0];
+  const T c1 = (T)st->chebC1;
+  double v[2] = {0.0, 0.0};  // rho, sum|rn|
+  if (st->pend) {
+    const T alpha = (T)st->alpha;
+    const bool copy = st->copyBest != 0;
+    for_rows(A, r0, r1, [&](int i, auto row) {
+      const T ri = rin[i] - alpha * Ap[i];
+      const T di = delta[i], pi = p[i], rDi = rD[i];
+      const T z0 = c1 * rDi * ri;
+      const T y = row(z0, [&](int j) { return c1 * rD[j] * (rin[j] - alpha * Ap[j]); });
+      const T wi = c2 * z0 + c3 * rDi * (ri - y);
+      const T zn = z0 + wi;
+      if (copy) deltaBest[i] = di;
+      delta[i] = di + alpha * pi;
+      rout[i] = ri;
+      if (!LAST) w[i] = wi;
+      zout[i] = zn;
+      v[1] += (double)fabs(ri);
+      if (LAST) v[0] += (double)(zn * ri);
+    });
+  } else {
+    for_rows(A, r0, r1, [&](int i, auto row) {
+      const T ri = rin[i], rDi = rD[i];
+      const T z0 = c1 * rDi * ri;
+      const T y = row(z0, [&](int j) { return c1 * rD[j] * rin[j]; });
+      const T wi = c2 * z0 + c3 * rDi * (ri - y);
+      const T zn = z0 + wi;
+      rout[i] = ri;
+      if (!LAST) w[i] = wi;
+      zout[i] = zn;
+      if (LAST) v[0] += (double)(zn * ri);
+    });
+  }
+  block_partials<2>(v, partials, pstride, lb);
+}
+
 // Jacobi apply fused with dot(wA, rA) (pcg_precond_dot, metalFoamKernels.metal:211-233)
 template <typename T>
 __global__ __launch_bounds__(1024) void k_jacobi_dot(int n, int xcdRemap, const T* __restrict__ r,
@@ -663,9 +732,10 @@ __global__ __launch_bounds__(1024) void k_jacobi_dot(int n, int xcdRemap, const 
 // metalFoamKernels.metal:62-109, 236-247); p ping-pongs (pold -> pnew) so
 // neighbour gathers read the previous p.  Finalises wApA and alpha
 // (reduce_scalars mode 1) or, in OpenFOAM mode, the singularity check.
-// MERGED: beta's finaliser (FinBeta of the Chebyshev or Jacobi dot) runs
-// here (merged_finalize).
-template <typename T, typename F, bool MERGED>
+// MERGED: beta's finaliser runs here (merged_finalize over NVM partial
+// rows): FinBeta of the Jacobi dot, or FinResBeta of the last Chebyshev
+// step's rho and k_cheb_first's sum|r|.
+template <typename T, typename F, bool MERGED, int NVM = 1>
 __global__ __launch_bounds__(1024) void k_pcg_p_spmv(LduOp<T> A, const T* __restrict__ wA,
                                                      const T* __restrict__ pold,
                                                      T* __restrict__ pnew, T* __restrict__ Ap,
@@ -674,7 +744,7 @@ __global__ __launch_bounds__(1024) void k_pcg_p_spmv(LduOp<T> A, const T* __rest
   const KState* st = stIn;
   if constexpr (MERGED) {
     __shared__ KState ks;
-    st = merged_finalize<1>(stIn, M, pstride, fin, &ks);
+    st = merged_finalize<NVM>(stIn, M, pstride, fin, &ks);
   }
   if (st->status != ST_RUNNING) return;
   const int lb = logical_block(A.xcdRemap);
@@ -1192,20 +1262,27 @@ struct FinBeta {
 
 // wApA, alpha (reduce_scalars mode 1); OpenFOAM mode: checkSingularity(|wApA|/normFactor)
 template <typename T>
+// Runs only after a beta (alphaDue): merged into k_cheb_first, it also sees
+// the first iteration, where no p.Ap has been reduced.  Leaves the update
+// pending (pend) for k_cheb_first.
 struct FinAlpha {
   KState* st;
   __device__ void operator()(const double* t) const {
+    if (!st->alphaDue) return;
+    st->alphaDue = 0;
     if (st->mode == MODE_INNER) {
       const T wApA = (T)t[0];
       st->wApA = wApA;
       st->alpha = (double)((T)st->rho / wApA);
     } else {
       st->wApA = t[0];
-      if (fabs(t[0]) / st->normFactor < kVSmall)
+      if (fabs(t[0]) / st->normFactor < kVSmall) {
         st->status = ST_SINGULAR;
-      else
-        st->alpha = st->rho / t[0];
+        return;
+      }
+      st->alpha = st->rho / t[0];
     }
+    st->pend = 1;
   }
 };
 
@@ -1221,6 +1298,23 @@ struct FinPcgRes {
       st->res = t[0] / st->normFactor;
       openfoam_iteration_end(st);
     }
+  }
+};
+
+// Chebyshev PCG: t[0] = rho (last Chebyshev step), t[1] = sum|r| of the
+// update k_cheb_first applied.  The previous iteration's end-of-iteration
+// checks (FinPcgRes) run first, then beta, exactly as the separate stages
+// would in that order.
+template <typename T>
+struct FinResBeta {
+  KState* st;
+  __device__ void operator()(const double* t) const {
+    if (st->pend) {
+      st->pend = 0;
+      FinPcgRes<T>{st}(t + 1);
+      if (st->status != ST_RUNNING) return;
+    }
+    finalize_beta<T>(st, t[0]);
   }
 };
 
