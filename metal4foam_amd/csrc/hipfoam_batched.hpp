@@ -170,22 +170,24 @@ __global__ __launch_bounds__(256) void kb_cheb_step(BatchMap B, LduOp<float> A,
       const float c2 = (float)st->chebC2[step - 1];
       const float c3 = (float)st->chebC3[step - 1];
       const float c1 = (float)st->chebC1;
+      // the row's own operands load before the gathers, in one batch
+      const float ri = rr[i], di = dd[i];
       float zi_new, wi;
       if (FIRST) {
-        const float z0 = c1 * dd[i] * rr[i];
+        const float z0 = c1 * di * ri;
         const float y = batch_row(A, rs, voff, i, z0, [&](int j) { return c1 * dd[j] * rr[j]; });
-        wi = c2 * z0 + c3 * dd[i] * (rr[i] - y);
+        wi = c2 * z0 + c3 * di * (ri - y);
         zi_new = z0 + wi;
       } else {
         const float* zi_ = zin + voff;
-        const float zi = zi_[i];
+        const float zi = zi_[i], wold = w[voff + i];
         const float y = batch_row(A, rs, voff, i, zi, [&](int j) { return zi_[j]; });
-        wi = c2 * w[voff + i] + c3 * dd[i] * (rr[i] - y);
+        wi = c2 * wold + c3 * di * (ri - y);
         zi_new = zi + wi;
       }
       if (!LAST) w[voff + i] = wi;
       zout[voff + i] = zi_new;
-      if (LAST) v[0] = (double)(zi_new * rr[i]);
+      if (LAST) v[0] = (double)(zi_new * ri);
     }
     if (LAST) part.add(q, v);
   }
@@ -314,6 +316,7 @@ __global__ __launch_bounds__(256) void kb_bicg_py_spmv(BatchMap B, LduOp<float> 
       const float* d_ = rD + voff;
       const float* po = pold + voff;
       const float* ao = AyAold + voff;
+      const float a0 = rA0[voff + i];
       const float pi = r_[i] + beta * (po[i] - omega * ao[i]);
       const float yi = d_[i] * pi;
       const float y = batch_row(A, rs, voff, i, yi, [&](int j) {
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(256) void kb_bicg_py_spmv(BatchMap B, LduOp<float> 
       pnew[voff + i] = pi;
       yA[voff + i] = yi;
       AyAnew[voff + i] = y;
-      v[0] = (double)(rA0[voff + i] * y);
+      v[0] = (double)(a0 * y);
     }
     part.add(q, v);
   }

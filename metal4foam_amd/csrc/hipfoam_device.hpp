@@ -352,8 +352,10 @@ struct RowSlots {
 #pragma unroll
     for (int q = 0; q < 2 * kShortRow; ++q) xq[q] = xf(col[q]);
 #pragma unroll
-    for (int q = 0; q < 2 * kShortRow; ++q)
-      if ((m >> q) & 1u) s += c[q] * xq[q];
+    for (int q = 0; q < 2 * kShortRow; ++q) {
+      const T t = s + c[q] * xq[q];
+      s = ((m >> q) & 1u) ? t : s;
+    }
     return s;
   }
 };
