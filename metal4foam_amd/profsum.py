@@ -64,30 +64,30 @@ def _bytes(name: str, n: int, nf: int, asym: bool, slots: bool = False,
 
 
 def _ldu_bytes(name: str, n: int, nf: int, asym: bool) -> int | None:
+    """LDU-format algorithmic bytes per launch, matched on the kernel name's
+    prefix (template arguments after the first ones are finalisers)."""
     face32 = 20 if asym else 16
     face64 = 28 if asym else 24
-    if "k_cheb_step<float, false, false>" in name:
-        return 36 * n + 8 + face32 * nf
-    if "k_cheb_step<float, true, false>" in name:
-        return 28 * n + 8 + face32 * nf
-    if "k_cheb_step<float, false, true>" in name:
-        return 32 * n + 8 + face32 * nf
-    if "k_pcg_p_spmv<float>" in name:
-        return 28 * n + 8 + face32 * nf
-    if "k_pcg_update<float>" in name:
-        return 24 * n
-    if "k_spmv<float>" in name:
-        return 20 * n + 8 + face32 * nf
-    if "k_cheb_step<double, false, false>" in name:
-        return 64 * n + 8 + face64 * nf
-    if "k_pcg_p_spmv<double>" in name:
-        return 48 * n + 8 + face64 * nf
-    if "k_bicg_py_spmv<float>" in name:
-        return 44 * n + 8 + face32 * nf
-    if "k_bicg_sz_spmv<float>" in name:
-        return 36 * n + 8 + face32 * nf
-    if "k_bicg_update_rho<float>" in name:
-        return 32 * n
+    table = (
+        ("k_cheb_step<float, false, false>", 36 * n + 8 + face32 * nf),
+        ("k_cheb_step<float, true, false>", 28 * n + 8 + face32 * nf),
+        ("k_cheb_step<float, false, true>", 32 * n + 8 + face32 * nf),
+        # previous iteration's update applied (every launch but an inner
+        # solve's first): reads r, Ap, delta, p, rD; writes r, delta, w, z
+        ("k_cheb_first<float, false,", 48 * n + 8 + face32 * nf),
+        ("k_pcg_p_spmv<float", 28 * n + 8 + face32 * nf),
+        ("k_pcg_update<float", 24 * n),
+        ("k_spmv<float", 20 * n + 8 + face32 * nf),
+        ("k_cheb_step<double, false, false>", 64 * n + 8 + face64 * nf),
+        ("k_cheb_first<double, false,", 88 * n + 8 + face64 * nf),
+        ("k_pcg_p_spmv<double", 48 * n + 8 + face64 * nf),
+        ("k_bicg_py_spmv<float", 44 * n + 8 + face32 * nf),
+        ("k_bicg_sz_spmv<float", 36 * n + 8 + face32 * nf),
+        ("k_bicg_update_rho<float", 32 * n),
+    )
+    for prefix, b in table:
+        if prefix in name:
+            return b
     return None
 
 
