@@ -307,12 +307,20 @@ int32_t hipfoam_synchronize(hipfoam_context* ctx);
  * and the iteration driver:
  *   graphChunk = iterations captured per hipGraph launch (0 = eager; even;
  *                default 4).
- * A row kernel launches min(ceil(n / blockSize), gridLimit) workgroups; each
- * walks a contiguous run of blockSize-row tiles. */
+ * blockSize 0 (auto, the default) picks the launch shape from the mesh: one
+ * 256-thread workgroup per 256-row tile up to 512 tiles, 512 workgroups of
+ * 512 threads up to 49,152 tiles (every reduction's second stage then runs
+ * inside the next row kernel), and 8 workgroups of 256 threads per CU
+ * beyond.  An explicit blockSize launches min(ceil(n / blockSize),
+ * gridLimit) workgroups; each walks a contiguous run of blockSize-row tiles,
+ * and second stages run inside the next kernel when there are at most 512
+ * workgroups of at least 256 threads.  Results are deterministic for a given
+ * mesh and schedule; grids of different sizes group the reductions
+ * differently, so their FP64 sums may differ in the last bits. */
 typedef struct
 {
     int32_t spmvSchedule;
-    int32_t blockSize;    /* 64 .. 1024, multiple of 64 */
+    int32_t blockSize;    /* 0 = auto (default), or 64 .. 1024, multiple of 64 */
     int32_t xcdRemap;     /* 0 = hardware order (workgroup b on XCD b mod 8),
                              1 = contiguous row range per XCD,
                              2 = auto (default): 0 when the mesh's largest
@@ -321,7 +329,8 @@ typedef struct
                              rows, else 1; batches use 1.  Results do not
                              depend on it */
     int32_t graphChunk;
-    int32_t gridLimit;    /* max workgroups per row kernel; 0 = 8 per CU */
+    int32_t gridLimit;    /* max workgroups per row kernel; 0 = 8 per CU
+                             (auto blockSize: its own grid) */
     int32_t rowForm;      /* 0 = unrolled predicated rows when every row has
                              <= 3 owner and <= 3 neighbour faces, else loops;
                              1 = always loops */

@@ -177,7 +177,7 @@ struct hipfoam_context {
   hipEvent_t evPoll[2] = {nullptr, nullptr};
   hipEvent_t evT0 = nullptr, evT1 = nullptr;
 
-  hipfoam_schedule sched{2, 256, 2, 4, 0, 0};
+  hipfoam_schedule sched{2, 0, 2, 4, 0, 0};
   std::map<GraphKey, hipGraphExec_t> graphs;
   KernelTimer timer;
 };
@@ -185,12 +185,41 @@ struct hipfoam_context {
 namespace {
 
 
-// Workgroups of a row-kernel launch: one per tile of blockSize rows, capped at
-// gridLimit (each workgroup then walks a contiguous run of tiles).
+// Launch shape of the row kernels.  blockSize 0 (auto, the default), by the
+// mesh's number of 256-row tiles:
+//  - at most kAutoSmallTiles: one 256-thread workgroup per tile;
+//  - up to kAutoMidTiles: kAutoMidGrid workgroups of kAutoMidBlock threads,
+//    each walking a contiguous run of tiles;
+//  - beyond: 8 workgroups of 256 threads per CU.
+// The first two stay within kMergeMaxBlocks, so every reduction's second
+// stage runs inside the next row kernel.  Against 8 x 256 per CU (same box,
+// lognormal Poisson PCG unless noted): 128^3 16.2 -> 15.7 ms, 160^3 44.0 ->
+// 36.5 ms, 192^3 88.6 -> 83.3 ms, 224^3 170.3 -> 167.0 ms, 128^3 BiCGStab
+// 4.16 -> 3.86 ms, 96^3 BiCGStab 2.91 -> 2.80 ms; 96^3 7.26 vs 7.28 ms; at
+// 256^3 (65,536 tiles) 8 x 256 per CU is 3-6 % faster, where every pass
+// streams from HBM and full occupancy pays.  An explicit blockSize launches
+// min(ceil(n / blockSize), gridLimit) workgroups (gridLimit 0: 8 per CU).
+constexpr int kMergeMaxBlocks = 512;
+constexpr int kAutoSmallTiles = 512;
+constexpr int kAutoMidTiles = 49152;
+constexpr int kAutoMidBlock = 512;
+constexpr int kAutoMidGrid = 512;
+
+inline int64_t tiles256(const hipfoam_context* c) { return ((int64_t)c->n + 255) / 256; }
+
+inline int row_block(const hipfoam_context* c) {
+  if (c->sched.blockSize > 0) return c->sched.blockSize;
+  const int64_t t = tiles256(c);
+  return (t <= kAutoSmallTiles || t > kAutoMidTiles) ? 256 : kAutoMidBlock;
+}
+
 inline int row_grid(const hipfoam_context* c) {
-  const int bs = c->sched.blockSize;
+  const int bs = row_block(c);
   const int tiles = std::max(1, (c->n + bs - 1) / bs);
-  const int cap = c->sched.gridLimit > 0 ? c->sched.gridLimit : 8 * std::max(c->numCUs, 1);
+  int cap = c->sched.gridLimit > 0 ? c->sched.gridLimit : 8 * std::max(c->numCUs, 1);
+  if (c->sched.blockSize == 0 && c->sched.gridLimit <= 0 && tiles256(c) > kAutoSmallTiles &&
+      tiles256(c) <= kAutoMidTiles)
+    cap = kAutoMidGrid;
   return std::min(tiles, cap);
 }
 
@@ -204,7 +233,7 @@ inline int row_grid(const hipfoam_context* c) {
 // partials are indexed by logical block.
 inline int effective_remap(const hipfoam_context* c) {
   if (c->sched.xcdRemap != 2) return c->sched.xcdRemap;
-  const int64_t bs = c->sched.blockSize;
+  const int64_t bs = row_block(c);
   const int64_t tiles = std::max<int64_t>(1, (c->n + bs - 1) / bs);
   const int64_t g = row_grid(c);
   const int64_t rows = (tiles + g - 1) / g * bs;  // rows per workgroup
@@ -384,15 +413,15 @@ std::pair<hipEvent_t, hipEvent_t> timer_slot(hipfoam_context* c) {
 // one Krylov iteration, enqueued on the context stream
 // -------------------------------------------------------------------------
 
-// Second stages merged into the next kernel (Merge) need 256-thread
-// workgroups: the fold then has k_finalize's tree, so merged and separate
-// stages give the same bits.  Every workgroup of the merging kernel reads all
+// Second stages merged into the next kernel (Merge) need workgroups of at
+// least 256 threads: the fold always has k_finalize's 256-thread tree, so
+// merged and separate stages give the same bits.  Every workgroup of the
+// merging kernel reads all
 // partials, so merging pays only on small grids: 10k cells (40 workgroups)
 // 2.57 -> 2.44 ms; 128^3 (2048) unchanged; 96^3 BiCGStab (1728, three
 // values per partial) 3.19 -> 3.49 ms.
-constexpr int kMergeMaxBlocks = 512;
 inline bool merge_stages(const hipfoam_context* c) {
-  return c->sched.blockSize == 256 && row_grid(c) <= kMergeMaxBlocks;
+  return row_block(c) >= kFoldThreads && row_grid(c) <= kMergeMaxBlocks;
 }
 
 // One PCG iteration.  With Chebyshev (order > 1):
@@ -410,7 +439,7 @@ template <typename T>
 void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order,
                       int parity, bool capturing) {
   Work<T>& w = work<T>(c);
-  const int bs = c->sched.blockSize;
+  const int bs = row_block(c);
   const dim3 grid(row_grid(c)), block(bs);
   const int nb = grid.x;
   KState* st = c->kst.p;
@@ -497,7 +526,7 @@ template <typename T>
 void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int parity,
                        bool capturing) {
   Work<T>& w = work<T>(c);
-  const int bs = c->sched.blockSize;
+  const int bs = row_block(c);
   const dim3 grid(row_grid(c)), block(bs);
   KState* st = c->kst.p;
   double* P = c->partials.p;
@@ -543,7 +572,7 @@ void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int p
 template <typename T>
 int enqueue_bicg_start(hipfoam_context* c, const LduOp<T>& A) {
   Work<T>& w = work<T>(c);
-  const dim3 grid(row_grid(c)), block(c->sched.blockSize);
+  const dim3 grid(row_grid(c)), block(row_block(c));
   hipLaunchKernelGGL((k_bicg_rho<T>), grid, block, 0, c->stream, c->n, A.xcdRemap, w.rA0.p,
                      w.r.p, c->kst.p, c->partials.p, c->pstride);
   launch_fin<1>(c, grid.x, true, kRunning, FinBicgRho<T>{c->kst.p});
@@ -569,7 +598,7 @@ int get_graph(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int
               int chunk, hipGraphExec_t* out) {
   GraphKey key{(int)kind, (int)sizeof(T), order, chunk, A.direct + 2 * A.slots + 4 * A.slotSym,
                A.xcdRemap,
-               c->sched.blockSize, row_grid(c), A.shortRows, c->gen};
+               row_block(c), row_grid(c), A.shortRows, c->gen};
   auto it = c->graphs.find(key);
   if (it != c->graphs.end()) {
     *out = it->second;
@@ -785,7 +814,7 @@ int stage_matrix(hipfoam_context* c, const double* diag, const double* upper,
 // FP32 inner operator diagonal: diag* = deflate ? diag - u : diag (+ rD, and
 // the Gershgorin bound when wantBeta).  Writes os->bail / os->beta*.
 int prepare_inner(hipfoam_context* c, int deflate, int wantBeta) {
-  const int bs = c->sched.blockSize;
+  const int bs = row_block(c);
   LduOp<double> A = op64(c);
   hipLaunchKernelGGL((k_prepare_inner<float>), dim3(row_grid(c)), dim3(bs), 0, c->stream,
                      A, c->upper64.p, c->losort.p, c->u64.p, deflate, wantBeta, c->diag32.p,
@@ -804,7 +833,7 @@ int read_ostate(hipfoam_context* c) {
 
 // initial FP64 residual + normFactor (metalFoamSolver.C:101-129): psi in psiCur.
 int initial_residual(hipfoam_context* c, const double* psiCur, const double* src) {
-  const int bs = c->sched.blockSize;
+  const int bs = row_block(c);
   LduOp<double> A = op64(c);
   const dim3 g(row_grid(c)), bl(bs);
   hipLaunchKernelGGL(k_rowsum, g, bl, 0, c->stream, A, psiCur, c->u64.p, c->ost.p, c->partials.p,
@@ -906,7 +935,7 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
     int totalIter = 0, outer = 0;
     bool singular = false;
     double prevRes = pf.initialResidual;
-    const int bs = c->sched.blockSize;
+    const int bs = row_block(c);
     const dim3 g(row_grid(c)), bl(bs);
     for (outer = 0; outer <= ctl->maxRestarts; ++outer) {
       hipfoam_stats st{};
@@ -1015,7 +1044,7 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
   double* result = c->psi0.p;
   if (ctl->minIter > 0 ||
       !of_converged_h(pf.finalResidual, pf.initialResidual, ctl->tolerance, ctl->relTol)) {
-    const int bs = c->sched.blockSize;
+    const int bs = row_block(c);
     const dim3 g1(blocks_for(n, 256)), b1(256);
     // Gershgorin bound of the exact operator for Chebyshev
     double beta = 2.0;
@@ -1088,7 +1117,7 @@ int amul_impl(hipfoam_context* c, int precision, const void* x, void* y, bool de
   int rc = check_ready(c);
   if (rc) return rc;
   if (!x || !y) return fail(c, HIPFOAM_ERR_ARG, "NULL x/y");
-  const int n = c->n, bs = c->sched.blockSize;
+  const int n = c->n, bs = row_block(c);
   const dim3 g(row_grid(c)), bl(bs);
   if (precision == HIPFOAM_FP32) {
     if (!c->have32) return fail(c, HIPFOAM_ERR_STATE, "no FP32 operator staged");
@@ -1707,8 +1736,8 @@ int32_t hipfoam_set_schedule(hipfoam_context* c, const hipfoam_schedule* s) {
   if (!c || !s) return HIPFOAM_ERR_ARG;
   if (s->spmvSchedule < 0 || s->spmvSchedule > 2)
     return fail(c, HIPFOAM_ERR_ARG, "spmvSchedule must be 0, 1 or 2");
-  if (s->blockSize < 64 || s->blockSize > 1024 || s->blockSize % 64)
-    return fail(c, HIPFOAM_ERR_ARG, "blockSize must be a multiple of 64 in [64, 1024]");
+  if (s->blockSize != 0 && (s->blockSize < 64 || s->blockSize > 1024 || s->blockSize % 64))
+    return fail(c, HIPFOAM_ERR_ARG, "blockSize must be 0 (auto) or a multiple of 64 in [64, 1024]");
   if (s->graphChunk < 0 || s->graphChunk > 4096 || (s->graphChunk & 1))
     return fail(c, HIPFOAM_ERR_ARG, "graphChunk must be even in [0, 4096]");
   if (s->gridLimit < 0) return fail(c, HIPFOAM_ERR_ARG, "gridLimit must be >= 0");
@@ -1742,7 +1771,7 @@ int32_t hipfoam_autotune(hipfoam_context* c, hipfoam_schedule* best) {
   const int cus = std::max(c->numCUs, 1);
   for (int form = 0; form <= 1; ++form)
   for (int sch = 0; sch <= (c->slotOK ? 2 : 1); ++sch)
-    for (int bs : {128, 256, 512, 1024})
+    for (int bs : {0, 128, 256, 512, 1024})
       for (int rm = 0; rm <= 1; ++rm)
         for (int gl : {0, 4 * cus, 16 * cus, 1 << 30}) {
         c->sched.rowForm = form;
@@ -1751,7 +1780,7 @@ int32_t hipfoam_autotune(hipfoam_context* c, hipfoam_schedule* best) {
         c->sched.xcdRemap = rm;
         c->sched.gridLimit = gl;
         const LduOp<float> A = op32(c);
-        const dim3 g(row_grid(c)), bl(bs);
+        const dim3 g(row_grid(c)), bl(row_block(c));
         auto spmv = [&]() {
           hipLaunchKernelGGL((k_spmv<float>), g, bl, 0, c->stream, A, c->w32.z0.p, c->w32.z1.p);
         };

@@ -188,44 +188,52 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* partials
   }
 }
 
-// Fold partials[j*pstride + 0 .. nblk) in a fixed order (deterministic).
+// Fold partials[j*pstride + 0 .. nblk) in a fixed order (deterministic):
+// always the tree of 256 threads, whatever the workgroup size (at least 256;
+// the other threads only join the barrier), so a stage merged into a row
+// kernel of any workgroup size folds exactly as k_finalize does.
+constexpr int kFoldThreads = 256;
+
 template <int NV, bool LASTMAX = false>
 __device__ __forceinline__ void fold_partials(const double* partials, int pstride, int nblk,
                                               double (&out)[NV]) {
-  __shared__ double sh[NV][16];
+  __shared__ double sh[NV][kFoldThreads / 64];
   double v[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) v[j] = (LASTMAX && j == NV - 1) ? -1.0e308 : 0.0;
   constexpr int U = 8;  // independent loads in flight per thread
-  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * blockDim.x) {
-    double t[U][NV];
+  const int tid = threadIdx.x;
+  if (tid < kFoldThreads) {
+    for (int b0 = tid; b0 < nblk; b0 += U * kFoldThreads) {
+      double t[U][NV];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int b = b0 + u * blockDim.x;
+      for (int u = 0; u < U; ++u) {
+        const int b = b0 + u * kFoldThreads;
 #pragma unroll
-      for (int j = 0; j < NV; ++j)
-        t[u][j] = b < nblk ? partials[j * pstride + b] : ((LASTMAX && j == NV - 1) ? -1.0e308 : 0.0);
+        for (int j = 0; j < NV; ++j)
+          t[u][j] = b < nblk ? partials[j * pstride + b]
+                             : ((LASTMAX && j == NV - 1) ? -1.0e308 : 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+          v[j] = (LASTMAX && j == NV - 1) ? fmax(v[j], t[u][j]) : v[j] + t[u][j];
     }
+    const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int j = 0; j < NV; ++j)
-        v[j] = (LASTMAX && j == NV - 1) ? fmax(v[j], t[u][j]) : v[j] + t[u][j];
-  }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    const bool mx = LASTMAX && j == NV - 1;
-    const double t = mx ? wave_max(v[j]) : wave_sum(v[j]);
-    if (lane == 0) sh[j][wid] = t;
+    for (int j = 0; j < NV; ++j) {
+      const bool mx = LASTMAX && j == NV - 1;
+      const double t = mx ? wave_max(v[j]) : wave_sum(v[j]);
+      if (lane == 0) sh[j][wid] = t;
+    }
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     const bool mx = LASTMAX && j == NV - 1;
     double t = sh[j][0];
-    for (int w = 1; w < nw; ++w) t = mx ? fmax(t, sh[j][w]) : t + sh[j][w];
+    for (int w = 1; w < kFoldThreads / 64; ++w) t = mx ? fmax(t, sh[j][w]) : t + sh[j][w];
     out[j] = t;
   }
 }
@@ -253,8 +261,8 @@ struct Merge {
 };
 
 // Every workgroup reads the Krylov state stIn and the partials M.pin, folds
-// them exactly as k_finalize does (256-thread workgroups give the same fold
-// tree) and applies the finaliser f, when the state is running, to its own
+// them exactly as k_finalize does (fold_partials' 256-thread tree, in a
+// workgroup of at least 256 threads) and applies the finaliser f, when the state is running, to its own
 // LDS copy ks; workgroup 0 writes that copy to M.stOut.  No workgroup writes
 // memory another one reads (stIn, M.pin and this kernel's partials are three
 // different buffers), so the workgroups need no ordering.  Returns the state

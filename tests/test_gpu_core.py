@@ -206,6 +206,40 @@ def test_refined_solve_matches_oracle(ctx, orc, case):
     assert err < max(10 * err_o, 1e-5)
 
 
+@pytest.mark.parametrize("kind", ["pcg", "bicg", "pcg_fp64", "bicg_fp64"])
+def test_launch_shapes(ctx, kind):
+    """768 256-row tiles: the automatic launch shape runs 384 workgroups of 512
+    threads with every second stage merged into a row kernel, and an explicit
+    512-thread schedule with the same grid gives the same bits.  Other grids
+    group the reductions differently: 256-thread workgroups without a limit
+    (768, separate finalise kernels), 1024-thread ones (192, merged), a
+    512-workgroup limit (merged).  Their FP64 sums may round differently in
+    the last bits, so they agree to 1e-9 with the same iteration count."""
+    bicg = kind.startswith("bicg")
+    fp64 = kind.endswith("fp64")
+    m = rand_asym((64, 64, 48), 11) if bicg else mg.poisson(64, 64, 48, seed=11, lognormal=True)
+    ctx.set_topology(m)
+    lower = m.lower if bicg else None
+    solver = H.PBICGSTAB if bicg else H.PCG
+    outs = {}
+    for bs, gl in ((0, 0), (512, 0), (256, 0), (1024, 0), (256, 512)):
+        ctx.set_schedule(blockSize=bs, gridLimit=gl)
+        psi = np.zeros(m.n)
+        perf = ctx.solve(solver, m.diag, m.upper, lower, psi, m.source, fp64=fp64)
+        assert perf.converged
+        outs[(bs, gl)] = (psi, perf.nIterations, perf.finalResidual)
+    ref, it0, res0 = outs[(0, 0)]
+    psi, it, res = outs[(512, 0)]
+    np.testing.assert_array_equal(psi, ref)
+    assert it == it0 and res == res0
+    for key in ((256, 0), (1024, 0), (256, 512)):
+        psi, it, res = outs[key]
+        np.testing.assert_allclose(psi, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+        assert it == it0
+        assert res == pytest.approx(res0, rel=1e-6)
+    ctx.set_schedule(blockSize=256, gridLimit=0)
+
+
 def test_solve_device_entry_and_graph_modes_agree_bitwise(ctx):
     m = mg.poisson(24, 20, 16, seed=7, lognormal=True)
     ctx.set_topology(m)
@@ -318,7 +352,7 @@ def test_autotune_picks_a_valid_schedule(ctx):
     ctx.set_topology(m)
     ctx.set_matrix(m.diag, m.upper, None)
     s = ctx.autotune()
-    assert s.spmvSchedule in (0, 1, 2) and s.blockSize in (128, 256, 512, 1024)
+    assert s.spmvSchedule in (0, 1, 2) and s.blockSize in (0, 128, 256, 512, 1024)
     psi = np.zeros(m.n)
     perf = ctx.solve(H.PCG, m.diag, m.upper, None, psi, m.source)
     assert perf.converged
