@@ -223,6 +223,21 @@ inline int row_grid(const hipfoam_context* c) {
   return std::min(tiles, cap);
 }
 
+// Launch shape of the row kernels that reduce nothing (the interior
+// Chebyshev steps, the plain SpMV): no partials to fold, so the automatic
+// schedule keeps them at full occupancy, 256-thread workgroups, 8 per CU.
+// Their results do not depend on the launch shape.
+inline int stream_block(const hipfoam_context* c) {
+  return c->sched.blockSize > 0 ? c->sched.blockSize : 256;
+}
+
+inline int stream_grid(const hipfoam_context* c) {
+  const int bs = stream_block(c);
+  const int tiles = std::max(1, (c->n + bs - 1) / bs);
+  const int cap = c->sched.gridLimit > 0 ? c->sched.gridLimit : 8 * std::max(c->numCUs, 1);
+  return std::min(tiles, cap);
+}
+
 // Block-to-XCD mapping of the row kernels.  Hardware dispatch places
 // workgroup b on XCD b mod 8; xcdRemap 1 gives each XCD one contiguous row
 // range instead.  Auto (2): keep the hardware order when it already keeps
@@ -231,11 +246,11 @@ inline int row_grid(const hipfoam_context* c) {
 // workgroup's rows (128^3: strides 16384 and 128 at 1024 rows per
 // workgroup); else contiguous ranges.  The mapping moves no arithmetic:
 // partials are indexed by logical block.
-inline int effective_remap(const hipfoam_context* c) {
+inline int effective_remap(const hipfoam_context* c, bool stream = false) {
   if (c->sched.xcdRemap != 2) return c->sched.xcdRemap;
-  const int64_t bs = row_block(c);
+  const int64_t bs = stream ? stream_block(c) : row_block(c);
   const int64_t tiles = std::max<int64_t>(1, (c->n + bs - 1) / bs);
-  const int64_t g = row_grid(c);
+  const int64_t g = stream ? stream_grid(c) : row_grid(c);
   const int64_t rows = (tiles + g - 1) / g * bs;  // rows per workgroup
   const bool local = c->strideMajor > 0 && c->strideMajor % (8 * rows) == 0 &&
                      c->strideMinor * 8 <= rows;
@@ -327,6 +342,13 @@ LduOp<T> make_op(hipfoam_context* c, const T* diag, const T* upper, const T* low
     op.lowerL = lowerL;
   }
   return op;
+}
+
+// The operator with the block-to-XCD mapping of the stream launch shape.
+template <typename T>
+LduOp<T> stream_op(LduOp<T> A, const hipfoam_context* c) {
+  A.xcdRemap = effective_remap(c, true);
+  return A;
 }
 
 LduOp<float> op32(hipfoam_context* c) {
@@ -475,6 +497,9 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
                        st, P, ps, none, FinAlpha<T>{nullptr});
     }
     const T* zin = zout;
+    // the interior steps reduce nothing: the stream launch shape
+    const LduOp<T> As = stream_op(A, c);
+    const dim3 sgrid(stream_grid(c)), sblock(stream_block(c));
     for (int s = 2; s < order; ++s) {
       const bool last = (s == order - 1);
       T* zo = last ? w.wA.p : ((s & 1) ? w.z0.p : w.z1.p);
@@ -482,8 +507,8 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
         LAUNCH_CLASSED(4, capturing, (k_cheb_step<T, false, true>), grid, block, A, rout, rD,
                        zin, w.w.p, zo, st, P, ps, s);
       else
-        LAUNCH_CLASSED(0, capturing, (k_cheb_step<T, false, false>), grid, block, A, rout, rD,
-                       zin, w.w.p, zo, st, P, ps, s);
+        LAUNCH_CLASSED(0, capturing, (k_cheb_step<T, false, false>), sgrid, sblock, As, rout,
+                       rD, zin, w.w.p, zo, st, P, ps, s);
       zin = zo;
     }
     if (merge) {
@@ -1117,8 +1142,8 @@ int amul_impl(hipfoam_context* c, int precision, const void* x, void* y, bool de
   int rc = check_ready(c);
   if (rc) return rc;
   if (!x || !y) return fail(c, HIPFOAM_ERR_ARG, "NULL x/y");
-  const int n = c->n, bs = row_block(c);
-  const dim3 g(row_grid(c)), bl(bs);
+  const int n = c->n, bs = stream_block(c);
+  const dim3 g(stream_grid(c)), bl(bs);
   if (precision == HIPFOAM_FP32) {
     if (!c->have32) return fail(c, HIPFOAM_ERR_STATE, "no FP32 operator staged");
     const float* xd = static_cast<const float*>(x);
@@ -1129,7 +1154,7 @@ int amul_impl(hipfoam_context* c, int precision, const void* x, void* y, bool de
       xd = c->w32.z0.p;
       yd = c->w32.z1.p;
     }
-    LAUNCH_CLASSED(5, false, (k_spmv<float>), g, bl, op32(c), xd, yd);
+    LAUNCH_CLASSED(5, false, (k_spmv<float>), g, bl, stream_op(op32(c), c), xd, yd);
     CKL();
     if (!device)
       CK(hipMemcpyAsync(y, yd, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
@@ -1144,7 +1169,7 @@ int amul_impl(hipfoam_context* c, int precision, const void* x, void* y, bool de
       xd = c->psi1.p;
       yd = c->scratch64.p;
     }
-    LAUNCH_CLASSED(5, false, (k_spmv<double>), g, bl, op64(c), xd, yd);
+    LAUNCH_CLASSED(5, false, (k_spmv<double>), g, bl, stream_op(op64(c), c), xd, yd);
     CKL();
     if (!device)
       CK(hipMemcpyAsync(y, yd, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
