@@ -27,6 +27,7 @@
 #include "hipfoam_batched.hpp"
 #include "hipfoam_device.hpp"
 #include "hipfoam_slots.hpp"
+#include "hipfoam_stage.hpp"
 
 using namespace hipfoam;
 
@@ -180,6 +181,8 @@ struct hipfoam_context {
   hipfoam_schedule sched{2, 0, 2, 4, 0, 0};
   std::map<GraphKey, hipGraphExec_t> graphs;
   KernelTimer timer;
+  // pinned staging of large host arrays (created on first use)
+  std::unique_ptr<HostStager> stager;
 };
 
 namespace {
@@ -298,6 +301,37 @@ void launch_fin(hipfoam_context* c, int nblk, bool checkStatus, int runMask, F f
       c->err = std::string("kernel launch: ") + hipGetErrorString(e_);        \
       return HIPFOAM_ERR_DEVICE;                                              \
     }                                                                         \
+  } while (0)
+
+// Copy of `bytes` on the context stream.  Host <-> device copies of at least
+// HostStager::kMin bytes go through the pinned staging ring: host to device
+// returns once the host array has been read, device to host once the host
+// array holds the data (after the work already on the stream).  Other copies
+// are hipMemcpyAsync.
+int xfer(hipfoam_context* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  const bool staged = (kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToHost) &&
+                      bytes >= HostStager::kMin;
+  if (!staged) {
+    CK(hipMemcpyAsync(dst, src, bytes, kind, c->stream));
+    return HIPFOAM_OK;
+  }
+  if (!c->stager) {
+    auto st = std::make_unique<HostStager>();
+    const int hw = (int)std::thread::hardware_concurrency();
+    CK(st->init(std::max(0, std::min(7, hw - 1))));
+    c->stager = std::move(st);
+  }
+  if (kind == hipMemcpyHostToDevice)
+    CK(c->stager->h2d(dst, src, bytes, c->stream));
+  else
+    CK(c->stager->d2h(dst, src, bytes, c->stream));
+  return HIPFOAM_OK;
+}
+
+#define XF(dst, src, bytes, kind)                    \
+  do {                                               \
+    const int rc_ = xfer(c, (dst), (src), (bytes), (kind)); \
+    if (rc_) return rc_;                             \
   } while (0)
 
 int fail(hipfoam_context* c, int code, const std::string& msg) {
@@ -802,14 +836,14 @@ int stage_matrix(hipfoam_context* c, const double* diag, const double* upper,
   CK(c->lowerL64.ensure(nf));
   CK(c->upper32.ensure(nf));
   CK(c->lowerL32.ensure(nf));
-  CK(hipMemcpyAsync(c->diag64.p, diag, sizeof(double) * n, kind, c->stream));
-  CK(hipMemcpyAsync(c->upper64.p, upper, sizeof(double) * nf, kind, c->stream));
+  XF(c->diag64.p, diag, sizeof(double) * n, kind);
+  XF(c->upper64.p, upper, sizeof(double) * nf, kind);
   c->sym64 = (lower == nullptr);
   c->sym32 = c->sym64;
   if (lower) {
     CK(c->lower64.ensure(nf));
     CK(c->lower32.ensure(nf));
-    CK(hipMemcpyAsync(c->lower64.p, lower, sizeof(double) * nf, kind, c->stream));
+    XF(c->lower64.p, lower, sizeof(double) * nf, kind);
   }
   if (nf > 0) {
     hipLaunchKernelGGL(k_stage_faces, dim3(blocks_for(nf, 256)), dim3(256), 0, c->stream, nf,
@@ -908,8 +942,8 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
   rc = stage_matrix(c, diag, upper, lower, kind);
   if (rc) return rc;
   const double tStage = now_seconds() - t0;
-  CK(hipMemcpyAsync(c->psi0.p, psi, sizeof(double) * n, kind, c->stream));
-  CK(hipMemcpyAsync(c->b.p, source, sizeof(double) * n, kind, c->stream));
+  XF(c->psi0.p, psi, sizeof(double) * n, kind);
+  XF(c->b.p, source, sizeof(double) * n, kind);
   double* psiCur = c->psi0.p;
   double* psiNxt = c->psi1.p;
   rc = initial_residual(c, psiCur, c->b.p);
@@ -1027,8 +1061,8 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
       of_converged_h(pf.finalResidual, pf.initialResidual, ctl->tolerance, ctl->relTol);
   const hipMemcpyKind back =
       kind == hipMemcpyHostToDevice ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-  CK(hipMemcpyAsync(psi, psiCur, sizeof(double) * n, back, c->stream));
-  if (residual) CK(hipMemcpyAsync(residual, c->r64.p, sizeof(double) * n, back, c->stream));
+  XF(psi, psiCur, sizeof(double) * n, back);
+  if (residual) XF(residual, c->r64.p, sizeof(double) * n, back);
   CK(hipStreamSynchronize(c->stream));
   pf.tTotal = now_seconds() - t0;
   if (perf) *perf = pf;
@@ -1054,8 +1088,8 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
   CK(c->w64.ensure(n));
   rc = stage_matrix(c, diag, upper, useBiCG ? lower : nullptr, kind);
   if (rc) return rc;
-  CK(hipMemcpyAsync(c->psi0.p, psi, sizeof(double) * n, kind, c->stream));
-  CK(hipMemcpyAsync(c->b.p, source, sizeof(double) * n, kind, c->stream));
+  XF(c->psi0.p, psi, sizeof(double) * n, kind);
+  XF(c->b.p, source, sizeof(double) * n, kind);
   rc = initial_residual(c, c->psi0.p, c->b.p);
   if (rc) return rc;
   const OState& o = *c->ho;
@@ -1124,13 +1158,13 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
       of_converged_h(pf.finalResidual, pf.initialResidual, ctl->tolerance, ctl->relTol);
   const hipMemcpyKind back =
       kind == hipMemcpyHostToDevice ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-  CK(hipMemcpyAsync(psi, result, sizeof(double) * n, back, c->stream));
+  XF(psi, result, sizeof(double) * n, back);
   if (residual) {
     // Chebyshev PCG: the residual of the last update went to the buffer of
     // parity it (k_cheb_first of iteration it wrote it)
     const double* rlast = (!useBiCG && order > 1 && (c->hk->it & 1) == 0) ? w.r1.p : w.r.p;
     const double* rr = (result == c->psi0.p) ? c->r64.p : rlast;
-    CK(hipMemcpyAsync(residual, rr, sizeof(double) * n, back, c->stream));
+    XF(residual, rr, sizeof(double) * n, back);
   }
   CK(hipStreamSynchronize(c->stream));
   pf.tTotal = now_seconds() - t0;
@@ -1150,14 +1184,14 @@ int amul_impl(hipfoam_context* c, int precision, const void* x, void* y, bool de
     float* yd = static_cast<float*>(y);
     if (!device) {
       CK(c->w32.ensure(n));
-      CK(hipMemcpyAsync(c->w32.z0.p, x, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+      XF(c->w32.z0.p, x, sizeof(float) * n, hipMemcpyHostToDevice);
       xd = c->w32.z0.p;
       yd = c->w32.z1.p;
     }
     LAUNCH_CLASSED(5, false, (k_spmv<float>), g, bl, stream_op(op32(c), c), xd, yd);
     CKL();
     if (!device)
-      CK(hipMemcpyAsync(y, yd, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+      XF(y, yd, sizeof(float) * n, hipMemcpyDeviceToHost);
   } else if (precision == HIPFOAM_FP64) {
     if (!c->have64) return fail(c, HIPFOAM_ERR_STATE, "no FP64 matrix staged");
     const double* xd = static_cast<const double*>(x);
@@ -1165,14 +1199,14 @@ int amul_impl(hipfoam_context* c, int precision, const void* x, void* y, bool de
     if (!device) {
       CK(c->psi1.ensure(n));
       CK(c->scratch64.ensure(n));
-      CK(hipMemcpyAsync(c->psi1.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+      XF(c->psi1.p, x, sizeof(double) * n, hipMemcpyHostToDevice);
       xd = c->psi1.p;
       yd = c->scratch64.p;
     }
     LAUNCH_CLASSED(5, false, (k_spmv<double>), g, bl, stream_op(op64(c), c), xd, yd);
     CKL();
     if (!device)
-      CK(hipMemcpyAsync(y, yd, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+      XF(y, yd, sizeof(double) * n, hipMemcpyDeviceToHost);
   } else {
     return fail(c, HIPFOAM_ERR_ARG, "precision must be HIPFOAM_FP32 or HIPFOAM_FP64");
   }
@@ -1253,28 +1287,28 @@ int adjoint_host(hipfoam_context* c, int solver, int precision, const double* di
   rc = ensure_vectors(c);
   if (rc) return rc;
   CK(c->diag64.ensure(n));
-  CK(hipMemcpyAsync(c->adjX.p, x, sizeof(double) * n, h2d, c->stream));
-  CK(hipMemcpyAsync(c->adjG.p, g, sizeof(double) * n, h2d, c->stream));
+  XF(c->adjX.p, x, sizeof(double) * n, h2d);
+  XF(c->adjG.p, g, sizeof(double) * n, h2d);
   // device copies of the coefficients (stage_matrix inside the solve copies again)
   Buf<double> d, u, l;
   CK(d.ensure(n));
   CK(u.ensure(nf));
   if (lower) CK(l.ensure(nf));
-  CK(hipMemcpyAsync(d.p, diag, sizeof(double) * n, h2d, c->stream));
-  CK(hipMemcpyAsync(u.p, upper, sizeof(double) * nf, h2d, c->stream));
-  if (lower) CK(hipMemcpyAsync(l.p, lower, sizeof(double) * nf, h2d, c->stream));
+  XF(d.p, diag, sizeof(double) * n, h2d);
+  XF(u.p, upper, sizeof(double) * nf, h2d);
+  if (lower) XF(l.p, lower, sizeof(double) * nf, h2d);
   rc = adjoint_device(c, solver, precision, d.p, u.p, lower ? l.p : nullptr, c->adjX.p,
                       c->adjG.p,
                       c->adjLam.p, dUpper ? c->adjDU.p : nullptr,
                       (dLower && lower) ? c->adjDL.p : nullptr, dDiag ? c->adjDD.p : nullptr,
                       dSource ? c->adjDB.p : nullptr, ctl, perf);
   if (rc == HIPFOAM_OK) {
-    CK(hipMemcpyAsync(lambda, c->adjLam.p, sizeof(double) * n, d2h, c->stream));
-    if (dUpper) CK(hipMemcpyAsync(dUpper, c->adjDU.p, sizeof(double) * nf, d2h, c->stream));
+    XF(lambda, c->adjLam.p, sizeof(double) * n, d2h);
+    if (dUpper) XF(dUpper, c->adjDU.p, sizeof(double) * nf, d2h);
     if (dLower && lower)
-      CK(hipMemcpyAsync(dLower, c->adjDL.p, sizeof(double) * nf, d2h, c->stream));
-    if (dDiag) CK(hipMemcpyAsync(dDiag, c->adjDD.p, sizeof(double) * n, d2h, c->stream));
-    if (dSource) CK(hipMemcpyAsync(dSource, c->adjDB.p, sizeof(double) * n, d2h, c->stream));
+      XF(dLower, c->adjDL.p, sizeof(double) * nf, d2h);
+    if (dDiag) XF(dDiag, c->adjDD.p, sizeof(double) * n, d2h);
+    if (dSource) XF(dSource, c->adjDB.p, sizeof(double) * n, d2h);
   }
   CK(hipStreamSynchronize(c->stream));
   d.release();
@@ -1381,6 +1415,7 @@ void hipfoam_destroy(hipfoam_context* c) {
   if (c->hstatus) (void)hipHostFree(c->hstatus);
   for (hipEvent_t e : {c->evPoll[0], c->evPoll[1], c->evT0, c->evT1})
     if (e) (void)hipEventDestroy(e);
+  c->stager.reset();  // waits for its DMAs, frees the pinned chunks, joins its threads
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1505,7 +1540,7 @@ static int32_t inner_entry(hipfoam_context* c, Kind kind, const double* r, doubl
   if (!c->have32) return fail(c, HIPFOAM_ERR_STATE, "set_coeffs not called");
   if (order < 1 || order > kMaxCheb + 1)
     return fail(c, HIPFOAM_ERR_ARG, "chebOrder must be in [1, 33]");
-  CK(hipMemcpyAsync(c->r64.p, r, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+  XF(c->r64.p, r, sizeof(double) * c->n, hipMemcpyHostToDevice);
   rc = inner_solve(c, kind, c->r64.p, nullptr, nullptr, target, maxIter, ips, order, beta, ratio,
                    stats);
   if (rc) return rc;
@@ -1533,8 +1568,7 @@ int32_t hipfoam_delta(hipfoam_context* c, float* out) {
   if (rc) return rc;
   if (!out) return fail(c, HIPFOAM_ERR_ARG, "NULL out");
   if (!c->w32.delta.p) return fail(c, HIPFOAM_ERR_STATE, "no inner solve yet");
-  CK(hipMemcpyAsync(out, c->w32.delta.p, sizeof(float) * c->n, hipMemcpyDeviceToHost,
-                    c->stream));
+  XF(out, c->w32.delta.p, sizeof(float) * c->n, hipMemcpyDeviceToHost);
   CK(hipStreamSynchronize(c->stream));
   return HIPFOAM_OK;
 }
@@ -1660,8 +1694,8 @@ int32_t hipfoam_batch_get_psi(hipfoam_batch* bt, int32_t system, double* psi) {
   const int n = bt->sysRow0[system + 1] - bt->sysRow0[system];
   const double* psiCur = bt->psiIs0 ? bt->psi0.p : bt->psi1.p;
   for (int r = 0; r < bt->nrhs; ++r)
-    CK(hipMemcpyAsync(psi + (long)r * n, psiCur + (long)r * bt->N + bt->sysRow0[system],
-                      sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    XF(psi + (long)r * n, psiCur + (long)r * bt->N + bt->sysRow0[system],
+       sizeof(double) * n, hipMemcpyDeviceToHost);
   CK(hipStreamSynchronize(c->stream));
   return HIPFOAM_OK;
 }
@@ -1704,7 +1738,7 @@ int32_t hipfoam_copy_to_device(hipfoam_context* c, void* dst, const void* src, u
   FpTrapGuard fpe;
   if (!c || (!dst && bytes) || (!src && bytes)) return HIPFOAM_ERR_ARG;
   CK(hipSetDevice(c->device));
-  CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  XF(dst, src, bytes, hipMemcpyHostToDevice);
   CK(hipStreamSynchronize(c->stream));
   return HIPFOAM_OK;
 }
@@ -1713,7 +1747,7 @@ int32_t hipfoam_copy_to_host(hipfoam_context* c, void* dst, const void* src, uin
   FpTrapGuard fpe;
   if (!c || (!dst && bytes) || (!src && bytes)) return HIPFOAM_ERR_ARG;
   CK(hipSetDevice(c->device));
-  CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  XF(dst, src, bytes, hipMemcpyDeviceToHost);
   CK(hipStreamSynchronize(c->stream));
   return HIPFOAM_OK;
 }
