@@ -115,7 +115,11 @@ const char* hipfoam_last_error(const hipfoam_context* ctx);
 /* ---- backend (inner) interface: metalFoamBackend -------------------- */
 
 /* metalFoamBackend::setTopology (B.C:454-519).  Uploads int32 addressing
- * once; a no-op when (nCells, nFaces, lowAddr, uppAddr) are unchanged. */
+ * once; a no-op when (nCells, nFaces, lowAddr, uppAddr) are unchanged, as in
+ * the reference, and a fingerprint of the addressing content (1024 evenly
+ * spaced entries of each array, the first and last included) is too:
+ * addressing rewritten in place at the same addresses is uploaded again
+ * when the rewrite touches a sampled entry. */
 int32_t hipfoam_set_topology(hipfoam_context* ctx, int32_t nCells,
                              int32_t nFaces, const int32_t* lowAddr,
                              const int32_t* uppAddr, const int32_t* ownStart,

@@ -148,6 +148,7 @@ struct hipfoam_context {
   Buf<double> slotCoef64;
   const int32_t* lastLow = nullptr;
   const int32_t* lastUpp = nullptr;
+  uint64_t topoPrint = 0;  // addressing_fingerprint of the uploaded mesh
   int gen = 0;  // bumps whenever device buffers are reallocated
   Buf<int32_t> ownStart, losortStart, uppAddr, lowAddr, losort, losortLow;
 
@@ -1430,6 +1431,35 @@ const char* hipfoam_last_error(const hipfoam_context* c) {
   return c ? c->err.c_str() : "NULL context";
 }
 
+// Fingerprint of a mesh's addressing: the sizes and 1024 evenly spaced
+// entries (the first and last included) of each addressing array, mixed
+// with splitmix64.  A few thousand loads, so it is taken on every call.
+uint64_t addressing_fingerprint(int32_t n, int32_t nf, const int32_t* low, const int32_t* upp,
+                                const int32_t* ownStart, const int32_t* losort,
+                                const int32_t* losortStart) {
+  auto mix = [](uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+  };
+  uint64_t h = mix(((uint64_t)(uint32_t)n << 32) | (uint32_t)nf);
+  auto add = [&](const int32_t* a, int64_t len) {
+    if (!a || len <= 0) return;
+    const int64_t k = std::min<int64_t>(len, 1024);
+    for (int64_t j = 0; j < k; ++j) {
+      const int64_t i = k > 1 ? j * (len - 1) / (k - 1) : 0;
+      h = mix(h ^ (((uint64_t)i << 32) | (uint32_t)a[i]));
+    }
+  };
+  add(low, nf);
+  add(upp, nf);
+  add(losort, nf);
+  add(ownStart, (int64_t)n + 1);
+  add(losortStart, (int64_t)n + 1);
+  return h;
+}
+
 int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
                              const int32_t* lowAddr, const int32_t* uppAddr,
                              const int32_t* ownStart, const int32_t* losort,
@@ -1442,9 +1472,13 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
     return fail(c, HIPFOAM_ERR_ARG, "NULL addressing");
   if (!ownStart || !losortStart) return fail(c, HIPFOAM_ERR_ARG, "NULL start arrays");
   CK(hipSetDevice(c->device));
-  // no-op when unchanged (metalFoamBackend.C:469-477)
+  // no-op when unchanged: the reference keys on the sizes and the addressing
+  // pointers (metalFoamBackend.C:469-477); the content fingerprint also
+  // catches addressing rewritten in place at the same addresses
+  const uint64_t print = addressing_fingerprint(nCells, nFaces, lowAddr, uppAddr, ownStart,
+                                                losort, losortStart);
   if (c->ownStart.p && c->n == nCells && c->nf == nFaces && c->lastLow == lowAddr &&
-      c->lastUpp == uppAddr)
+      c->lastUpp == uppAddr && c->topoPrint == print)
     return HIPFOAM_OK;
   CK(hipStreamSynchronize(c->stream));
   clear_graphs(c);
@@ -1461,8 +1495,8 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
   c->slotPatterned = c->slotOK ? lay.patternedRows : 0;
   c->strideMajor = c->slotOK ? lay.strideMajor : 0;
   c->strideMinor = c->slotOK ? lay.strideMinor : 0;
-  c->lastLow = lowAddr;
-  c->lastUpp = uppAddr;
+  // the cache key is set once the upload has succeeded
+  c->lastLow = c->lastUpp = nullptr;
   c->have32 = c->have64 = false;
   const size_t nf = (size_t)nFaces, n1 = (size_t)nCells + 1;
   CK(c->ownStart.ensure(n1));
@@ -1507,6 +1541,9 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
   int rc = ensure_vectors(c);
   if (rc) return rc;
   CK(hipStreamSynchronize(c->stream));
+  c->lastLow = lowAddr;
+  c->lastUpp = uppAddr;
+  c->topoPrint = print;
   return HIPFOAM_OK;
 }
 

@@ -153,3 +153,29 @@ def test_host_array_solve_matches_device_solve(ctx):
     assert perf_h.nIterations == perf_d.nIterations
     for a in (dd, du, ds, dp, dr):
         a.free()
+
+
+def test_topology_rewritten_in_place_is_uploaded_again(ctx):
+    """The topology cache keys on the sizes, the addressing pointers and a
+    fingerprint of the addressing content.  Another mesh of the same sizes
+    written into the same buffers is uploaded again: the solve then equals a
+    solve on that mesh uploaded from its own arrays."""
+    a = mg.poisson(8, 6, 5, seed=1, lognormal=True)
+    b = mg.poisson(6, 8, 5, seed=2, lognormal=True)
+    assert a.n == b.n and a.nfaces == b.nfaces
+    ctx.set_topology(a)
+    psi = np.zeros(a.n)
+    assert ctx.solve(H.PCG, a.diag, a.upper, None, psi, a.source).converged
+    names = ("lower_addr", "upper_addr", "owner_start", "losort", "losort_start")
+    bufs = [getattr(a, k) for k in names]
+    for buf, k in zip(bufs, names):
+        np.copyto(buf, getattr(b, k))
+    ctx.set_topology(a)  # the same buffers, now holding b's addressing
+    assert all(H._i32(getattr(a, k)) is buf for buf, k in zip(bufs, names))
+    psi = np.zeros(b.n)
+    perf = ctx.solve(H.PCG, b.diag, b.upper, None, psi, b.source)
+    ctx.set_topology(b)
+    ref = np.zeros(b.n)
+    perf_ref = ctx.solve(H.PCG, b.diag, b.upper, None, ref, b.source)
+    assert perf.converged and perf.nIterations == perf_ref.nIterations
+    np.testing.assert_array_equal(psi, ref)
