@@ -256,7 +256,7 @@ def main():
     patterned, _ = ctx.slot_layout()
     if solver == "pcg":
         ref_kbytes = cheb_step_bytes(m.n, m.nfaces, sym)
-        kname = "k_cheb_step<float, false, false> (interior Chebyshev step fused with its SpMV)"
+        kname = "k_cheb_step<float, false> (interior Chebyshev step fused with its SpMV)"
     else:
         # mean of k_bicg_py_spmv (reads r, rD, p, AyA, rA0; writes p, yA, AyA)
         # and k_bicg_sz_spmv (reads r, rD, AyA; writes sA, zA, tA)

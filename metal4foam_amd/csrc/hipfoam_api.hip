@@ -539,10 +539,10 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
       const bool last = (s == order - 1);
       T* zo = last ? w.wA.p : ((s & 1) ? w.z0.p : w.z1.p);
       if (last)
-        LAUNCH_CLASSED(4, capturing, (k_cheb_step<T, false, true>), grid, block, A, rout, rD,
+        LAUNCH_CLASSED(4, capturing, (k_cheb_step<T, true>), grid, block, A, rout, rD,
                        zin, w.w.p, zo, st, P, ps, s);
       else
-        LAUNCH_CLASSED(0, capturing, (k_cheb_step<T, false, false>), sgrid, sblock, As, rout,
+        LAUNCH_CLASSED(0, capturing, (k_cheb_step<T, false>), sgrid, sblock, As, rout,
                        rD, zin, w.w.p, zo, st, P, ps, s);
       zin = zo;
     }

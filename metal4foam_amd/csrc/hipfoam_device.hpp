@@ -609,12 +609,12 @@ __device__ __forceinline__ void finalize_beta(KState* st, double t) {
 // PCG kernels
 // ---------------------------------------------------------------------------
 
-// One Chebyshev step fused with its SpMV (cheb_init + ldu_spmv + cheb_step of
-// metalFoamKernels.metal:112-194):
+// One Chebyshev step after the first, fused with its SpMV (ldu_spmv +
+// cheb_step of metalFoamKernels.metal:112-194):
 //   v = A z_{s-1};  w_s = c2 w_{s-1} + c3 rD (r - v);  z_s = z_{s-1} + w_s
-// FIRST: z_0 = w_0 = c1 rD r is formed on the fly from gathered rD and r.
 // LAST:  z_s is the preconditioned residual wA; fuses dot(wA, rA) and beta.
-template <typename T, bool FIRST, bool LAST>
+// (The first step, z_0 = w_0 = c1 rD r, is k_cheb_first.)
+template <typename T, bool LAST>
 __global__ __launch_bounds__(1024) void k_cheb_step(LduOp<T> A, const T* __restrict__ r,
                                                     const T* __restrict__ rD,
                                                     const T* __restrict__ zin,
@@ -627,23 +627,14 @@ __global__ __launch_bounds__(1024) void k_cheb_step(LduOp<T> A, const T* __restr
   block_rows(A.n, lb, r0, r1);
   const T c2 = (T)st->chebC2[step - 1];
   const T c3 = (T)st->chebC3[step - 1];
-  const T c1 = (T)st->chebC1;
   double dot = 0.0;
   for_rows(A, r0, r1, [&](int i, auto row) {
     // the row's own operands load before the gathers, in one batch
     const T ri = r[i], rDi = rD[i];
-    T zi_new, wi;
-    if (FIRST) {
-      const T z0 = c1 * rDi * ri;
-      const T v = row(z0, [&](int j) { return c1 * rD[j] * r[j]; });
-      wi = c2 * z0 + c3 * rDi * (ri - v);
-      zi_new = z0 + wi;
-    } else {
-      const T zi = zin[i], wold = w[i];
-      const T v = row(zi, [&](int j) { return zin[j]; });
-      wi = c2 * wold + c3 * rDi * (ri - v);
-      zi_new = zi + wi;
-    }
+    const T zi = zin[i], wold = w[i];
+    const T v = row(zi, [&](int j) { return zin[j]; });
+    const T wi = c2 * wold + c3 * rDi * (ri - v);
+    const T zi_new = zi + wi;
     if (!LAST) w[i] = wi;
     zout[i] = zi_new;
     if (LAST) dot += (double)(zi_new * ri);

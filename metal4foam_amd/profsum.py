@@ -69,16 +69,15 @@ def _ldu_bytes(name: str, n: int, nf: int, asym: bool) -> int | None:
     face32 = 20 if asym else 16
     face64 = 28 if asym else 24
     table = (
-        ("k_cheb_step<float, false, false>", 36 * n + 8 + face32 * nf),
-        ("k_cheb_step<float, true, false>", 28 * n + 8 + face32 * nf),
-        ("k_cheb_step<float, false, true>", 32 * n + 8 + face32 * nf),
+        ("k_cheb_step<float, false>", 36 * n + 8 + face32 * nf),
+        ("k_cheb_step<float, true>", 32 * n + 8 + face32 * nf),
         # previous iteration's update applied (every launch but an inner
         # solve's first): reads r, Ap, delta, p, rD; writes r, delta, w, z
         ("k_cheb_first<float, false,", 48 * n + 8 + face32 * nf),
         ("k_pcg_p_spmv<float", 28 * n + 8 + face32 * nf),
         ("k_pcg_update<float", 24 * n),
         ("k_spmv<float", 20 * n + 8 + face32 * nf),
-        ("k_cheb_step<double, false, false>", 64 * n + 8 + face64 * nf),
+        ("k_cheb_step<double, false>", 64 * n + 8 + face64 * nf),
         ("k_cheb_first<double, false,", 88 * n + 8 + face64 * nf),
         ("k_pcg_p_spmv<double", 48 * n + 8 + face64 * nf),
         ("k_bicg_py_spmv<float", 44 * n + 8 + face32 * nf),
@@ -112,7 +111,7 @@ def main():
     ap.add_argument("--layout", choices=["faces", "slots"], default="faces")
     ap.add_argument("--patterned-rows", type=int, default=0)
     ap.add_argument("--traffic-json")
-    ap.add_argument("--traffic-kernel", default="k_cheb_step<float, false, false>")
+    ap.add_argument("--traffic-kernel", default="k_cheb_step<float, false>")
     ap.add_argument("--workload", default="")
     a = ap.parse_args()
     slots = a.layout == "slots"
