@@ -5,8 +5,10 @@
 // (global row and face numbering, addressing rebased once at set-up), so the
 // LduOp of the single-system path applies unchanged.  Right-hand sides are
 // stored [rhs][row] (stride N = total rows); the off-diagonal coefficients
-// are shared by every RHS of a system, the FP32 diagonal / rD are per RHS
-// (a segment may switch to the exact operator on its own).
+// are shared by every RHS of a system.  The FP32 diagonal and rD come in two
+// planes over the super-mesh, exact and deflated (diag - A 1); each segment
+// selects one (segPlane), so a segment may switch to the exact operator on
+// its own while the RHS of a system still read one diagonal.
 //
 // Launch shape.  Rows are cut into 256-row tiles that never straddle a
 // system.  The Krylov iteration kernels run one workgroup per (tile, group of
@@ -31,6 +33,7 @@ struct BatchMap {
   const int32_t* __restrict__ tileRow0;  // [ntiles + 1] first global row of each tile
   const int32_t* __restrict__ tileSys;   // [ntiles] system of each tile
   const int32_t* __restrict__ sysTile0;  // [nsys + 1] first tile of each system
+  const int32_t* __restrict__ segPlane;  // [nseg] diagonal plane: 0 exact, 1 deflated
   int32_t ntiles, nrhs, N, xcdRemap;
 };
 
@@ -51,10 +54,15 @@ __device__ __forceinline__ BlockSeg block_seg(const BatchMap& B, int lb) {
   return s;
 }
 
-// Per-segment FP32 diagonal view: the operator with diag/rD of RHS `rhs`.
+// Offset of a segment's diagonal plane in the FP32 diag / rD arrays
+__device__ __forceinline__ long diag_off(const BatchMap& B, int seg) {
+  return (long)B.segPlane[seg] * B.N;
+}
+
+// Per-segment FP32 diagonal view: the operator with the segment's diag plane.
 template <typename T>
-__device__ __forceinline__ LduOp<T> seg_op(LduOp<T> A, long voff) {
-  A.diag += voff;
+__device__ __forceinline__ LduOp<T> seg_op(LduOp<T> A, long doff) {
+  A.diag += doff;
   return A;
 }
 
@@ -133,9 +141,9 @@ struct RhsPartials {
 // operator uses them, the face loops otherwise.
 template <typename XF>
 __device__ __forceinline__ float batch_row(const LduOp<float>& A, const RowSlots<float>& rs,
-                                           long voff, int i, float xi, XF xf) {
-  if (A.slots) return rs.apply(A.diag + voff, i, xi, xf);
-  return ldu_row(seg_op(A, voff), i, xi, xf);
+                                           long doff, int i, float xi, XF xf) {
+  if (A.slots) return rs.apply(A.diag + doff, i, xi, xf);
+  return ldu_row(seg_op(A, doff), i, xi, xf);
 }
 
 // ---------------------------------------------------------------------------
@@ -163,10 +171,11 @@ __global__ __launch_bounds__(256) void kb_cheb_step(BatchMap B, LduOp<float> A,
     if (!((run >> q) & 1)) continue;
     double v[1] = {0.0};
     if (in) {
-      const KState* st = stBase + u.sys * B.nrhs + u.rb + q;
-      const long voff = (long)(u.rb + q) * B.N;
+      const int seg = u.sys * B.nrhs + u.rb + q;
+      const KState* st = stBase + seg;
+      const long voff = (long)(u.rb + q) * B.N, doff = diag_off(B, seg);
       const float* rr = r + voff;
-      const float* dd = rD + voff;
+      const float* dd = rD + doff;
       const float c2 = (float)st->chebC2[step - 1];
       const float c3 = (float)st->chebC3[step - 1];
       const float c1 = (float)st->chebC1;
@@ -175,13 +184,13 @@ __global__ __launch_bounds__(256) void kb_cheb_step(BatchMap B, LduOp<float> A,
       float zi_new, wi;
       if (FIRST) {
         const float z0 = c1 * di * ri;
-        const float y = batch_row(A, rs, voff, i, z0, [&](int j) { return c1 * dd[j] * rr[j]; });
+        const float y = batch_row(A, rs, doff, i, z0, [&](int j) { return c1 * dd[j] * rr[j]; });
         wi = c2 * z0 + c3 * di * (ri - y);
         zi_new = z0 + wi;
       } else {
         const float* zi_ = zin + voff;
         const float zi = zi_[i], wold = w[voff + i];
-        const float y = batch_row(A, rs, voff, i, zi, [&](int j) { return zi_[j]; });
+        const float y = batch_row(A, rs, doff, i, zi, [&](int j) { return zi_[j]; });
         wi = c2 * wold + c3 * di * (ri - y);
         zi_new = zi + wi;
       }
@@ -213,12 +222,13 @@ __global__ __launch_bounds__(256) void kb_pcg_p_spmv(BatchMap B, LduOp<float> A,
     if (!((run >> q) & 1)) continue;
     double v[1] = {0.0};
     if (in) {
-      const long voff = (long)(u.rb + q) * B.N;
-      const float beta = (float)stBase[u.sys * B.nrhs + u.rb + q].beta;
+      const int seg = u.sys * B.nrhs + u.rb + q;
+      const long voff = (long)(u.rb + q) * B.N, doff = diag_off(B, seg);
+      const float beta = (float)stBase[seg].beta;
       const float* w_ = wA + voff;
       const float* po = pold + voff;
       const float pi = w_[i] + beta * po[i];
-      const float y = batch_row(A, rs, voff, i, pi, [&](int j) { return w_[j] + beta * po[j]; });
+      const float y = batch_row(A, rs, doff, i, pi, [&](int j) { return w_[j] + beta * po[j]; });
       pnew[voff + i] = pi;
       Ap[voff + i] = y;
       v[0] = (double)(pi * y);
@@ -309,17 +319,18 @@ __global__ __launch_bounds__(256) void kb_bicg_py_spmv(BatchMap B, LduOp<float> 
     if (!((run >> q) & 1)) continue;
     double v[1] = {0.0};
     if (in) {
-      const KState* st = stBase + u.sys * B.nrhs + u.rb + q;
-      const long voff = (long)(u.rb + q) * B.N;
+      const int seg = u.sys * B.nrhs + u.rb + q;
+      const KState* st = stBase + seg;
+      const long voff = (long)(u.rb + q) * B.N, doff = diag_off(B, seg);
       const float beta = (float)st->beta, omega = (float)st->omega;
       const float* r_ = r + voff;
-      const float* d_ = rD + voff;
+      const float* d_ = rD + doff;
       const float* po = pold + voff;
       const float* ao = AyAold + voff;
       const float a0 = rA0[voff + i];
       const float pi = r_[i] + beta * (po[i] - omega * ao[i]);
       const float yi = d_[i] * pi;
-      const float y = batch_row(A, rs, voff, i, yi, [&](int j) {
+      const float y = batch_row(A, rs, doff, i, yi, [&](int j) {
         const float pj = r_[j] + beta * (po[j] - omega * ao[j]);
         return d_[j] * pj;
       });
@@ -354,14 +365,15 @@ __global__ __launch_bounds__(256) void kb_bicg_sz_spmv(BatchMap B, LduOp<float> 
     if (!((run >> q) & 1)) continue;
     double v[3] = {0.0, 0.0, 0.0};
     if (in) {
-      const long voff = (long)(u.rb + q) * B.N;
-      const float alpha = (float)stBase[u.sys * B.nrhs + u.rb + q].alpha;
+      const int seg = u.sys * B.nrhs + u.rb + q;
+      const long voff = (long)(u.rb + q) * B.N, doff = diag_off(B, seg);
+      const float alpha = (float)stBase[seg].alpha;
       const float* r_ = r + voff;
-      const float* d_ = rD + voff;
+      const float* d_ = rD + doff;
       const float* a_ = AyA + voff;
       const float si = r_[i] - alpha * a_[i];
       const float zi = d_[i] * si;
-      const float t = batch_row(A, rs, voff, i, zi, [&](int j) {
+      const float t = batch_row(A, rs, doff, i, zi, [&](int j) {
         const float sj = r_[j] - alpha * a_[j];
         return d_[j] * sj;
       });
@@ -599,11 +611,16 @@ __global__ __launch_bounds__(256) void kb_prepare_inner(BatchMap B, LduOp<double
                                                         const int32_t* __restrict__ segMode,
                                                         float* __restrict__ diag32,
                                                         float* __restrict__ rD32,
+                                                        int32_t* __restrict__ segPlane,
                                                         double* partials, int pstride) {
-  // segMode: 0 = leave, 1 = exact diag, 2 = deflated diag (diag - u), +4 = want beta
+  // segMode: 0 = leave, 1 = exact diag, 2 = deflated diag (diag - u), +4 = want beta.
+  // The segment's plane (0 exact, 1 deflated) is written by every segment
+  // that selects it, with the same values: both depend on the matrix only.
   const int lb = logical_block(B.xcdRemap);
   const BlockSeg g = block_seg(B, lb);
   const int mode = segMode[g.seg];
+  const int plane = (mode & 3) == 2 ? 1 : 0;
+  if ((mode & 3) && threadIdx.x == 0) segPlane[g.seg] = plane;
   double v[2] = {0.0, -1.0e308};
   const int i = g.r0 + threadIdx.x;
   if (i < g.r1 && (mode & 3)) {
@@ -613,8 +630,9 @@ __global__ __launch_bounds__(256) void kb_prepare_inner(BatchMap B, LduOp<double
       ds = d - u[i];
       if (ds * d <= kVSmall * fabs(ds)) v[0] = 1.0;
     }
-    diag32[g.voff + i] = (float)ds;
-    rD32[g.voff + i] = (float)(1.0 / ds);
+    const long doff = (long)plane * B.N;
+    diag32[doff + i] = (float)ds;
+    rD32[doff + i] = (float)(1.0 / ds);
     if (mode & 4) {
       double so = 0.0;
       for (int f = A.ownStart[i]; f < A.ownStart[i + 1]; ++f) so += fabs(upperF[f]);

@@ -87,3 +87,44 @@ def test_batch_rejects_mixed_symmetry(ctx):
         H.Batch(ctx, H.PBICGSTAB, [a, b], 1)
     with pytest.raises(H.HipFoamError):
         H.Batch(ctx, H.PCG, [b], 1)
+
+
+@pytest.mark.parametrize("schedule", [0, 2])
+def test_segments_of_one_system_on_different_diagonals(ctx, schedule):
+    """The RHS of one system share its FP32 diagonal planes (exact and
+    deflated) and pick one each.  On the pinned Poisson a zero-mean RHS stays
+    deflated, while a RHS with a mean stalls under deflation and is polished
+    on the exact diagonal: both planes are read within one system in the
+    same launches, and every segment must still equal its own
+    single-system solve."""
+    ctx.set_schedule(spmvSchedule=schedule, blockSize=0, xcdRemap=2, graphChunk=4,
+                     gridLimit=0, rowForm=0)
+    nrhs = 3
+    m0 = mg.poisson(12, 13, 11, seed=510, lognormal=True, zero_mean_rhs=True, rhs="normal",
+                    nrhs=nrhs)
+    m0.source[:, 1] += 1.0
+    m1 = mg.poisson(9, 8, 10, seed=511, lognormal=True, zero_mean_rhs=False, rhs="normal",
+                    nrhs=nrhs)
+    systems = [m0, m1]
+    batch = H.Batch(ctx, H.PCG, systems, nrhs)
+    for k, m in enumerate(systems):
+        batch.set_rhs(k, m.source)
+    ctl = H.default_controls(maxRestarts=8)
+    perfs = batch.solve(ctl)
+    pol0 = [perfs[r].polished for r in range(nrhs)]
+    print("system 0 deflated", [perfs[r].deflated for r in range(nrhs)], "polished", pol0)
+    assert all(perfs[r].deflated for r in range(nrhs))
+    assert not pol0[0] and pol0[1]
+    for k, m in enumerate(systems):
+        psi_b = batch.psi(k)
+        ctx.set_topology(m)
+        for r in range(nrhs):
+            p = perfs[k * nrhs + r]
+            psi = np.zeros(m.n)
+            ps = ctx.solve(H.PCG, m.diag, m.upper, None, psi, m.source[:, r].copy(), ctl)
+            assert p.converged and p.finalResidual < 1e-6
+            assert (p.deflated, p.polished) == (ps.deflated, ps.polished)
+            assert abs(p.nIterations - ps.nIterations) <= max(3, ps.nIterations // 10)
+            np.testing.assert_allclose(psi_b[r], psi, rtol=0,
+                                       atol=1e-4 * max(np.abs(psi).max(), 1.0))
+    batch.close()
