@@ -428,19 +428,31 @@ def _common(args, value, unit, ms, desc, higher):
 
 
 def run_batch(args, H, mg, desc):
-    """Batched throughput: 320 refined solves (64 systems x 5 RHS) per step."""
+    """Batched throughput: 320 refined solves (64 systems x 5 RHS) per step, after
+    the batch autotuner has picked each batch's schedule and RHS group."""
     systems = mg.config3()
     nrhs = systems[0].source.shape[1]
     sym = [m for m in systems if m.lower is None]
     asym = [m for m in systems if m.lower is not None]
     log(f"batch: {len(sym)} symmetric + {len(asym)} asymmetric systems, {nrhs} RHS, "
         f"{sum(m.n for m in systems):,} cells")
-    ctx = H.Context(0)
+    # one context per batch: each keeps the schedule its autotuner picks
+    ctxs = [H.Context(0), H.Context(0)]
+    ctx = ctxs[0]
     log(ctx.description)
-    batches = [H.Batch(ctx, H.PCG, sym, nrhs), H.Batch(ctx, H.PBICGSTAB, asym, nrhs)]
+    batches = [H.Batch(ctxs[0], H.PCG, sym, nrhs), H.Batch(ctxs[1], H.PBICGSTAB, asym, nrhs)]
     for bt in batches:
         for k, m in enumerate(bt.systems):
             bt.set_rhs(k, m.source)
+    # batch autotuner (outside the timed region): SpMV schedule, row form,
+    # XCD mapping and RHS group from timings of the batched SpMV
+    tuned = []
+    for name, bt in zip(("pcg", "bicg"), batches):
+        sc, grp = bt.autotune()
+        tuned.append({"batch": name, "spmvSchedule": sc.spmvSchedule, "rowForm": sc.rowForm,
+                      "xcdRemap": sc.xcdRemap, "rhsGroup": grp})
+        log(f"batch autotune ({name}): schedule {sc.spmvSchedule}, rowForm {sc.rowForm}, "
+            f"xcdRemap {sc.xcdRemap}, RHS group {grp}")
     ctl = H.default_controls(maxRestarts=8, maxIter=20000)
 
     def step():
@@ -452,11 +464,13 @@ def run_batch(args, H, mg, desc):
 
     for _ in range(args.warmup):
         step()
-    ctx.synchronize()
+    for c in ctxs:
+        c.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         perfs = step()
-    ctx.synchronize()
+    for c in ctxs:
+        c.synchronize()
     ms = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
     nsolves = nrhs * len(systems)
     flat = [p for ps in perfs for p in ps]
@@ -482,7 +496,8 @@ def run_batch(args, H, mg, desc):
     out["config"].update({"systems": len(systems), "rhs": nrhs,
                           "cells": int(sum(m.n for m in systems)), "tolerance": 1e-6})
     out["batch"] = {"solves": nsolves, "converged": conv, "worst_numpy_residual": worst,
-                    "iterations_min_median_max": [min(iters), int(np.median(iters)), max(iters)]}
+                    "iterations_min_median_max": [min(iters), int(np.median(iters)), max(iters)],
+                    "autotuned": tuned}
     out["roofline"] = None
     out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -262,6 +262,11 @@ int32_t hipfoam_batch_solve(hipfoam_batch* batch,
                             hipfoam_performance* perf);
 /* psi: [nRhs][nCells] of one system. */
 int32_t hipfoam_batch_get_psi(hipfoam_batch* batch, int32_t system, double* psi);
+/* Right-hand sides per workgroup of the batch's Krylov iteration kernels,
+ * 1..8 (default 8): each row's matrix slots are loaded once per group.
+ * Results do not depend on it.  No reference counterpart (the reference
+ * has no batched solver). */
+int32_t hipfoam_batch_set_rhs_group(hipfoam_batch* batch, int32_t group);
 void hipfoam_batch_destroy(hipfoam_batch* batch);
 
 /* ---- operator primitives --------------------------------------------- */
@@ -358,6 +363,13 @@ int32_t hipfoam_set_schedule(hipfoam_context* ctx, const hipfoam_schedule* s);
 /* Times the candidate SpMV schedules on the staged operator and keeps the
  * fastest; *best (may be NULL) receives it. */
 int32_t hipfoam_autotune(hipfoam_context* ctx, hipfoam_schedule* best);
+/* Times the batched SpMV for every SpMV schedule, row form, XCD mapping and
+ * RHS group, and keeps the fastest: the schedule in the context (as
+ * hipfoam_autotune does, so it also applies to the context's
+ * single-system solves), the group in the batch.  best and group may be
+ * NULL.  Results do not depend on the choice.  No reference counterpart. */
+int32_t hipfoam_batch_autotune(hipfoam_batch* batch, hipfoam_schedule* best,
+                               int32_t* group);
 
 /* Kernel timing: when enabled, the inner loops launch without graphs and
  * bracket every launch of kernel class `kclass` with HIP events on the

@@ -114,6 +114,8 @@ SIGNATURES = {
     "hipfoam_batch_solve": (C.c_int32, [_vp, C.POINTER(Controls), C.POINTER(Performance)]),
     "hipfoam_batch_get_psi": (C.c_int32, [_vp, C.c_int32, _vp]),
     "hipfoam_batch_destroy": (None, [_vp]),
+    "hipfoam_batch_set_rhs_group": (C.c_int32, [_vp, C.c_int32]),
+    "hipfoam_batch_autotune": (C.c_int32, [_vp, C.POINTER(Schedule), C.POINTER(C.c_int32)]),
     "hipfoam_amul": (C.c_int32, [_P, C.c_int32, _vp, _vp]),
     "hipfoam_amul_device": (C.c_int32, [_P, C.c_int32, _vp, _vp]),
     "hipfoam_set_matrix": (C.c_int32, [_P, _vp, _vp, _vp]),

@@ -1721,6 +1721,26 @@ int32_t hipfoam_batch_solve(hipfoam_batch* bt, const hipfoam_controls* controls,
   return batch_solve_impl(bt, controls, perf);
 }
 
+int32_t hipfoam_batch_set_rhs_group(hipfoam_batch* bt, int32_t group) {
+  FpTrapGuard fpe;
+  if (!bt) return HIPFOAM_ERR_ARG;
+  if (group < 1 || group > kRhsGroup)
+    return fail(bt->c, HIPFOAM_ERR_ARG,
+                "rhs group must be 1.." + std::to_string(kRhsGroup));
+  if (group != bt->group) {
+    bt->group = group;
+    bclear_graphs(bt);
+  }
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_batch_autotune(hipfoam_batch* bt, hipfoam_schedule* best, int32_t* group) {
+  FpTrapGuard fpe;
+  if (!bt) return HIPFOAM_ERR_ARG;
+  CK_C(bt->c, hipSetDevice(bt->c->device));
+  return batch_autotune_impl(bt, best, group);
+}
+
 int32_t hipfoam_batch_get_psi(hipfoam_batch* bt, int32_t system, double* psi) {
   FpTrapGuard fpe;
   if (!bt) return HIPFOAM_ERR_ARG;

@@ -35,6 +35,7 @@ struct BatchMap {
   const int32_t* __restrict__ sysTile0;  // [nsys + 1] first tile of each system
   const int32_t* __restrict__ segPlane;  // [nseg] diagonal plane: 0 exact, 1 deflated
   int32_t ntiles, nrhs, N, xcdRemap;
+  int32_t group;  // RHS per workgroup of the Krylov iteration kernels, 1..kRhsGroup
 };
 
 struct BlockSeg {
@@ -68,7 +69,7 @@ __device__ __forceinline__ LduOp<T> seg_op(LduOp<T> A, long doff) {
 
 // ---------------------------------------------------------------------------
 // Krylov iteration kernels: one workgroup per (tile, group of up to
-// kRhsGroup right-hand sides).  Each thread owns one row: it loads the row's
+// B.group <= kRhsGroup right-hand sides).  Each thread owns one row: it loads the row's
 // matrix slots once and applies them to every RHS of the group whose segment
 // is still running, one RHS after another.  Each RHS's values are reduced
 // across the wavefront at once into LDS, so no per-RHS accumulator stays
@@ -84,11 +85,11 @@ struct TileGroup {
 };
 
 __device__ __forceinline__ TileGroup tile_group(const BatchMap& B, int lb) {
-  const int ng = (B.nrhs + kRhsGroup - 1) / kRhsGroup;
+  const int ng = (B.nrhs + B.group - 1) / B.group;
   TileGroup u;
   u.t = lb / ng;
-  u.rb = (lb - u.t * ng) * kRhsGroup;
-  u.nr = B.nrhs - u.rb < kRhsGroup ? B.nrhs - u.rb : kRhsGroup;
+  u.rb = (lb - u.t * ng) * B.group;
+  u.nr = B.nrhs - u.rb < B.group ? B.nrhs - u.rb : B.group;
   u.r0 = B.tileRow0[u.t];
   u.r1 = B.tileRow0[u.t + 1];
   u.sys = B.tileSys[u.t];
@@ -144,6 +145,26 @@ __device__ __forceinline__ float batch_row(const LduOp<float>& A, const RowSlots
                                            long doff, int i, float xi, XF xf) {
   if (A.slots) return rs.apply(A.diag + doff, i, xi, xf);
   return ldu_row(seg_op(A, doff), i, xi, xf);
+}
+
+// y = A x for every RHS, with each segment's diagonal plane: the timing
+// kernel of the batch autotuner (same rows, slots and RHS groups as the
+// iteration kernels)
+__global__ __launch_bounds__(256) void kb_spmv(BatchMap B, LduOp<float> A,
+                                               const float* __restrict__ x,
+                                               float* __restrict__ y) {
+  const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
+  const int i = u.r0 + threadIdx.x;
+  if (i >= u.r1) return;
+  RowSlots<float> rs;
+  if (A.slots) rs.load(A, i, u.t);
+#pragma unroll 1
+  for (int q = 0; q < u.nr; ++q) {
+    const int seg = u.sys * B.nrhs + u.rb + q;
+    const long voff = (long)(u.rb + q) * B.N, doff = diag_off(B, seg);
+    const float* xx = x + voff;
+    y[voff + i] = batch_row(A, rs, doff, i, xx[i], [&](int j) { return xx[j]; });
+  }
 }
 
 // ---------------------------------------------------------------------------

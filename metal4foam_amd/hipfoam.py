@@ -344,6 +344,20 @@ class Batch:
                         "batch_solve")
         return list(perf)
 
+    def set_rhs_group(self, group):
+        """Right-hand sides per workgroup of the iteration kernels (1..8)."""
+        self.ctx._check(self.ctx.lib.hipfoam_batch_set_rhs_group(self.h, int(group)),
+                        "batch_set_rhs_group")
+
+    def autotune(self):
+        """Times the batched SpMV over schedules and RHS groups; keeps the fastest
+        (schedule in the context, group in the batch).  Returns (schedule, group)."""
+        s = Schedule()
+        g = C.c_int32()
+        self.ctx._check(self.ctx.lib.hipfoam_batch_autotune(self.h, C.byref(s), C.byref(g)),
+                        "batch_autotune")
+        return s, g.value
+
     def psi(self, k):
         out = np.empty((self.nrhs, self.systems[k].n))
         self.ctx._check(self.ctx.lib.hipfoam_batch_get_psi(self.h, k, _ptr(out)),

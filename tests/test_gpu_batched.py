@@ -128,3 +128,68 @@ def test_segments_of_one_system_on_different_diagonals(ctx, schedule):
             np.testing.assert_allclose(psi_b[r], psi, rtol=0,
                                        atol=1e-4 * max(np.abs(psi).max(), 1.0))
     batch.close()
+
+
+def _solve_all(batch, systems, ctl):
+    batch.reset_psi()
+    perfs = batch.solve(ctl)
+    return perfs, [batch.psi(k) for k in range(len(systems))]
+
+
+@pytest.mark.parametrize("kind,solver", [("sym", H.PCG), ("asym", H.PBICGSTAB)])
+def test_rhs_groups_give_identical_solves(ctx, kind, solver):
+    """The RHS group only sets how many right-hand sides a workgroup of the
+    iteration kernels serves (5 RHS: groups 3 and 2 leave a partial group);
+    every segment's partials keep one slot per (tile, rhs), so the solves
+    are bit-identical for every group size."""
+    ctx.set_schedule(spmvSchedule=2, blockSize=0, xcdRemap=2, graphChunk=4, gridLimit=0,
+                     rowForm=0)
+    nrhs = 5
+    systems = _systems(kind, [10, 7, 13], nrhs, seed=620)
+    batch = H.Batch(ctx, solver, systems, nrhs)
+    for k, m in enumerate(systems):
+        batch.set_rhs(k, m.source)
+    ctl = H.default_controls(maxRestarts=6)
+    ref_p, ref_psi = _solve_all(batch, systems, ctl)
+    assert all(p.converged for p in ref_p)
+    for g in (1, 2, 3, 5, 8):
+        batch.set_rhs_group(g)
+        perfs, psi = _solve_all(batch, systems, ctl)
+        print(f"group {g}: iterations {[p.nIterations for p in perfs]}")
+        assert [p.nIterations for p in perfs] == [p.nIterations for p in ref_p]
+        assert [p.finalResidual for p in perfs] == [p.finalResidual for p in ref_p]
+        for a, b in zip(psi, ref_psi):
+            np.testing.assert_array_equal(a, b)
+    for bad in (0, 9):
+        with pytest.raises(H.HipFoamError):
+            batch.set_rhs_group(bad)
+    batch.close()
+
+
+def test_batch_autotune_keeps_results(ctx):
+    """The batch autotuner picks a schedule (context) and an RHS group
+    (batch) from timings alone: the solve after it is bit-identical."""
+    keep = ctx.get_schedule()
+    nrhs = 3
+    systems = _systems("sym", [12, 9], nrhs, seed=640)
+    batch = H.Batch(ctx, H.PCG, systems, nrhs)
+    for k, m in enumerate(systems):
+        batch.set_rhs(k, m.source)
+    ctl = H.default_controls(maxRestarts=6)
+    ref_p, ref_psi = _solve_all(batch, systems, ctl)
+    sched, group = batch.autotune()
+    print(f"batch autotune: schedule {sched.spmvSchedule} rowForm {sched.rowForm} "
+          f"xcdRemap {sched.xcdRemap} group {group}")
+    assert sched.spmvSchedule in (0, 1, 2) and sched.rowForm in (0, 1)
+    assert sched.xcdRemap in (0, 1) and 1 <= group <= nrhs
+    got = ctx.get_schedule()
+    assert (got.spmvSchedule, got.rowForm, got.xcdRemap) == (
+        sched.spmvSchedule, sched.rowForm, sched.xcdRemap)
+    perfs, psi = _solve_all(batch, systems, ctl)
+    assert [p.nIterations for p in perfs] == [p.nIterations for p in ref_p]
+    for a, b in zip(psi, ref_psi):
+        np.testing.assert_array_equal(a, b)
+    batch.close()
+    ctx.set_schedule(spmvSchedule=keep.spmvSchedule, blockSize=keep.blockSize,
+                     xcdRemap=keep.xcdRemap, graphChunk=keep.graphChunk,
+                     gridLimit=keep.gridLimit, rowForm=keep.rowForm)
