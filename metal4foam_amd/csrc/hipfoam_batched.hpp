@@ -115,7 +115,7 @@ struct RhsPartials {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-      const double t = wave_sum(v[j]);
+      const double t = wave_sum_valu(v[j]);
       if (lane == 0) sh[q * NV + j][wid] = t;
     }
   }
@@ -293,8 +293,9 @@ __global__ __launch_bounds__(256) void kb_pcg_update(BatchMap B, float* __restri
 }
 
 // ---------------------------------------------------------------------------
-// BiCGStab: the three fused row kernels of the single-system path, per RHS
-// (p and AyA ping-pong; rho of the next iteration reduced in the update)
+// BiCGStab (metalFoamKernels.metal:279-458 per segment): p and y, A y,
+// s and z, A z, then the update, which also reduces the next iteration's rho
+// (p and AyA ping-pong)
 // ---------------------------------------------------------------------------
 
 // rho = rA0.rA of the first iteration
@@ -316,17 +317,46 @@ __global__ __launch_bounds__(256) void kb_bicg_rho(BatchMap B, const float* __re
   part.write(partials, pstride, u.t * B.nrhs + u.rb, run);
 }
 
-__global__ __launch_bounds__(256) void kb_bicg_py_spmv(BatchMap B, LduOp<float> A,
-                                                       const float* __restrict__ r,
-                                                       const float* __restrict__ rD,
-                                                       const float* __restrict__ pold,
-                                                       float* __restrict__ pnew,
-                                                       const float* __restrict__ AyAold,
-                                                       float* __restrict__ AyAnew,
-                                                       float* __restrict__ yA,
-                                                       const float* __restrict__ rA0,
-                                                       KState* stBase, double* partials,
-                                                       int pstride) {
+// The BiCGStab SpMVs of a batch are limited by gathered loads rather than
+// HBM bytes: forming p_j and y_j = rD_j p_j at each neighbour costs four
+// gathers per neighbour and RHS (three for s and z).  So a streaming pass
+// forms the vector and its preconditioned image at every row first, and the
+// SpMV gathers that one vector.  Every value is the same float expression as
+// when formed at the gathers.
+
+// p = r + beta (p - omega AyA), y = rD p, for every running RHS
+__global__ __launch_bounds__(256) void kb_bicg_py(BatchMap B, const float* __restrict__ r,
+                                                  const float* __restrict__ rD,
+                                                  const float* __restrict__ pold,
+                                                  float* __restrict__ pnew,
+                                                  const float* __restrict__ AyAold,
+                                                  float* __restrict__ yA,
+                                                  const KState* stBase) {
+  const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
+  const unsigned run = status_bits(stBase, u, B.nrhs, 1 << ST_RUNNING);
+  const int i = u.r0 + threadIdx.x;
+  if (!run || i >= u.r1) return;
+#pragma unroll 1
+  for (int q = 0; q < u.nr; ++q) {
+    if (!((run >> q) & 1)) continue;
+    const int seg = u.sys * B.nrhs + u.rb + q;
+    const KState* st = stBase + seg;
+    const long k = (long)(u.rb + q) * B.N + i;
+    const float beta = (float)st->beta, omega = (float)st->omega;
+    const float di = rD[diag_off(B, seg) + i];
+    const float pi = r[k] + beta * (pold[k] - omega * AyAold[k]);
+    pnew[k] = pi;
+    yA[k] = di * pi;
+  }
+}
+
+// AyA = A y and rA0.AyA
+__global__ __launch_bounds__(256) void kb_bicg_y_spmv(BatchMap B, LduOp<float> A,
+                                                      const float* __restrict__ yA,
+                                                      float* __restrict__ AyAnew,
+                                                      const float* __restrict__ rA0,
+                                                      KState* stBase, double* partials,
+                                                      int pstride) {
   const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
   const unsigned run = status_bits(stBase, u, B.nrhs, 1 << ST_RUNNING);
   if (!run) return;
@@ -341,22 +371,10 @@ __global__ __launch_bounds__(256) void kb_bicg_py_spmv(BatchMap B, LduOp<float> 
     double v[1] = {0.0};
     if (in) {
       const int seg = u.sys * B.nrhs + u.rb + q;
-      const KState* st = stBase + seg;
       const long voff = (long)(u.rb + q) * B.N, doff = diag_off(B, seg);
-      const float beta = (float)st->beta, omega = (float)st->omega;
-      const float* r_ = r + voff;
-      const float* d_ = rD + doff;
-      const float* po = pold + voff;
-      const float* ao = AyAold + voff;
+      const float* y_ = yA + voff;
       const float a0 = rA0[voff + i];
-      const float pi = r_[i] + beta * (po[i] - omega * ao[i]);
-      const float yi = d_[i] * pi;
-      const float y = batch_row(A, rs, doff, i, yi, [&](int j) {
-        const float pj = r_[j] + beta * (po[j] - omega * ao[j]);
-        return d_[j] * pj;
-      });
-      pnew[voff + i] = pi;
-      yA[voff + i] = yi;
+      const float y = batch_row(A, rs, doff, i, y_[i], [&](int j) { return y_[j]; });
       AyAnew[voff + i] = y;
       v[0] = (double)(a0 * y);
     }
@@ -365,14 +383,35 @@ __global__ __launch_bounds__(256) void kb_bicg_py_spmv(BatchMap B, LduOp<float> 
   part.write(partials, pstride, u.t * B.nrhs + u.rb, run);
 }
 
-__global__ __launch_bounds__(256) void kb_bicg_sz_spmv(BatchMap B, LduOp<float> A,
-                                                       const float* __restrict__ r,
-                                                       const float* __restrict__ rD,
-                                                       const float* __restrict__ AyA,
-                                                       float* __restrict__ sA,
-                                                       float* __restrict__ zA,
-                                                       float* __restrict__ tA, KState* stBase,
-                                                       double* partials, int pstride) {
+// s = r - alpha AyA, z = rD s, for every running RHS
+__global__ __launch_bounds__(256) void kb_bicg_sz(BatchMap B, const float* __restrict__ r,
+                                                  const float* __restrict__ rD,
+                                                  const float* __restrict__ AyA,
+                                                  float* __restrict__ sA, float* __restrict__ zA,
+                                                  const KState* stBase) {
+  const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
+  const unsigned run = status_bits(stBase, u, B.nrhs, 1 << ST_RUNNING);
+  const int i = u.r0 + threadIdx.x;
+  if (!run || i >= u.r1) return;
+#pragma unroll 1
+  for (int q = 0; q < u.nr; ++q) {
+    if (!((run >> q) & 1)) continue;
+    const int seg = u.sys * B.nrhs + u.rb + q;
+    const long k = (long)(u.rb + q) * B.N + i;
+    const float alpha = (float)stBase[seg].alpha;
+    const float di = rD[diag_off(B, seg) + i];
+    const float si = r[k] - alpha * AyA[k];
+    sA[k] = si;
+    zA[k] = di * si;
+  }
+}
+
+// t = A z, and |s|, s.t, t.t
+__global__ __launch_bounds__(256) void kb_bicg_z_spmv(BatchMap B, LduOp<float> A,
+                                                      const float* __restrict__ sA,
+                                                      const float* __restrict__ zA,
+                                                      float* __restrict__ tA, KState* stBase,
+                                                      double* partials, int pstride) {
   const TileGroup u = tile_group(B, logical_block(B.xcdRemap));
   const unsigned run = status_bits(stBase, u, B.nrhs, 1 << ST_RUNNING);
   if (!run) return;
@@ -388,18 +427,9 @@ __global__ __launch_bounds__(256) void kb_bicg_sz_spmv(BatchMap B, LduOp<float> 
     if (in) {
       const int seg = u.sys * B.nrhs + u.rb + q;
       const long voff = (long)(u.rb + q) * B.N, doff = diag_off(B, seg);
-      const float alpha = (float)stBase[seg].alpha;
-      const float* r_ = r + voff;
-      const float* d_ = rD + doff;
-      const float* a_ = AyA + voff;
-      const float si = r_[i] - alpha * a_[i];
-      const float zi = d_[i] * si;
-      const float t = batch_row(A, rs, doff, i, zi, [&](int j) {
-        const float sj = r_[j] - alpha * a_[j];
-        return d_[j] * sj;
-      });
-      sA[voff + i] = si;
-      zA[voff + i] = zi;
+      const float* z_ = zA + voff;
+      const float si = sA[voff + i];
+      const float t = batch_row(A, rs, doff, i, z_[i], [&](int j) { return z_[j]; });
       tA[voff + i] = t;
       v[0] = (double)fabsf(si);
       v[1] = (double)(si * t);

@@ -155,6 +155,42 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// wave_sum on VALU lane moves instead of LDS permutes (__shfl_xor lowers to
+// ds_bpermute, two per step for a double): the gfx950 swaps of the two
+// 32-lane halves and of 16-lane rows for the steps 32 and 16, then rotations
+// within 16-lane rows by 8, 4, 2 and 1.  After the steps for 32, 16 and 8,
+// lanes i and i ^ m hold equal values for every m those steps covered, so
+// lane (i + k) mod 16 holds the value of lane i ^ k, and every lane adds the
+// same two operands as wave_sum does (a swap hands half of the lanes their
+// operands in the other order, which IEEE addition does not see): the result
+// is bit-identical to wave_sum in every lane.  Needs all 64 lanes active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xf, 0xf, false),
+                          __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ double wave_sum_valu(double v) {
+  {
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    v = __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+  }
+  {
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    v = __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+  }
+  v = v + dpp_f64<0x128>(v);  // row_ror:8
+  v = v + dpp_f64<0x124>(v);  // row_ror:4
+  v = v + dpp_f64<0x122>(v);  // row_ror:2
+  v = v + dpp_f64<0x121>(v);  // row_ror:1
+  return v;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
