@@ -152,6 +152,8 @@ def main():
                          "iterations and scale")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--spmv-tune", type=int, default=1,
+                    help="also time the plain SpMV on the launch shape hipfoam_autotune picks")
     ap.add_argument("--cold-reps", type=int, default=10,
                     help="SpMV launches timed after a MALL-flushing copy")
     ap.add_argument("--adjoint-precision", choices=["fp32", "fp64"], default="fp32",
@@ -308,6 +310,34 @@ def main():
         log(f"plain FP32 SpMV after a MALL flush: avg {1e6 * cavg:.2f} us, "
             f"{cold['GB_s']:.0f} GB/s")
 
+    # the same SpMV on the launch shape the library's autotuner picks for it
+    # (hipfoam_autotune times every SpMV schedule, row form, workgroup size,
+    # XCD mapping and grid limit); the solves above ran on the automatic shape
+    tuned = None
+    if args.spmv_tune and args.spmv_reps > 0:
+        keep = ctx.get_schedule()
+        best = ctx.autotune()
+        ctx.set_kernel_timing(True, 5)
+        for _ in range(args.spmv_reps):
+            ctx.amul_device(x, y, H.FP32)
+        nt, tsecs = ctx.kernel_timing()
+        ctx.set_kernel_timing(False, 0)
+        ctx.set_schedule(spmvSchedule=keep.spmvSchedule, blockSize=keep.blockSize,
+                         xcdRemap=keep.xcdRemap, graphChunk=keep.graphChunk,
+                         gridLimit=keep.gridLimit, rowForm=keep.rowForm)
+        tbytes = sbytes if (slots and best.spmvSchedule == 2) else ref_sbytes
+        tavg = tsecs / max(nt, 1)
+        tuned = {"schedule": {"spmvSchedule": best.spmvSchedule, "blockSize": best.blockSize,
+                              "xcdRemap": best.xcdRemap, "gridLimit": best.gridLimit,
+                              "rowForm": best.rowForm},
+                 "bytes_per_launch": tbytes,
+                 "avg_launch_us": round(1e6 * tavg, 3),
+                 "GB_s": round(tbytes / tavg / 1e9, 1),
+                 "frac": round(tbytes / tavg / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"plain FP32 SpMV on the autotuned shape {tuned['schedule']}: avg "
+            f"{1e6 * tavg:.2f} us, {tuned['GB_s']:.0f} GB/s "
+            f"({100 * tuned['frac']:.1f}% of HBM peak)")
+
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tfile):
@@ -383,6 +413,7 @@ def main():
             "reference_format_bytes": ref_sbytes,
             "reference_format_GB_s": round(ref_sgbs, 1),
             "mall_flushed": cold,
+            "autotuned": tuned,
         },
         "solve": {
             "iterations": perf.nIterations,
