@@ -115,7 +115,7 @@ struct RhsPartials {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-      const double t = wave_sum_valu(v[j]);
+      const double t = wave_sum(v[j]);
       if (lane == 0) sh[q * NV + j][wid] = t;
     }
   }

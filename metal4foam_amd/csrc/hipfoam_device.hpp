@@ -149,21 +149,18 @@ __device__ __forceinline__ void block_rows(int n, int lb, int& r0, int& r1) {
 // reductions
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// wave_sum on VALU lane moves instead of LDS permutes (__shfl_xor lowers to
-// ds_bpermute, two per step for a double): the gfx950 swaps of the two
-// 32-lane halves and of 16-lane rows for the steps 32 and 16, then rotations
-// within 16-lane rows by 8, 4, 2 and 1.  After the steps for 32, 16 and 8,
-// lanes i and i ^ m hold equal values for every m those steps covered, so
-// lane (i + k) mod 16 holds the value of lane i ^ k, and every lane adds the
-// same two operands as wave_sum does (a swap hands half of the lanes their
-// operands in the other order, which IEEE addition does not see): the result
-// is bit-identical to wave_sum in every lane.  Needs all 64 lanes active.
+// Sum over the 64 lanes of a wavefront, every lane receiving the total: the
+// butterfly v += v[lane ^ m] for m = 32, 16, 8, 4, 2, 1, on VALU lane moves
+// rather than __shfl_xor, which lowers to ds_bpermute (two LDS operations per
+// step for a double).  The steps 32 and 16 use the gfx950 swaps of the two
+// 32-lane halves and of 16-lane rows; 8, 4, 2 and 1 use DPP rotations within
+// 16-lane rows.  After the steps for 32, 16 and 8, lanes i and i ^ m hold
+// equal values for every m those steps covered, so lane (i + k) mod 16 holds
+// the value of lane i ^ k, and every lane adds the same two operands as the
+// xor butterfly (a swap hands half of the lanes their operands in the other
+// order, which IEEE addition does not see): the sum is bit-identical to the
+// xor butterfly's in every lane.  Needs all 64 lanes active: every caller
+// reduces whole waves (workgroup sizes are multiples of 64).
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
   const int lo = __double2loint(v), hi = __double2hiint(v);
@@ -171,7 +168,7 @@ __device__ __forceinline__ double dpp_f64(double v) {
                           __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xf, 0xf, false));
 }
 
-__device__ __forceinline__ double wave_sum_valu(double v) {
+__device__ __forceinline__ double wave_sum(double v) {
   {
     const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
     const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
