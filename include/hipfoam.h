@@ -366,8 +366,10 @@ int32_t hipfoam_autotune(hipfoam_context* ctx, hipfoam_schedule* best);
 /* Times the batched SpMV for every SpMV schedule, row form, XCD mapping and
  * RHS group, and keeps the fastest: the schedule in the context (as
  * hipfoam_autotune does, so it also applies to the context's
- * single-system solves), the group in the batch.  best and group may be
- * NULL.  Results do not depend on the choice.  No reference counterpart. */
+ * single-system solves), the group in the batch.  The default (row slots,
+ * XCD mapping 1, one group of all RHS up to 8) is kept unless another
+ * candidate is faster by more than 2 %.  best and group may be NULL.
+ * Results do not depend on the choice.  No reference counterpart. */
 int32_t hipfoam_batch_autotune(hipfoam_batch* batch, hipfoam_schedule* best,
                                int32_t* group);
 
