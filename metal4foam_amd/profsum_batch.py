@@ -3,8 +3,19 @@ right-hand sides, ``bench.py --workload batch64``) into a short Markdown table.
 
     rocprofv3 --kernel-trace --stats --output-format csv -d OUT/batch -o run -- \\
         python3 bench.py --workload batch64 --steps 1 --warmup 0
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- \\
+        python3 bench.py --workload batch64 --steps 1 --warmup 0
+    rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- \\
+        python3 bench.py --workload batch64 --steps 1 --warmup 0
     python -m metal4foam_amd.profsum_batch --trace OUT/batch/run_kernel_trace.csv \\
+        [--fetch OUT/fetch/run_counter_collection.csv \\
+         --write OUT/write/run_counter_collection.csv] \\
         --command "..." > profiles/kb_cheb_step_batch64.md
+
+FETCH_SIZE / WRITE_SIZE (KiB per dispatch) are scaled by the k_copy<double>
+calibration of the single-system profile (profiles/traffic_poisson128.json:
+FETCH_SIZE x 1.998, WRITE_SIZE x 1.0) and averaged over the full launches,
+those within 10 % of the kernel's largest count.
 
 The iteration kernels run one workgroup per (256-row tile, group of RHS) and
 skip the right-hand sides whose segment has stopped, so a launch moves fewer
@@ -83,6 +94,10 @@ def main():
     ap.add_argument("--full", type=float, default=0.9,
                     help="launches at least this fraction of the kernel's longest count as full")
     ap.add_argument("--top", type=int, default=14)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--fetch-scale", type=float, default=1.998)
+    ap.add_argument("--write-scale", type=float, default=1.0)
     args = ap.parse_args()
 
     geo, nrhs = batch_geometry()
@@ -124,6 +139,29 @@ def main():
     if "kb_spmv" in durs:
         print("`kb_spmv` is the batch autotuner's timing kernel (40 candidates, before the "
               "timed step).\n")
+    if args.fetch and args.write:
+        cnt = {}
+        for path, name, scale in ((args.fetch, "FETCH_SIZE", args.fetch_scale),
+                                  (args.write, "WRITE_SIZE", args.write_scale)):
+            vals = collections.defaultdict(list)
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    if r.get("Counter_Name", name) == name:
+                        vals[_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+            cnt[name] = {k: scale * 1024.0 * sum(x for x in v if x >= args.full * max(v)) /
+                         len([x for x in v if x >= args.full * max(v)])
+                         for k, v in vals.items() if v and max(v) > 0}
+        print("HBM traffic of the full launches (FETCH_SIZE x %.3f + WRITE_SIZE x %.3f):\n"
+              % (args.fetch_scale, args.write_scale))
+        print("| kernel | fetch MB | write MB | traffic MB | algorithmic MB | traffic / algorithmic |")
+        print("|---|---:|---:|---:|---:|---:|")
+        for k in KERNELS:
+            if k in cnt["FETCH_SIZE"] and k in cnt["WRITE_SIZE"]:
+                fb, wb = cnt["FETCH_SIZE"][k], cnt["WRITE_SIZE"][k]
+                alg = launch_bytes(k, geo, nrhs)
+                print(f"| `{k}` | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {(fb + wb) / 1e6:.1f} | "
+                      f"{alg / 1e6:.1f} | {(fb + wb) / alg:.2f} |")
+        print()
     print("Bytes per full launch: matrix slots (when the kernel applies A) + diagonal "
           "planes + RHS x per-RHS vector bytes, per row: " + "; ".join(
               f"`{k}` {v[1]} B/RHS{' + slots' if v[2] else ''}{f' + {v[3]} B diag' if v[3] else ''}"
