@@ -314,8 +314,10 @@ int32_t hipfoam_synchronize(hipfoam_context* ctx);
  *       offsets fit the bases, otherwise schedule 0 runs.  Same terms in the
  *       same order as 0 and 1, so all three give identical results,
  * and the iteration driver:
- *   graphChunk = iterations captured per hipGraph launch (0 = eager; even;
- *                default 4).
+ *   graphChunk = iterations captured per hipGraph launch (even), 0 = eager
+ *                launches, -1 = auto (the default): eager from 524,288 rows
+ *                on, where the kernels outlast their launches, graphs of 4
+ *                iterations below.  Results do not depend on it.
  * blockSize 0 (auto, the default) picks the launch shape from the mesh: one
  * 256-thread workgroup per 256-row tile up to 512 tiles, 512 workgroups of
  * 512 threads up to 49,152 tiles (every reduction's second stage then runs

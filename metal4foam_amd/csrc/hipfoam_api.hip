@@ -179,7 +179,7 @@ struct hipfoam_context {
   hipEvent_t evPoll[2] = {nullptr, nullptr};
   hipEvent_t evT0 = nullptr, evT1 = nullptr;
 
-  hipfoam_schedule sched{2, 0, 2, 4, 0, 0};
+  hipfoam_schedule sched{2, 0, 2, -1, 0, 0};
   std::map<GraphKey, hipGraphExec_t> graphs;
   KernelTimer timer;
   // pinned staging of large host arrays (created on first use)
@@ -203,6 +203,8 @@ namespace {
 // 256^3 (65,536 tiles) 8 x 256 per CU is 3-6 % faster, where every pass
 // streams from HBM and full occupancy pays.  An explicit blockSize launches
 // min(ceil(n / blockSize), gridLimit) workgroups (gridLimit 0: 8 per CU).
+// Rows from which the automatic graph chunk (-1) launches eagerly.
+constexpr int kEagerRows = 1 << 19;
 constexpr int kMergeMaxBlocks = 512;
 constexpr int kAutoSmallTiles = 512;
 constexpr int kAutoMidTiles = 49152;
@@ -688,7 +690,12 @@ template <typename T>
 int run_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
              int maxIter, double* gpuSeconds) {
   const bool timing = c->timer.on;
-  const bool eager = c->sched.graphChunk <= 0 || timing;
+  // graphChunk -1 (auto): eager launches from kEagerRows rows on, where the
+  // kernels outlast their launches and the host stays ahead of the GPU
+  // without a graph launch's boundary every few iterations; graphs of 4
+  // iterations below, where the launches themselves would set the pace
+  const bool eager = c->sched.graphChunk == 0 || timing ||
+                     (c->sched.graphChunk < 0 && c->n >= kEagerRows);
   const int chunk = timing ? 1 : (c->sched.graphChunk > 0 ? c->sched.graphChunk : 4);
   hipGraphExec_t ex = nullptr;
   if (!eager) {
@@ -1854,8 +1861,8 @@ int32_t hipfoam_set_schedule(hipfoam_context* c, const hipfoam_schedule* s) {
     return fail(c, HIPFOAM_ERR_ARG, "spmvSchedule must be 0, 1 or 2");
   if (s->blockSize != 0 && (s->blockSize < 64 || s->blockSize > 1024 || s->blockSize % 64))
     return fail(c, HIPFOAM_ERR_ARG, "blockSize must be 0 (auto) or a multiple of 64 in [64, 1024]");
-  if (s->graphChunk < 0 || s->graphChunk > 4096 || (s->graphChunk & 1))
-    return fail(c, HIPFOAM_ERR_ARG, "graphChunk must be even in [0, 4096]");
+  if (s->graphChunk < -1 || s->graphChunk > 4096 || (s->graphChunk > 0 && (s->graphChunk & 1)))
+    return fail(c, HIPFOAM_ERR_ARG, "graphChunk must be -1 (auto), 0 (eager) or even in [2, 4096]");
   if (s->gridLimit < 0) return fail(c, HIPFOAM_ERR_ARG, "gridLimit must be >= 0");
   if (s->rowForm < 0 || s->rowForm > 1) return fail(c, HIPFOAM_ERR_ARG, "rowForm must be 0 or 1");
   if (c->good) {

@@ -61,7 +61,7 @@ def test_bad_device_gives_context_that_is_not_good():
 def test_schedule_validation_without_device():
     ctx = H.Context(device=999, require=False)
     s = ctx.get_schedule()
-    assert (s.spmvSchedule, s.blockSize, s.xcdRemap, s.graphChunk) == (2, 0, 2, 4)
+    assert (s.spmvSchedule, s.blockSize, s.xcdRemap, s.graphChunk) == (2, 0, 2, -1)
     bad = _lib.Schedule(3, 256, 1, 8, 0)
     assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(bad)) == _lib.ERR_ARG
     bad = _lib.Schedule(0, 100, 1, 8, 0)

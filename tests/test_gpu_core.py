@@ -246,8 +246,8 @@ def test_solve_device_entry_and_graph_modes_agree_bitwise(ctx):
     outs = []
     # graph chunks, eager launches, the looped row form and every block-to-XCD
     # mapping all give the same bits
-    for chunk, form, remap in ((4, 0, 1), (0, 0, 1), (2, 0, 1), (8, 0, 1), (4, 1, 1),
-                               (4, 0, 0), (4, 0, 2)):
+    for chunk, form, remap in ((4, 0, 1), (0, 0, 1), (2, 0, 1), (8, 0, 1), (-1, 0, 1),
+                               (4, 1, 1), (4, 0, 0), (4, 0, 2)):
         ctx.set_schedule(graphChunk=chunk, rowForm=form, xcdRemap=remap)
         psi = np.zeros(m.n)
         perf = ctx.solve(H.PCG, m.diag, m.upper, None, psi, m.source)
