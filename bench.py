@@ -474,7 +474,7 @@ def run_batch(args, H, mg, desc):
     batches = [H.Batch(ctxs[0], H.PCG, sym, nrhs), H.Batch(ctxs[1], H.PBICGSTAB, asym, nrhs)]
     for bt in batches:
         for k, m in enumerate(bt.systems):
-            bt.set_rhs(k, m.source)
+            bt.set_rhs(k, m.source, layout="cell_major")
     # batch autotuner (outside the timed region): SpMV schedule, row form,
     # XCD mapping and RHS group from timings of the batched SpMV
     tuned = []

@@ -324,13 +324,33 @@ class Batch:
         self.h = h
         self._keep = None  # the library copied everything
 
-    def set_rhs(self, k, source, psi=None):
-        """source, psi: [nrhs, n] (or [n, nrhs] from the generators: transposed here)."""
+    def _rhs_block(self, a, n, layout, what):
+        """[nrhs, n] contiguous FP64 copy of a right-hand-side block given in
+        ``layout``: "rhs_major" = [nrhs, n] (the C-ABI's), "cell_major" =
+        [n, nrhs] (the generators').  A 1-D array of n values is accepted when
+        nrhs == 1, where both layouts coincide."""
+        a = np.asarray(a, dtype=np.float64)
+        if a.ndim == 1 and self.nrhs == 1 and a.shape[0] == n:
+            return np.ascontiguousarray(a.reshape(1, n))
+        if layout == "rhs_major":
+            want = (self.nrhs, n)
+        elif layout == "cell_major":
+            want = (n, self.nrhs)
+        else:
+            raise ValueError(f"layout must be 'rhs_major' or 'cell_major', not {layout!r}")
+        if a.shape != want:
+            raise ValueError(f"{what} of system has shape {a.shape}, expected {want} "
+                             f"({layout})")
+        return np.ascontiguousarray(a.T if layout == "cell_major" else a)
+
+    def set_rhs(self, k, source, psi=None, layout="rhs_major"):
+        """source, psi (optional start): [nrhs, n] with layout "rhs_major"
+        (the C-ABI's), or [n, nrhs] with layout "cell_major" (the generators'
+        ``LduMatrix.source``); both arrays are read in the same layout."""
         n = self.systems[k].n
-        src = np.ascontiguousarray(np.asarray(source, dtype=np.float64).reshape(-1, n)
-                                   if np.asarray(source).shape[0] == self.nrhs
-                                   else np.asarray(source, dtype=np.float64).T)
-        ps = None if psi is None else np.ascontiguousarray(psi, dtype=np.float64)
+        src = self._rhs_block(source, n, layout, "source")
+        ps = None if psi is None else self._rhs_block(psi, n, layout, "psi")
+        assert src.size == self.nrhs * n and (ps is None or ps.size == self.nrhs * n)
         self.ctx._check(self.ctx.lib.hipfoam_batch_set_rhs(self.h, k, _ptr(src), _ptr(ps)),
                         "batch_set_rhs")
 

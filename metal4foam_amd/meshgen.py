@@ -213,6 +213,61 @@ def convection_diffusion(nx, ny, nz, seed, nrhs=1, name="convdiff", umax=2.0,
 # benchmark workloads
 # --------------------------------------------------------------------------
 
+# --------------------------------------------------------------------------
+# matrices whose Jacobi-scaled operator has exactly two eigenvalues
+# --------------------------------------------------------------------------
+#
+# With a constant diagonal d, D^-1 A = A / d.  When A is diagonalisable with
+# two distinct eigenvalues, every Krylov method whose iterates are the exact
+# polynomial recurrences of CG / BiCG terminates in two iterations (its
+# residual polynomial of degree 2 vanishes on both eigenvalues), whatever
+# right-hand side it is given.  A method with a wrong beta (steepest descent,
+# or BiCGStab without its alpha/omega factor) does not.  The same holds for
+# any preconditioner that is a polynomial in D^-1 A (Chebyshev-Jacobi).
+
+def pair_blocks(npairs, seed, diag=3.0, upper=-1.0, lower=None, name="pairs"):
+    """``npairs`` disjoint, identical 2x2 blocks [[d, u], [l, d]] on cells
+    (2k, 2k+1), one face each.  D^-1 A has the eigenvalues 1 +- sqrt(u l)/d
+    (u l > 0); ``lower=None`` is the symmetric block l = u.  Every row has one
+    face, so the mesh fits any row layout.  RHS ~ N(0, 1)."""
+    rng = _rng(seed)
+    n = 2 * npairs
+    low = np.arange(0, n, 2, dtype=I32)
+    upp = low + 1
+    own, los, loss = derived_addressing(n, low, upp)
+    up = np.full(npairs, float(upper))
+    lo = None if lower is None else np.full(npairs, float(lower))
+    d = np.full(n, float(diag))
+    b = rng.standard_normal(n)
+    return LduMatrix(n, low, upp, own, los, loss, d, up, lo, b, (n, 1, 1), name)
+
+
+def complete_graph(n, seed, shift=1.0, weights=None, name="complete"):
+    """Every pair of the n cells shares a face (n(n-1)/2 faces, owner-major
+    order), so every row has n-1 faces.  A = (n + shift) I - a b^T with
+    b_i = 1/a_i: the off-diagonals are upper_f = -a_l/a_u and
+    lower_f = -a_u/a_l (the complete-graph Laplacian plus shift when a = 1,
+    symmetric), the diagonal is n - 1 + shift.  A rank-one perturbation of a
+    multiple of I with b^T a = n, so A has the eigenvalues n + shift (n-1
+    times) and shift, and is diagonalisable.  ``weights`` = a (positive);
+    None gives the symmetric matrix.  RHS ~ N(0, 1)."""
+    rng = _rng(seed)
+    ii, jj = np.triu_indices(n, 1)
+    low = ii.astype(I32)
+    upp = jj.astype(I32)
+    own, los, loss = derived_addressing(n, low, upp)
+    d = np.full(n, n - 1.0 + shift)
+    if weights is None:
+        up = -np.ones(low.shape[0])
+        lo = None
+    else:
+        a = np.asarray(weights, dtype=np.float64)
+        up = -a[low] / a[upp]
+        lo = -a[upp] / a[low]
+    b = rng.standard_normal(n)
+    return LduMatrix(n, low, upp, own, los, loss, d, up, lo, b, (n, 1, 1), name)
+
+
 def config0():
     """2D Neumann Poisson 100x100x1: unit faces, cell 0 pinned, U[-1,1]."""
     return poisson(100, 100, 1, seed=1000, lognormal=False, name="poisson2d_100x100")

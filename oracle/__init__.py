@@ -63,48 +63,63 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def declare(L):
+    """Declares the signatures of an oracle library build (a CDLL)."""
+    A = C.POINTER(Addr)
+    L.orc_amul.argtypes = [A, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_tmul.argtypes = [A, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_sumA.argtypes = [A, _f64p, _f64p, _f64p, _f64p]
+    L.orc_norm_factor.argtypes = [A, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_norm_factor.restype = C.c_double
+    L.orc_spmv_f.argtypes = [A, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.orc_cheb_constants.argtypes = [C.c_int, C.c_double, C.c_double,
+                                     _f32p, _f32p, _f32p]
+    L.orc_cheb_apply_f.argtypes = [A, _f32p, _f32p, _f32p, _f32p, C.c_int,
+                                   C.c_double, C.c_double, _f32p, _f32p,
+                                   _f32p, _f32p]
+    L.orc_cheb_apply_d.argtypes = [A, _f64p, _f64p, _f64p, _f64p, C.c_int,
+                                   C.c_double, C.c_double, _f64p, _f64p,
+                                   _f64p, _f64p]
+    L.orc_inner_pcg_f.argtypes = [A, _f64p, _f64p, _f64p, C.c_double,
+                                  C.c_int, C.c_int, C.c_int, C.c_double,
+                                  C.c_double, _f32p]
+    L.orc_inner_pcg_f.restype = InnerStats
+    L.orc_inner_bicgstab_f.argtypes = [A, _f64p, _f64p, _f64p, _f64p,
+                                       C.c_double, C.c_int, _f32p]
+    L.orc_inner_bicgstab_f.restype = InnerStats
+    L.orc_gershgorin.argtypes = [A, _f64p, _f64p]
+    L.orc_gershgorin.restype = C.c_double
+    L.orc_refined_solve.argtypes = [A, _f64p, _f64p, _f64p, C.c_int,
+                                    C.POINTER(Controls), _f64p, _f64p,
+                                    C.POINTER(Perf)]
+    L.orc_pcg.argtypes = [A, _f64p, _f64p, C.c_int, C.c_int, C.c_double,
+                          C.c_double, C.c_double, C.c_double, C.c_int,
+                          C.c_int, _f64p, _f64p, C.POINTER(Perf)]
+    L.orc_pbicgstab.argtypes = [A, _f64p, _f64p, _f64p, C.c_int,
+                                C.c_double, C.c_double, C.c_int, C.c_int,
+                                _f64p, _f64p, C.POINTER(Perf)]
+    L.orc_adjoint_grads.argtypes = [A, _f64p, _f64p, _f64p, _f64p, _f64p]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
-        L = C.CDLL(_LIB)
-        A = C.POINTER(Addr)
-        L.orc_amul.argtypes = [A, _f64p, _f64p, _f64p, _f64p, _f64p]
-        L.orc_tmul.argtypes = [A, _f64p, _f64p, _f64p, _f64p, _f64p]
-        L.orc_sumA.argtypes = [A, _f64p, _f64p, _f64p, _f64p]
-        L.orc_norm_factor.argtypes = [A, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
-        L.orc_norm_factor.restype = C.c_double
-        L.orc_spmv_f.argtypes = [A, _f32p, _f32p, _f32p, _f32p, _f32p]
-        L.orc_cheb_constants.argtypes = [C.c_int, C.c_double, C.c_double,
-                                         _f32p, _f32p, _f32p]
-        L.orc_cheb_apply_f.argtypes = [A, _f32p, _f32p, _f32p, _f32p, C.c_int,
-                                       C.c_double, C.c_double, _f32p, _f32p,
-                                       _f32p, _f32p]
-        L.orc_cheb_apply_d.argtypes = [A, _f64p, _f64p, _f64p, _f64p, C.c_int,
-                                       C.c_double, C.c_double, _f64p, _f64p,
-                                       _f64p, _f64p]
-        L.orc_inner_pcg_f.argtypes = [A, _f64p, _f64p, _f64p, C.c_double,
-                                      C.c_int, C.c_int, C.c_int, C.c_double,
-                                      C.c_double, _f32p]
-        L.orc_inner_pcg_f.restype = InnerStats
-        L.orc_inner_bicgstab_f.argtypes = [A, _f64p, _f64p, _f64p, _f64p,
-                                           C.c_double, C.c_int, _f32p]
-        L.orc_inner_bicgstab_f.restype = InnerStats
-        L.orc_gershgorin.argtypes = [A, _f64p, _f64p]
-        L.orc_gershgorin.restype = C.c_double
-        L.orc_refined_solve.argtypes = [A, _f64p, _f64p, _f64p, C.c_int,
-                                        C.POINTER(Controls), _f64p, _f64p,
-                                        C.POINTER(Perf)]
-        L.orc_pcg.argtypes = [A, _f64p, _f64p, C.c_int, C.c_int, C.c_double,
-                              C.c_double, C.c_double, C.c_double, C.c_int,
-                              C.c_int, _f64p, _f64p, C.POINTER(Perf)]
-        L.orc_pbicgstab.argtypes = [A, _f64p, _f64p, _f64p, C.c_int,
-                                    C.c_double, C.c_double, C.c_int, C.c_int,
-                                    _f64p, _f64p, C.POINTER(Perf)]
-        L.orc_adjoint_grads.argtypes = [A, _f64p, _f64p, _f64p, _f64p, _f64p]
-        _lib = L
+        _lib = declare(C.CDLL(_LIB))
     return _lib
+
+
+def swap_library(L):
+    """Makes every function of this module call the oracle build L (a CDLL
+    passed through declare()) and returns the one it replaces.  Used by the
+    mutation tests, which load deliberately broken builds of ldu_oracle.c to
+    show that the pinning checks reject them."""
+    global _lib
+    prev = lib()
+    _lib = L
+    return prev
 
 
 def _p(a, t):

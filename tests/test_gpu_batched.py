@@ -35,7 +35,7 @@ def test_batched_matches_single_system_solves(ctx, orc, kind, solver):
     systems = _systems(kind, [9, 14, 6, 11], nrhs, seed=300)
     batch = H.Batch(ctx, solver, systems, nrhs)
     for k, m in enumerate(systems):
-        batch.set_rhs(k, m.source)
+        batch.set_rhs(k, m.source, layout="cell_major")
     ctl = H.default_controls(maxRestarts=6)
     perfs = batch.solve(ctl)
     for k, m in enumerate(systems):
@@ -65,7 +65,7 @@ def test_batched_uneven_sizes_and_repeat(ctx, orc):
     systems = _systems("sym", [5, 17, 8], nrhs, seed=400)
     batch = H.Batch(ctx, H.PCG, systems, nrhs)
     for k, m in enumerate(systems):
-        batch.set_rhs(k, m.source)
+        batch.set_rhs(k, m.source, layout="cell_major")
     first = batch.solve()
     sol1 = [batch.psi(k) for k in range(len(systems))]
     # resolve from zero: same result bit for bit (deterministic reductions)
@@ -108,7 +108,7 @@ def test_segments_of_one_system_on_different_diagonals(ctx, schedule):
     systems = [m0, m1]
     batch = H.Batch(ctx, H.PCG, systems, nrhs)
     for k, m in enumerate(systems):
-        batch.set_rhs(k, m.source)
+        batch.set_rhs(k, m.source, layout="cell_major")
     ctl = H.default_controls(maxRestarts=8)
     perfs = batch.solve(ctl)
     pol0 = [perfs[r].polished for r in range(nrhs)]
@@ -148,7 +148,7 @@ def test_rhs_groups_give_identical_solves(ctx, kind, solver):
     systems = _systems(kind, [10, 7, 13], nrhs, seed=620)
     batch = H.Batch(ctx, solver, systems, nrhs)
     for k, m in enumerate(systems):
-        batch.set_rhs(k, m.source)
+        batch.set_rhs(k, m.source, layout="cell_major")
     ctl = H.default_controls(maxRestarts=6)
     ref_p, ref_psi = _solve_all(batch, systems, ctl)
     assert all(p.converged for p in ref_p)
@@ -174,7 +174,7 @@ def test_batch_autotune_keeps_results(ctx):
     systems = _systems("sym", [12, 9], nrhs, seed=640)
     batch = H.Batch(ctx, H.PCG, systems, nrhs)
     for k, m in enumerate(systems):
-        batch.set_rhs(k, m.source)
+        batch.set_rhs(k, m.source, layout="cell_major")
     ctl = H.default_controls(maxRestarts=6)
     ref_p, ref_psi = _solve_all(batch, systems, ctl)
     sched, group = batch.autotune()

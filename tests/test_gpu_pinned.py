@@ -71,7 +71,7 @@ def test_pinned_projection_batched(ctx):
                           rhs="normal", nrhs=nrhs) for s in (9, 13, 7)]
     b = H.Batch(ctx, H.PCG, systems, nrhs)
     for k, m in enumerate(systems):
-        b.set_rhs(k, m.source)
+        b.set_rhs(k, m.source, layout="cell_major")
     perfs = b.solve(H.default_controls(deflate=3, tolerance=1e-8, maxRestarts=8))
     for k, m in enumerate(systems):
         psi = b.psi(k)
