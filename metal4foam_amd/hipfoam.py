@@ -239,6 +239,22 @@ class Context:
                                              C.byref(perf)), "adjoint")
         return perf, lam, du, dl, dd, db
 
+    def adjoint_device(self, solver, diag, upper, lower, x, g, lam, d_upper, d_lower, d_diag,
+                       d_source, controls=None, precision=FP32):
+        """hipfoam_adjoint_device: every array argument is an object with a
+        ``ptr`` (a device address, or None to skip that gradient)."""
+        controls = controls or default_controls()
+        perf = Performance()
+
+        def p(a):
+            return None if a is None else a.ptr
+
+        self._check(self.lib.hipfoam_adjoint_device(
+            self.h, solver, precision, p(diag), p(upper), p(lower), p(x), p(g), p(lam),
+            p(d_upper), p(d_lower), p(d_diag), p(d_source), C.byref(controls), C.byref(perf)),
+            "adjoint_device")
+        return perf
+
     # -- primitives -------------------------------------------------------
     def amul(self, x, precision=FP64):
         dt = np.float64 if precision == FP64 else np.float32

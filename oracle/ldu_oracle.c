@@ -59,7 +59,7 @@ typedef struct
 {
     double tolerance, relTol;
     int maxIter, minIter;
-    int deflate; /* 0 off, 1 on, 2 auto */
+    int deflate; /* 0 off, 1 on, 2 auto, 3 auto + projection along A 1 */
     int chebyshevOrder;
     double chebyshevRatio;
     int maxRestarts;
@@ -633,7 +633,8 @@ int orc_refined_solve(const orc_addr* a, const double* diag,
 
         /* :167-168 */
         orc_sumA(a, diag, upper, lower, u);
-        /* :170-190 */
+        /* :170-190; deflate 3 detects like auto (2) */
+        const int pinnedProj = (ctl->deflate == 3);
         int deflate = 0;
         if (!useBiCG && ctl->deflate != 0)
         {
@@ -684,14 +685,27 @@ int orc_refined_solve(const orc_addr* a, const double* diag,
         double prevRes = perf->initialResidual;
         for (outer = 0; outer <= ctl->maxRestarts; ++outer)
         {
-            /* :278-287 */
+            /* :278-287; deflate 3 (no reference counterpart) projects the
+             * inner right-hand side along u = A 1 instead of the constant:
+             * r - (sum r / sum u) u has no component the deflated operator
+             * cannot produce, whatever sum(b) is */
             const double* rPtr = rA;
             if (deflate)
             {
                 double mean = 0;
                 for (int32_t i = 0; i < n; ++i) mean += rA[i];
-                mean /= n;
-                for (int32_t i = 0; i < n; ++i) rin[i] = rA[i] - mean;
+                if (pinnedProj)
+                {
+                    double su = 0;
+                    for (int32_t i = 0; i < n; ++i) su += u[i];
+                    const double w = su != 0.0 ? mean / su : 0.0;
+                    for (int32_t i = 0; i < n; ++i) rin[i] = rA[i] - w * u[i];
+                }
+                else
+                {
+                    mean /= n;
+                    for (int32_t i = 0; i < n; ++i) rin[i] = rA[i] - mean;
+                }
                 rPtr = rin;
             }
             /* :289-311 */

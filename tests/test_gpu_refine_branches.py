@@ -46,7 +46,11 @@ def bail_case():
     return m
 
 
-CASES = [(polish_case, 2, (1, 1)), (polish_case_positive_faces, 1, (1, 1)),
+# (case, deflate control, (deflated, polished) the oracle takes); deflate 3
+# (projection along A 1, no reference counterpart) solves the RHS with a mean
+# without polishing
+CASES = [(polish_case, 2, (1, 1)), (polish_case, 3, (1, 0)),
+         (polish_case_positive_faces, 1, (1, 1)),
          (polish_case_positive_faces, 2, (1, 1)), (bail_case, 1, (0, 0)),
          (bail_case, 2, (0, 0))]
 

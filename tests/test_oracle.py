@@ -306,3 +306,29 @@ def test_adjoint_grads_match_finite_differences(orc):
         lm = loss(m)
         m.diag[i] = old
         assert (lp - lm) / (2 * h) == pytest.approx(dd[i], rel=1e-5, abs=1e-9)
+
+
+def test_refined_solve_pinned_projection(orc):
+    """deflate 3 (no reference counterpart): the inner right-hand side is
+    projected along u = A 1, r - (sum r / sum u) u, which sums to zero and so
+    lies in the range of the deflated (singular, symmetric) operator whatever
+    sum(b) is.  On a pinned Neumann Poisson with a RHS of non-zero sum it
+    converges without polishing, where the constant projection (deflate 2)
+    stalls until the polish; both reach the dense solution."""
+    m = mg.poisson(12, 10, 8, seed=51, lognormal=True, zero_mean_rhs=False, rhs="normal")
+    assert abs(m.source.sum()) > 1.0
+    x = np.linalg.solve(mg.dense(m), m.source)
+    ctl = dict(tolerance=1e-9, maxRestarts=8, maxIter=20000)
+    psi3, p3 = orc.refined_solve(m, ctl=orc.controls(deflate=3, **ctl))
+    psi2, p2 = orc.refined_solve(m, ctl=orc.controls(deflate=2, **ctl))
+    assert p3.converged and p3.deflated and not p3.polished
+    assert p2.converged and p2.deflated and p2.polished
+    assert p3.restarts < p2.restarts
+    for psi in (psi3, psi2):
+        assert np.abs(psi - x).max() <= 1e-8 * np.abs(x).max()
+    # on a zero-sum RHS the two projections coincide for the first inner solve
+    mz = mg.poisson(12, 10, 8, seed=51, lognormal=True, zero_mean_rhs=True, rhs="normal")
+    _, pz3 = orc.refined_solve(mz, ctl=orc.controls(deflate=3, **ctl))
+    _, pz2 = orc.refined_solve(mz, ctl=orc.controls(deflate=2, **ctl))
+    assert pz3.converged and pz2.converged and not pz3.polished
+    assert abs(pz3.nIterations - pz2.nIterations) <= max(3, pz2.nIterations // 10)
