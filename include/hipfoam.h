@@ -171,6 +171,30 @@ int32_t hipfoam_solve_device(hipfoam_context* ctx, int32_t solver,
                              const hipfoam_controls* controls,
                              hipfoam_performance* perf, double* residual);
 
+/* Allocation-free variant of hipfoam_solve_device.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, semantics and
+ * results as hipfoam_solve_device (launched eagerly, never through graphs),
+ * but it calls no hipMalloc/hipFree: every per-solve buffer (work vectors,
+ * staged FP64/FP32 operator, row-slot coefficients, reduction partials,
+ * Krylov and outer states) lives in a fixed 64 MiB scratch array that each
+ * GPU holds for the life of the process.  That holds about 324 bytes per
+ * cell of a hexahedral mesh (3 faces per cell; 360 with `lower`), i.e. up to
+ * roughly 200,000 cells; larger meshes return HIPFOAM_ERR_SIZE.  The mesh's
+ * addressing must have been uploaded by hipfoam_set_topology beforehand (it
+ * allocates, once per mesh).  The context's own staged operator is left as
+ * it was.  All calls on one GPU share its array, which keeps the previous
+ * call's contents (the call clears what it needs zeroed): overlapping calls
+ * on one GPU would race, so the library runs them one at a time per device
+ * within a process (a lock held for the duration of the call). */
+int32_t hipfoam_solve_device_noalloc(hipfoam_context* ctx, int32_t solver,
+                                     const double* diag, const double* upper,
+                                     const double* lower, double* psi,
+                                     const double* source,
+                                     const hipfoam_controls* controls,
+                                     hipfoam_performance* perf,
+                                     double* residual);
+
 /* Plain-FP64 GPU comparator (no reference counterpart; the "plain FP64"
  * side of the metric): OpenFOAM's PCG / PBiCGStab loop in FP64 on the exact
  * operator, with the same preconditioner (Chebyshev-Jacobi of

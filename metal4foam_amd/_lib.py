@@ -95,6 +95,9 @@ SIGNATURES = {
     "hipfoam_solve_device": (C.c_int32, [_P, C.c_int32, _vp, _vp, _vp, _vp, _vp,
                                          C.POINTER(Controls),
                                          C.POINTER(Performance), _vp]),
+    "hipfoam_solve_device_noalloc": (C.c_int32, [_P, C.c_int32, _vp, _vp, _vp, _vp, _vp,
+                                                 C.POINTER(Controls),
+                                                 C.POINTER(Performance), _vp]),
     "hipfoam_solve_fp64": (C.c_int32, [_P, C.c_int32, _vp, _vp, _vp, _vp, _vp,
                                        C.POINTER(Controls),
                                        C.POINTER(Performance), _vp]),

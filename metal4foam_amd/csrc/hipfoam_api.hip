@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <tuple>
@@ -49,10 +50,16 @@ struct FpTrapGuard {
   ~FpTrapGuard() { fesetenv(&saved); }
 };
 
+// Set while hipfoam_solve_device_noalloc runs on this thread: a buffer that
+// would have to be allocated fails instead (every buffer that call uses is
+// carved from the scratch array beforehand).
+thread_local bool t_noAlloc = false;
+
 template <typename T>
 struct Buf {
   T* p = nullptr;
   size_t cap = 0;
+  bool borrowed = false;  // points into the scratch array: never freed
   Buf() = default;
   Buf(const Buf&) = delete;
   Buf& operator=(const Buf&) = delete;
@@ -60,6 +67,7 @@ struct Buf {
   hipError_t ensure(size_t n) {
     if (n == 0) n = 1;
     if (p && n <= cap) return hipSuccess;
+    if (t_noAlloc || borrowed) return hipErrorOutOfMemory;
     release();
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T));
     if (e != hipSuccess) {
@@ -71,9 +79,10 @@ struct Buf {
     return hipSuccess;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     cap = 0;
+    borrowed = false;
   }
 };
 
@@ -184,6 +193,9 @@ struct hipfoam_context {
   KernelTimer timer;
   // pinned staging of large host arrays (created on first use)
   std::unique_ptr<HostStager> stager;
+  // device address of this device's scratch array (hipGetSymbolAddress at
+  // create), used by hipfoam_solve_device_noalloc only
+  char* scratchDev = nullptr;
 };
 
 namespace {
@@ -1329,6 +1341,211 @@ int adjoint_host(hipfoam_context* c, int solver, int precision, const double* di
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Scratch array of hipfoam_solve_device_noalloc
+// ---------------------------------------------------------------------------
+
+// Every per-solve device buffer of an allocation-free refined solve (work
+// vectors, staged FP64/FP32 operator, row-slot coefficients, reduction
+// partials, Krylov and outer states) is carved from this array.  Each GPU
+// the library runs on holds its own copy, for the life of the process.  In
+// host code `scratch` names a host-side stand-in: the device address comes
+// from hipGetSymbolAddress, once per device with that device current, and
+// only that address is passed to kernels and copies.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchInitMu;
+char* g_scratchDev[kMaxScratchDevices] = {};
+// one allocation-free solve at a time per device in this process: the calls
+// share the device's array
+std::mutex g_scratchUseMu[kMaxScratchDevices];
+
+// Called at hipfoam_create with `device` current.
+hipError_t scratch_address(int device, char** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratchInitMu);
+  if (!g_scratchDev[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratchDev[device] = static_cast<char*>(ptr);
+  }
+  *out = g_scratchDev[device];
+  return hipSuccess;
+}
+
+__global__ void k_zero_words(size_t nw, uint32_t* __restrict__ p) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nw) p[i] = 0u;
+}
+
+// Points buffers of the context into the scratch array (or, with base ==
+// nullptr, only counts the bytes) and puts them back on destruction.
+class ScratchCarve {
+ public:
+  ScratchCarve(char* base, size_t cap) : base_(base), cap_(cap) {}
+  ScratchCarve(const ScratchCarve&) = delete;
+  ScratchCarve& operator=(const ScratchCarve&) = delete;
+  ~ScratchCarve() { restore(); }
+
+  template <typename T>
+  void take(Buf<T>& b, size_t n) {
+    if (n == 0) n = 1;
+    const size_t bytes = (n * sizeof(T) + 255) & ~size_t(255);
+    if (base_ && used_ + bytes <= cap_) {
+      saved_.push_back({reinterpret_cast<void**>(&b.p), &b.cap, &b.borrowed,
+                        static_cast<void*>(b.p), b.cap, b.borrowed});
+      b.p = reinterpret_cast<T*>(base_ + used_);
+      b.cap = n;
+      b.borrowed = true;
+    }
+    used_ += bytes;
+  }
+  size_t used() const { return used_; }
+  void restore() {
+    for (auto it = saved_.rbegin(); it != saved_.rend(); ++it) {
+      *it->pp = it->p;
+      *it->pcap = it->cap;
+      *it->pb = it->borrowed;
+    }
+    saved_.clear();
+  }
+
+ private:
+  struct Saved {
+    void** pp;
+    size_t* pcap;
+    bool* pb;
+    void* p;
+    size_t cap;
+    bool borrowed;
+  };
+  char* base_;
+  size_t cap_;
+  size_t used_ = 0;
+  std::vector<Saved> saved_;
+};
+
+// The buffers refined_solve touches on a device-array solve, at the sizes it
+// asks for: ensure_vectors, stage_matrix (lower staged when given), the
+// partials and the state structs.  The states come first, so that they can
+// be cleared in one fill.
+void carve_refined_solve(hipfoam_context* c, bool withLower, ScratchCarve& sc) {
+  const size_t n = (size_t)c->n, nf = (size_t)c->nf;
+  sc.take(c->kst, 1);
+  sc.take(c->kst2, 1);
+  sc.take(c->ost, 1);
+  sc.take(c->partials, (size_t)c->pstride * 4);
+  sc.take(c->partials2, (size_t)c->pstride * 4);
+  Buf<float>* w32[] = {&c->w32.r,   &c->w32.r1,  &c->w32.p0,  &c->w32.p1,  &c->w32.wA,
+                       &c->w32.Ap,  &c->w32.delta, &c->w32.deltaBest, &c->w32.z0,
+                       &c->w32.z1,  &c->w32.w,   &c->w32.rA0, &c->w32.yA,  &c->w32.AyA,
+                       &c->w32.AyA1, &c->w32.sA, &c->w32.zA,  &c->w32.tA};
+  for (auto* b : w32) sc.take(*b, n);
+  Buf<double>* v64[] = {&c->psi0, &c->psi1, &c->b, &c->r64, &c->u64, &c->scratch64,
+                        &c->diag64, &c->rD64};
+  for (auto* b : v64) sc.take(*b, n);
+  sc.take(c->diag32, n);
+  sc.take(c->rD32, n);
+  sc.take(c->upper64, nf);
+  sc.take(c->lowerL64, nf);
+  sc.take(c->upper32, nf);
+  sc.take(c->lowerL32, nf);
+  if (withLower) {
+    sc.take(c->lower64, nf);
+    sc.take(c->lower32, nf);
+  }
+  if (c->slotOK && n > 0) {
+    sc.take(c->slotCoef32, (size_t)2 * kShortRow * n);
+    sc.take(c->slotCoef64, (size_t)2 * kShortRow * n);
+  }
+}
+
+// Context state an allocation-free solve changes besides its carved buffers;
+// put back afterwards, so the context's own staged operator stays valid.
+struct ContextStateSave {
+  hipfoam_context* c;
+  size_t w32n;
+  bool have64, sym64, have32, sym32, timerOn;
+  int graphChunk;
+  explicit ContextStateSave(hipfoam_context* cc)
+      : c(cc), w32n(cc->w32.n), have64(cc->have64), sym64(cc->sym64), have32(cc->have32),
+        sym32(cc->sym32), timerOn(cc->timer.on), graphChunk(cc->sched.graphChunk) {}
+  ~ContextStateSave() {
+    c->w32.n = w32n;
+    c->have64 = have64;
+    c->sym64 = sym64;
+    c->have32 = have32;
+    c->sym32 = sym32;
+    c->timer.on = timerOn;
+    c->sched.graphChunk = graphChunk;
+  }
+};
+
+struct NoAllocFlag {
+  NoAllocFlag() { t_noAlloc = true; }
+  ~NoAllocFlag() { t_noAlloc = false; }
+};
+
+int solve_device_noalloc(hipfoam_context* c, int solver, const double* diag,
+                         const double* upper, const double* lower, double* psi,
+                         const double* source, const hipfoam_controls* ctl,
+                         hipfoam_performance* perf, double* residual) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!c->scratchDev)
+    return fail(c, HIPFOAM_ERR_DEVICE, "no scratch array address on device " +
+                                           std::to_string(c->device));
+  {
+    ScratchCarve count(nullptr, 0);
+    carve_refined_solve(c, lower != nullptr, count);
+    if (count.used() > kScratchBytes)
+      return fail(c, HIPFOAM_ERR_SIZE,
+                  "allocation-free solve needs " + std::to_string(count.used()) +
+                      " bytes of scratch for " + std::to_string(c->n) + " cells and " +
+                      std::to_string(c->nf) + " faces; the array holds " +
+                      std::to_string(kScratchBytes));
+  }
+  std::lock_guard<std::mutex> lk(g_scratchUseMu[c->device]);
+  ContextStateSave keep(c);
+  rc = HIPFOAM_OK;
+  {
+    ScratchCarve sc(c->scratchDev, kScratchBytes);
+    carve_refined_solve(c, lower != nullptr, sc);
+    c->w32.n = (size_t)c->n;
+    c->sched.graphChunk = 0;  // graph instantiation allocates: eager launches
+    c->timer.on = false;
+    {
+      NoAllocFlag guard;
+      // the previous call's states are still in the array: clear them
+      const size_t stateBytes = (reinterpret_cast<char*>(c->partials.p) -
+                                 reinterpret_cast<char*>(c->kst.p));
+      const size_t nw = stateBytes / sizeof(uint32_t);
+      hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0,
+                         c->stream, nw, reinterpret_cast<uint32_t*>(c->kst.p));
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess)
+        rc = fail(c, HIPFOAM_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+      else
+        rc = refined_solve(c, solver, diag, upper, lower, psi, source, ctl, perf, residual,
+                           hipMemcpyDeviceToDevice);
+    }
+    // nothing may still run on the array when the next call takes it
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    if (rc == HIPFOAM_OK && es != hipSuccess)
+      rc = fail(c, HIPFOAM_ERR_DEVICE, std::string("hipStreamSynchronize: ") +
+                                           hipGetErrorString(es));
+  }
+  return rc;
+}
+
+}  // namespace
+
 // ===========================================================================
 // extern "C" exports
 // ===========================================================================
@@ -1385,6 +1602,8 @@ int32_t hipfoam_create(int32_t device, hipfoam_context** out) {
   (void)hipMemset(c->kst2.p, 0, sizeof(KState));
   (void)hipMemset(c->ost.p, 0, sizeof(OState));
   (void)hipDeviceSynchronize();
+  // the device address of this device's scratch array (allocation-free solve)
+  if (scratch_address(device, &c->scratchDev) != hipSuccess) c->scratchDev = nullptr;
   char buf[512];
   snprintf(buf, sizeof(buf), "hipfoam: device %d %s(%s, %d CUs, %.0f GiB HBM)", device,
            prop.name[0] ? (std::string(prop.name) + " ").c_str() : "",
@@ -1633,6 +1852,15 @@ int32_t hipfoam_solve_device(hipfoam_context* c, int32_t solver, const double* d
   FpTrapGuard fpe;
   return refined_solve(c, solver, diag, upper, lower, psi, source, controls, perf, residual,
                        hipMemcpyDeviceToDevice);
+}
+
+int32_t hipfoam_solve_device_noalloc(hipfoam_context* c, int32_t solver, const double* diag,
+                                     const double* upper, const double* lower, double* psi,
+                                     const double* source, const hipfoam_controls* controls,
+                                     hipfoam_performance* perf, double* residual) {
+  FpTrapGuard fpe;
+  return solve_device_noalloc(c, solver, diag, upper, lower, psi, source, controls, perf,
+                              residual);
 }
 
 int32_t hipfoam_solve_fp64(hipfoam_context* c, int32_t solver, const double* diag,

@@ -217,6 +217,19 @@ class Context:
                        None if residual is None else residual.ptr), "solve_device")
         return perf
 
+    def solve_device_noalloc(self, solver, diag, upper, lower, psi, source, controls=None,
+                             residual=None):
+        """hipfoam_solve_device_noalloc: the refined solve of solve_device with
+        every per-solve buffer in the library's fixed scratch array (no device
+        allocation); arguments are objects with a ``ptr`` (DeviceArray)."""
+        controls = controls or default_controls()
+        perf = Performance()
+        self._check(self.lib.hipfoam_solve_device_noalloc(
+            self.h, solver, diag.ptr, upper.ptr, None if lower is None else lower.ptr, psi.ptr,
+            source.ptr, C.byref(controls), C.byref(perf),
+            None if residual is None else residual.ptr), "solve_device_noalloc")
+        return perf
+
     def adjoint(self, solver, diag, upper, lower, x, g, controls=None, precision=FP32):
         """Transpose solve A^T lambda = g and the gradients of L through A x = b.
 
