@@ -46,10 +46,16 @@ WORKLOADS = {
                 "64 independent systems, cube sides U{20..80} (8k-512k cells), 32 lognormal "
                 "Poisson (hipPCG) + 32 convection-diffusion (hipPBiCGStab), 5 RHS each: "
                 "320 refined solves per step"),
+    "poisson256": ("config1", "pcg",
+                   "3D pressure Poisson 256^3 (16,777,216 cells), lognormal face "
+                   "coefficients, cell 0 pinned; hipPCG refined (past the MALL)"),
     "adjoint128": ("config1", "adjoint",
                    "adjoint of L = 0.5|x - x_t|^2 through the 128^3 lognormal Poisson: "
                    "forward hipPCG + transpose solve + face/diag/source gradients"),
 }
+
+
+WORKLOAD_ARGS = {"poisson256": (256,)}
 
 
 def log(*a):
@@ -91,51 +97,246 @@ def to_slot_bytes(ldu_bytes, n, nf, symmetric, patterned_rows):
             + slot_matrix_bytes(n, not symmetric, 4, patterned_rows))
 
 
-def cpu_baseline(m, solver, sample_iters):
-    """Oracle PCG(DIC) / PBiCGStab(DILU) in FP64 on one host core.
+def _omp_threads():
+    """Threads for the all-cores CPU line: OMP_NUM_THREADS when set (16 on
+    the MI355X boxes, their CPU share), else every core."""
+    v = os.environ.get("OMP_NUM_THREADS", "")
+    return int(v) if v.isdigit() and int(v) > 0 else (os.cpu_count() or 1)
 
-    sample_iters == 0: the full solve to 1e-6, timed, and its solution
-    returned so the GPU results can state their error against it.  Otherwise
-    a bounded number of iterations is timed and scaled to the iteration count
-    of the full solve (committed fixture written by the oracle); no solution.
-    Returns (baseline dict, x or None)."""
+
+def _relmax(a, b):
+    """relMax of two fields: max |a - b| / max |b| (the measure of the
+    reference's compare_fields.py:38-53, with b the reference field)."""
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def cpu_baseline(m, solver, sample_iters):
+    """CPU solves on this host, in FP64, built here with -O3 -march=native:
+
+    * the baseline proper: OpenFOAM PCG(DIC) / PBiCGStab(DILU), 1 core (DIC and
+      DILU sweeps are sequential; OpenFOAM runs serially here, as the
+      reference's benchmark does);
+    * PCG(diagonal) on all host cores (OpenMP, row-gather SpMV), PCG only;
+    * the CPU-vs-CPU floor: how far the diagonal-preconditioned CPU solve
+      lands from the DIC (DILU) one at the same 1e-6 tolerance.
+
+    sample_iters == 0: every solve runs to 1e-6 and is timed whole, and the
+    DIC (DILU) solution is returned so the GPU results can state their error
+    against it.  Otherwise a bounded number of iterations of the baseline is
+    timed and scaled to the iteration count of the full solve (committed
+    fixture written by the oracle); no solution, no floor.
+    Returns (baseline dict, x or None, floor dict or None)."""
+    import tempfile
+
     import oracle
 
-    fpath = os.path.join(ROOT, "tests", "golden", "cpu_baseline_iterations.json")
-    full = json.load(open(fpath)) if os.path.exists(fpath) else {}
-    key = m.name
-    max_iter = sample_iters if sample_iters > 0 else 20000
-    t0 = time.perf_counter()
-    if solver == "pcg":
-        x, perf = oracle.pcg(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=max_iter)
-    else:
-        x, perf = oracle.pbicgstab(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=max_iter)
-    dt = time.perf_counter() - t0
-    name = "PCG(DIC)" if solver == "pcg" else "PBiCGStab(DILU)"
-    done = perf.nIterations
-    if perf.converged:
-        est, sample = dt, (f"{name} FP64, 1 core, the full solve to 1e-6 timed "
-                           f"({done} iterations, final residual {perf.finalResidual:.3e})")
-    else:
-        iters_full = full.get(key, {}).get("iterations") or done
-        est = dt / max(done, 1) * iters_full
-        sample = (f"{name} FP64, 1 core, {done} iterations timed ({dt:.2f} s) and scaled to "
-                  f"the {iters_full} iterations the full solve to 1e-6 takes")
-        x = None
-    return {
-        "value": round(1e3 * est, 3),
-        "unit": "ms",
-        "cores": 1,
-        "kind": "port",
-        "sample": sample,
-        "cpu": platform.processor() or platform.machine(),
-        "nproc": os.cpu_count(),
-    }, x
+    oracle.lib()
+    native = oracle.build_native(tempfile.mkdtemp(prefix="hipfoam_cpu_"))
+    prev = oracle.swap_library(native) if native is not None else None
+    build = ("gcc -O3 -march=native -fopenmp, built on this host" if native is not None else
+             "the prebuilt oracle (-O3 -march=x86-64-v2): the native build failed")
+    try:
+        fpath = os.path.join(ROOT, "tests", "golden", "cpu_baseline_iterations.json")
+        full = json.load(open(fpath)) if os.path.exists(fpath) else {}
+        max_iter = sample_iters if sample_iters > 0 else 20000
+        t0 = time.perf_counter()
+        if solver == "pcg":
+            x, perf = oracle.pcg(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=max_iter)
+        else:
+            x, perf = oracle.pbicgstab(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=max_iter)
+        dt = time.perf_counter() - t0
+        name = "PCG(DIC)" if solver == "pcg" else "PBiCGStab(DILU)"
+        done = perf.nIterations
+        if perf.converged:
+            est, sample = dt, (f"{name} FP64, 1 core, the full solve to 1e-6 timed "
+                               f"({done} iterations, final residual {perf.finalResidual:.3e})")
+        else:
+            iters_full = full.get(m.name, {}).get("iterations") or done
+            est = dt / max(done, 1) * iters_full
+            sample = (f"{name} FP64, 1 core, {done} iterations timed ({dt:.2f} s) and scaled "
+                      f"to the {iters_full} iterations the full solve to 1e-6 takes")
+            x = None
+        out = {
+            "value": round(1e3 * est, 3),
+            "unit": "ms",
+            "cores": 1,
+            "kind": "port",
+            "sample": sample,
+            "build": build,
+            "cpu": _cpu_model(),
+            "nproc": os.cpu_count(),
+        }
+        floor = None
+        if x is not None:
+            if solver == "pcg":
+                th = _omp_threads()
+                t0 = time.perf_counter()
+                xd, pd = oracle.pcg_diag_par(m, tol=1e-6, maxIter=20000, threads=th)
+                dtd = time.perf_counter() - t0
+                out["all_cores"] = {
+                    "value": round(1e3 * dtd, 3), "unit": "ms",
+                    "cores": th if native is not None else 1,
+                    "solver": "PCG(diagonal) FP64, OpenMP row-gather SpMV",
+                    "iterations": pd.nIterations, "converged": bool(pd.converged)}
+                other = "PCG(diagonal)"
+            else:
+                xd, pd = oracle.pbicgstab(m, precon=oracle.PRE_DIAGONAL, tol=1e-6,
+                                          maxIter=20000)
+                other = "PBiCGStab(diagonal)"
+            if pd.converged:
+                floor = {"cpu_vs_cpu_floor_relmax": _relmax(xd, x),
+                         "compared": f"{other} vs {name}, both FP64 CPU to 1e-6"}
+        return out, x, floor
+    finally:
+        if prev is not None:
+            oracle.swap_library(prev)
+
+
+def _cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def rel_diff(a, b):
     """max |a - b| / max |b|"""
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _traffic(tag, kernel_layout, patterned):
+    """Calibrated HBM bytes per launch from profiles/traffic_<tag>.json, when
+    those counters were taken on the layout this run uses."""
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{tag}.json")
+    if not os.path.exists(tfile):
+        return None, None
+    tj = json.load(open(tfile))
+    if tj.get("layout", "LDU face") != kernel_layout:
+        return None, None
+    if kernel_layout == "row-slot" and tj.get("patterned_rows") != patterned:
+        return None, None
+    return tj.get("bytes_per_launch"), os.path.relpath(tfile, ROOT)
+
+
+def _entry(kernel, layout, nbytes, ref_bytes, launches, secs, traffic=None, tfile=None):
+    avg = secs / max(launches, 1)
+    gbs = nbytes / avg / 1e9 if launches else 0.0
+    return {
+        "kernel": kernel,
+        "layout": layout,
+        "bytes_per_launch": int(nbytes),
+        "reference_format_bytes": int(ref_bytes),
+        "launches": int(launches),
+        "avg_launch_us": round(1e6 * avg, 3),
+        "GB_s": round(gbs, 1),
+        "frac": round(gbs / HBM_PEAK_GBS, 4),
+        "reference_format_GB_s": round(ref_bytes / avg / 1e9, 1) if launches else 0.0,
+        "traffic": traffic,
+        "traffic_frac_of_algorithmic": (round(traffic / nbytes, 3) if traffic else None),
+        "traffic_source": tfile,
+    }
+
+
+def measure_kernels(H, ctx, m, solver, spmv_reps, tag, solve_fn):
+    """Dominant solve kernel and plain FP32 SpMV of one mesh on the context's
+    current schedule, timed with per-launch HIP events on the library stream.
+
+    PCG: the interior Chebyshev step k_cheb_step<float, false> (class 0);
+    BiCGStab: the two SpMVs fused with their vector updates (class 2).
+    Bytes are the kernel's own algorithmic bytes on the layout it runs (the
+    row-slot layout, or the LDU face arrays of SURVEY-style pricing when the
+    schedule is 0 or the mesh falls back to it)."""
+    sym = m.lower is None
+    D = H.DeviceArray
+    ctx.set_kernel_timing(True, 0 if solver == "pcg" else 2)
+    perf = solve_fn()
+    nl, secs = ctx.kernel_timing()
+    ctx.set_kernel_timing(False, 0)
+    slots = ctx.active_schedule() == 2
+    patterned, _ = ctx.slot_layout()
+    layout = "row-slot" if slots else "LDU face"
+    if solver == "pcg":
+        ref_kbytes = cheb_step_bytes(m.n, m.nfaces, sym)
+        kname = "k_cheb_step<float, false> (interior Chebyshev step fused with its SpMV)"
+    else:
+        # mean of k_bicg_py_spmv (reads r, rD, p, AyA, rA0; writes p, yA, AyA)
+        # and k_bicg_sz_spmv (reads r, rD, AyA; writes sA, zA, tA)
+        ref_kbytes = spmv_bytes(m.n, m.nfaces, sym, extra_vectors=5)
+        kname = ("k_bicg_py_spmv / k_bicg_sz_spmv (BiCGStab SpMVs fused with their "
+                 "vector updates)")
+    kbytes = to_slot_bytes(ref_kbytes, m.n, m.nfaces, sym, patterned) if slots else ref_kbytes
+    ktraf, ktfile = _traffic(tag, layout, patterned)
+    solve_k = _entry(kname, layout, kbytes, ref_kbytes, nl, secs, ktraf, ktfile)
+    solve_k["patterned_rows"] = patterned if slots else None
+    spmv = None
+    if spmv_reps > 0:
+        x = D.from_host(ctx, np.random.default_rng(0).standard_normal(m.n).astype(np.float32))
+        y = D(ctx, (m.n,), np.float32)
+        ctx.set_kernel_timing(True, 5)
+        for _ in range(spmv_reps):
+            ctx.amul_device(x, y, H.FP32)
+        ns, ssecs = ctx.kernel_timing()
+        ctx.set_kernel_timing(False, 0)
+        ref_sbytes = spmv_bytes(m.n, m.nfaces, sym)
+        sbytes = to_slot_bytes(ref_sbytes, m.n, m.nfaces, sym, patterned) if slots else \
+            ref_sbytes
+        straf, stfile = _traffic(tag + "_spmv", layout, patterned)
+        spmv = _entry("k_spmv<float> (plain FP32 SpMV of the inner operator)", layout, sbytes,
+                      ref_sbytes, ns, ssecs, straf, stfile)
+        x.free()
+        y.free()
+    return perf, solve_k, spmv
+
+
+def extra_mesh(H, mg, args, gen_name, gen_args, solver, tag, what, schedule=None):
+    """Kernel rooflines (and the time to 1e-6) of another mesh, on a context
+    of its own: the past-MALL mesh, the forced LDU face schedule, the
+    convection-diffusion BiCGStab kernels."""
+    m = getattr(mg, gen_name)(*gen_args)
+    ctx = H.Context(0)
+    ctx.set_topology(m)
+    if schedule is not None:
+        ctx.set_schedule(spmvSchedule=schedule)
+    sym = m.lower is None
+    kind = H.PCG if solver == "pcg" else H.PBICGSTAB
+    D = H.DeviceArray
+    d = [D.from_host(ctx, a) for a in (m.diag, m.upper)]
+    lo = None if sym else D.from_host(ctx, m.lower)
+    src = D.from_host(ctx, m.source)
+    psi = D.from_host(ctx, np.zeros(m.n))
+    zero = D.from_host(ctx, np.zeros(m.n))
+    ctl = H.default_controls(maxRestarts=8, maxIter=20000)
+
+    def solve():
+        psi.copy_from(zero)
+        return ctx.solve_device(kind, d[0], d[1], lo, psi, src, ctl)
+
+    solve()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    p = solve()
+    ctx.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0)
+    _, solve_k, spmv = measure_kernels(H, ctx, m, solver, args.spmv_reps, tag, solve)
+    rsum, _ = numpy_residual(m, psi.download())
+    out = {"what": what, "mesh": m.name, "cells": m.n, "faces": m.nfaces,
+           "schedule": ctx.get_schedule().as_dict(),
+           "active_spmv_schedule": ctx.active_schedule(),
+           "solve_ms": round(ms, 3), "iterations": p.nIterations,
+           "converged": bool(p.converged and rsum / p.normFactor < 1e-6),
+           "solve_kernel": solve_k, "spmv": spmv}
+    log(f"{tag}: {m.name} solve {ms:.2f} ms ({p.nIterations} it.); {solve_k['kernel'][:24]} "
+        f"{solve_k['avg_launch_us']} us {100 * solve_k['frac']:.1f}% ({solve_k['layout']}); "
+        f"SpMV {spmv['avg_launch_us'] if spmv else '-'} us "
+        f"{100 * spmv['frac'] if spmv else 0:.1f}%")
+    for a in (*d, lo, src, psi, zero):
+        if a is not None:
+            a.free()
+    ctx.close()
+    return out
 
 
 def main():
@@ -148,18 +349,29 @@ def main():
     ap.add_argument("--host-steps", type=int, default=2,
                     help="solves timed with host arrays (PCIe copies included)")
     ap.add_argument("--cpu-iters", type=int, default=0,
-                    help="0: time the full CPU solve and compare solutions; N: time N "
-                         "iterations and scale")
+                    help="0: time the full CPU solves and compare solutions; N: time N "
+                         "iterations of the baseline and scale")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--spmv-tune", type=int, default=1,
                     help="also time the plain SpMV on the launch shape hipfoam_autotune picks")
+    ap.add_argument("--spmv-schedule", type=int, default=2, choices=[0, 1, 2],
+                    help="SpMV schedule of the solve (2: row slots where the mesh allows "
+                         "them, the default; 0: LDU face arrays)")
     ap.add_argument("--cold-reps", type=int, default=10,
                     help="SpMV launches timed after a MALL-flushing copy")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="poisson128: also measure the past-MALL 256^3 mesh, the LDU face "
+                         "schedule and the 96^3 BiCGStab kernels")
     ap.add_argument("--adjoint-precision", choices=["fp32", "fp64"], default="fp32",
                     help="adjoint128: transpose solve refined (FP32 inner) or plain FP64")
     ap.add_argument("--fd-faces", type=int, default=0,
                     help="adjoint128: finite-difference check on this many faces and cells")
+    ap.add_argument("--batch-rhs", choices=["as-specified", "zero-mean"],
+                    default="as-specified",
+                    help="batch64: Poisson right-hand sides as drawn (N(0,1), solved with "
+                         "the pinned projection, deflate 3) or projected to zero mean "
+                         "(solved with the reference's deflation)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -176,7 +388,8 @@ def main():
         return run_batch(args, H, mg, desc)
     if solver == "adjoint":
         return run_adjoint(args, H, mg, desc)
-    m = getattr(mg, gen)()
+    gen_args = WORKLOAD_ARGS.get(args.workload, ())
+    m = getattr(mg, gen)(*gen_args)
     sym = m.lower is None
     kind = H.PCG if solver == "pcg" else H.PBICGSTAB
     log(f"workload {m.name}: n={m.n} faces={m.nfaces}")
@@ -184,6 +397,8 @@ def main():
     ctx = H.Context(0)
     log(ctx.description)
     ctx.set_topology(m)
+    if args.spmv_schedule != 2:
+        ctx.set_schedule(spmvSchedule=args.spmv_schedule)
     D = H.DeviceArray
     d_diag = D.from_host(ctx, m.diag)
     d_upper = D.from_host(ctx, m.upper)
@@ -218,7 +433,8 @@ def main():
     if not (perf.converged and true_res < 1e-6):
         log("ERROR: refined solve did not reach 1e-6")
 
-    # ---- PCIe-inclusive: host arrays in and out, as the OpenFOAM plugin passes them
+    # ---- PCIe-inclusive: host arrays in and out, as the OpenFOAM plugin passes
+    # them, with the plugin's per-solve hipfoam_set_topology (a content check)
     host_ms = None
     if args.host_steps > 0:
         hpsi = np.zeros(m.n)
@@ -227,9 +443,24 @@ def main():
         t0 = time.perf_counter()
         for _ in range(args.host_steps):
             hpsi[:] = 0.0
+            ctx.set_topology(m)
             ctx.solve(kind, m.diag, m.upper, lower_h, hpsi, m.source, ctl)
         host_ms = 1e3 * (time.perf_counter() - t0) / args.host_steps
-        log(f"refined with host arrays (PCIe included): {host_ms:.3f} ms/solve")
+        log(f"refined with host arrays (PCIe and topology check included): "
+            f"{host_ms:.3f} ms/solve")
+
+    # ---- cost of hipfoam_set_topology on an unchanged mesh, by check mode
+    topo = {}
+    for name, mode in (("content", H.TOPO_CONTENT), ("pointers", H.TOPO_POINTERS)):
+        ctx.set_topology_check(mode)
+        ctx.set_topology(m)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            ctx.set_topology(m)
+        topo[name] = round(1e3 * (time.perf_counter() - t0) / 5, 4)
+    ctx.set_topology_check(H.TOPO_CONTENT)
+    log(f"hipfoam_set_topology on an unchanged mesh: content hash {topo['content']} ms, "
+        f"pointers {topo['pointers']} ms")
 
     # ---- plain FP64 comparator ----------------------------------------------------
     fp64_ms = None
@@ -248,50 +479,25 @@ def main():
             f"res={fp64_perf.finalResidual:.3e}")
 
     # ---- kernel roofline: per-launch HIP events on the library stream -------------
-    dom_class = 0 if solver == "pcg" else 2
-    ctx.set_kernel_timing(True, dom_class)
-    step()
-    nl, secs = ctx.kernel_timing()
-    ctx.set_kernel_timing(False, 0)
-    layout = ctx.active_schedule()
-    slots = layout == 2
-    patterned, _ = ctx.slot_layout()
-    if solver == "pcg":
-        ref_kbytes = cheb_step_bytes(m.n, m.nfaces, sym)
-        kname = "k_cheb_step<float, false> (interior Chebyshev step fused with its SpMV)"
-    else:
-        # mean of k_bicg_py_spmv (reads r, rD, p, AyA, rA0; writes p, yA, AyA)
-        # and k_bicg_sz_spmv (reads r, rD, AyA; writes sA, zA, tA)
-        ref_kbytes = spmv_bytes(m.n, m.nfaces, sym, extra_vectors=5)
-        kname = "k_bicg_py_spmv / k_bicg_sz_spmv (BiCGStab SpMVs fused with their vector updates)"
-    kbytes = to_slot_bytes(ref_kbytes, m.n, m.nfaces, sym, patterned) if slots else ref_kbytes
-    kavg = secs / max(nl, 1)
-    kgbs = kbytes / kavg / 1e9 if nl else 0.0
-    ref_kgbs = ref_kbytes / kavg / 1e9 if nl else 0.0
-    log(f"dominant kernel {kname} ({'row-slot' if slots else 'LDU face'} layout): {nl} "
-        f"launches, avg {1e6 * kavg:.2f} us, {kgbs:.0f} GB/s ({100 * kgbs / HBM_PEAK_GBS:.1f}% "
-        f"of HBM peak); LDU-format equivalent {ref_kgbs:.0f} GB/s")
-
-    # plain FP32 SpMV of the inner operator (the metric's SpMV GB/s)
-    x = D.from_host(ctx, np.random.default_rng(0).standard_normal(m.n).astype(np.float32))
-    y = D(ctx, (m.n,), np.float32)
-    ctx.set_kernel_timing(True, 5)
-    for _ in range(args.spmv_reps):
-        ctx.amul_device(x, y, H.FP32)
-    ns, ssecs = ctx.kernel_timing()
-    ctx.set_kernel_timing(False, 0)
-    ref_sbytes = spmv_bytes(m.n, m.nfaces, sym)
-    sbytes = to_slot_bytes(ref_sbytes, m.n, m.nfaces, sym, patterned) if slots else ref_sbytes
-    savg = ssecs / max(ns, 1)
-    sgbs = sbytes / savg / 1e9 if ns else 0.0
-    ref_sgbs = ref_sbytes / savg / 1e9 if ns else 0.0
-    log(f"plain FP32 SpMV: avg {1e6 * savg:.2f} us, {sgbs:.0f} GB/s "
-        f"(LDU-format equivalent {ref_sgbs:.0f} GB/s)")
+    tag = args.workload + ("" if args.spmv_schedule == 2 else f"_schedule{args.spmv_schedule}")
+    _, solve_k, spmv = measure_kernels(H, ctx, m, solver, args.spmv_reps, tag, step)
+    slots = solve_k["layout"] == "row-slot"
+    log(f"dominant kernel {solve_k['kernel']} ({solve_k['layout']} layout): "
+        f"{solve_k['launches']} launches, avg {solve_k['avg_launch_us']} us, "
+        f"{solve_k['GB_s']:.0f} GB/s ({100 * solve_k['frac']:.1f}% of HBM peak); LDU-format "
+        f"equivalent {solve_k['reference_format_GB_s']:.0f} GB/s")
+    if spmv:
+        log(f"plain FP32 SpMV: avg {spmv['avg_launch_us']} us, {spmv['GB_s']:.0f} GB/s "
+            f"(LDU-format equivalent {spmv['reference_format_GB_s']:.0f} GB/s)")
+    sbytes = spmv["bytes_per_launch"] if spmv else 0
+    ref_sbytes = spmv["reference_format_bytes"] if spmv else 0
 
     # the same SpMV with the 256 MB MALL (Infinity Cache) flushed before every
     # launch by a 2 GiB device-to-device copy
+    x = D.from_host(ctx, np.random.default_rng(0).standard_normal(m.n).astype(np.float32))
+    y = D(ctx, (m.n,), np.float32)
     cold = None
-    if args.cold_reps > 0:
+    if args.cold_reps > 0 and spmv:
         big = 256 * 1024 * 1024
         fa, fb = D(ctx, (big,), np.float64), D(ctx, (big,), np.float64)
         ctx.set_kernel_timing(True, 5)
@@ -337,33 +543,52 @@ def main():
         log(f"plain FP32 SpMV on the autotuned shape {tuned['schedule']}: avg "
             f"{1e6 * tavg:.2f} us, {tuned['GB_s']:.0f} GB/s "
             f"({100 * tuned['frac']:.1f}% of HBM peak)")
+    x.free()
+    y.free()
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(tfile):
-        tj = json.load(open(tfile))
-        # counters measured on the layout this run uses only
-        same = tj.get("layout", "LDU face") == ("row-slot" if slots else "LDU face")
-        if same and slots:
-            same = tj.get("patterned_rows") == patterned
-        if same:
-            traffic = tj.get("bytes_per_launch")
+    # ---- more rooflines: a mesh past the 256 MB MALL, the LDU face schedule
+    # priced on the LDU arrays, the BiCGStab kernels of the 96^3 workload
+    extras = None
+    if args.extras and args.workload == "poisson128" and args.spmv_schedule == 2:
+        extras = {
+            "past_mall": extra_mesh(H, mg, args, "config1", (256,), "pcg", "poisson256",
+                                    "256^3 lognormal Poisson (working set ~0.7 GB, past the "
+                                    "256 MB MALL): served from HBM"),
+            "ldu_face_schedule": extra_mesh(H, mg, args, "config1", (), "pcg",
+                                            "poisson128_schedule0",
+                                            "128^3 with SpMV schedule 0 forced: the LDU face "
+                                            "arrays (ownerStart/losortStart, upper, lowerL, "
+                                            "uppAddr, losortLow), priced on their own bytes",
+                                            schedule=0),
+            "convdiff96_bicgstab": extra_mesh(H, mg, args, "config2", (), "bicg", "convdiff96",
+                                              "96^3 convection-diffusion, refined "
+                                              "hipPBiCGStab: the BiCGStab SpMV kernels"),
+        }
 
     cpu = None
     errors = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu, x_cpu = cpu_baseline(m, solver, args.cpu_iters)
-        log(f"cpu baseline: {cpu['value']} ms ({cpu['sample']})")
+        cpu, x_cpu, floor = cpu_baseline(m, solver, args.cpu_iters)
+        log(f"cpu baseline: {cpu['value']} ms ({cpu['sample']}; {cpu['build']})")
+        if cpu.get("all_cores"):
+            log(f"cpu all cores: {cpu['all_cores']}")
         if x_cpu is not None:
             # both solutions meet the same 1e-6 normalised residual; their
-            # difference is what that tolerance leaves, not an error of either
-            errors = {"refined_vs_cpu_fp64_max_rel": rel_diff(psi, x_cpu)}
+            # difference is what that tolerance leaves, not an error of either.
+            # The floor is the same measure between two CPU solvers.
+            errors = {"refined_vs_cpu_fp64_relmax": _relmax(psi, x_cpu)}
             if psi64 is not None:
-                errors["plain_fp64_vs_cpu_fp64_max_rel"] = rel_diff(psi64, x_cpu)
+                errors["plain_fp64_vs_cpu_fp64_relmax"] = _relmax(psi64, x_cpu)
+            if floor:
+                errors.update(floor)
             log("solution vs CPU FP64 solve: " +
-                ", ".join(f"{k} {v:.3e}" for k, v in errors.items()))
+                ", ".join(f"{k} {v:.3e}" if isinstance(v, float) else f"{k} {v}"
+                          for k, v in errors.items()))
 
+    roof = dict(solve_k)
+    roof.update({"bound": "hbm", "achieved": solve_k["GB_s"], "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s"})
     out = {
         "metric": METRIC,
         "value": round(ms, 4),
@@ -387,34 +612,9 @@ def main():
             "refinement": "FP64 on GPU",
             "schedule": ctx.get_schedule().as_dict(),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(kgbs, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(kgbs / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel": kname,
-            "layout": "row-slot" if slots else "LDU face",
-            "patterned_rows": patterned if slots else None,
-            "bytes_per_launch": kbytes,
-            "reference_format_bytes": ref_kbytes,
-            "reference_format_GB_s": round(ref_kgbs, 1),
-            "avg_launch_us": round(1e6 * kavg, 3),
-            "launches": nl,
-        },
-        "spmv": {
-            "kernel": "k_spmv<float> (plain FP32 SpMV of the inner operator)",
-            "layout": "row-slot" if slots else "LDU face",
-            "bytes_per_launch": sbytes,
-            "avg_launch_us": round(1e6 * savg, 3),
-            "GB_s": round(sgbs, 1),
-            "frac": round(sgbs / HBM_PEAK_GBS, 4),
-            "reference_format_bytes": ref_sbytes,
-            "reference_format_GB_s": round(ref_sgbs, 1),
-            "mall_flushed": cold,
-            "autotuned": tuned,
-        },
+        "roofline": roof,
+        "spmv": None if spmv is None else dict(spmv, mall_flushed=cold, autotuned=tuned),
+        "rooflines_more": extras,
         "solve": {
             "iterations": perf.nIterations,
             "restarts": perf.restarts,
@@ -424,6 +624,7 @@ def main():
             "deflated": perf.deflated,
             "polished": perf.polished,
             "host_arrays_ms": None if host_ms is None else round(host_ms, 3),
+            "topology_check_ms": topo,
             "vs_cpu_fp64": errors,
         },
         "fp64_plain": None if fp64_perf is None else {
@@ -437,6 +638,8 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = round(cpu["value"] / ms, 1)
+        if cpu.get("all_cores"):
+            out["speedup_vs_cpu_all_cores"] = round(cpu["all_cores"]["value"] / ms, 1)
     print(json.dumps(out), flush=True)
 
 
@@ -461,7 +664,8 @@ def _common(args, value, unit, ms, desc, higher):
 def run_batch(args, H, mg, desc):
     """Batched throughput: 320 refined solves (64 systems x 5 RHS) per step, after
     the batch autotuner has picked each batch's schedule and RHS group."""
-    systems = mg.config3()
+    as_specified = args.batch_rhs == "as-specified"
+    systems = mg.config3(zero_mean_poisson=not as_specified)
     nrhs = systems[0].source.shape[1]
     sym = [m for m in systems if m.lower is None]
     asym = [m for m in systems if m.lower is not None]
@@ -484,7 +688,10 @@ def run_batch(args, H, mg, desc):
                       "xcdRemap": sc.xcdRemap, "rhsGroup": grp})
         log(f"batch autotune ({name}): schedule {sc.spmvSchedule}, rowForm {sc.rowForm}, "
             f"xcdRemap {sc.xcdRemap}, RHS group {grp}")
-    ctl = H.default_controls(maxRestarts=8, maxIter=20000)
+    # as specified, the Poisson right-hand sides are N(0,1) draws with a
+    # non-zero sum: the pinned projection (deflate 3) solves them; with the
+    # reference's zero-mean projection some stall until the polish
+    ctl = H.default_controls(maxRestarts=8, maxIter=20000, deflate=3 if as_specified else 2)
 
     def step():
         out = []
@@ -525,7 +732,10 @@ def run_batch(args, H, mg, desc):
         f"{int(np.median(iters))}/{max(iters)}, worst numpy residual {worst:.2e}")
     out = _common(args, round(nsolves / (ms / 1e3), 2), "solves/s", ms, desc, True)
     out["config"].update({"systems": len(systems), "rhs": nrhs,
-                          "cells": int(sum(m.n for m in systems)), "tolerance": 1e-6})
+                          "cells": int(sum(m.n for m in systems)), "tolerance": 1e-6,
+                          "poisson_rhs": ("N(0,1) as drawn (non-zero sum), deflate 3"
+                                          if as_specified else
+                                          "N(0,1) projected to zero mean, deflate auto")})
     out["batch"] = {"solves": nsolves, "converged": conv, "worst_numpy_residual": worst,
                     "iterations_min_median_max": [min(iters), int(np.median(iters)), max(iters)],
                     "autotuned": tuned}

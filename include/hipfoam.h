@@ -39,6 +39,10 @@ extern "C" {
 #define HIPFOAM_FP32 0
 #define HIPFOAM_FP64 1
 
+/* topology checks of hipfoam_set_topology (hipfoam_set_topology_check) */
+#define HIPFOAM_TOPO_POINTERS 0 /* sizes + lowAddr/uppAddr pointers      */
+#define HIPFOAM_TOPO_CONTENT 1  /* sizes + hash of every entry (default) */
+
 typedef struct hipfoam_context hipfoam_context;
 
 /* Inner-solve statistics: metalFoamBackend::Stats (B.H:41-50). */
@@ -115,16 +119,26 @@ const char* hipfoam_last_error(const hipfoam_context* ctx);
 /* ---- backend (inner) interface: metalFoamBackend -------------------- */
 
 /* metalFoamBackend::setTopology (B.C:454-519).  Uploads int32 addressing
- * once; a no-op when (nCells, nFaces, lowAddr, uppAddr) are unchanged, as in
- * the reference, and a fingerprint of the addressing content (1024 evenly
- * spaced entries of each array, the first and last included) is too:
- * addressing rewritten in place at the same addresses is uploaded again
- * when the rewrite touches a sampled entry. */
+ * once per mesh.  With HIPFOAM_TOPO_CONTENT (the default) it is a no-op when
+ * nCells, nFaces and a 64-bit hash of every entry of the five arrays are
+ * unchanged: addressing rewritten in place at the same addresses is uploaded
+ * again, the same mesh at new addresses is not.  The hash reads the whole
+ * addressing on every call (about 22 bytes per cell of a hexahedral mesh,
+ * split over up to 8 host threads).  With HIPFOAM_TOPO_POINTERS it is a
+ * no-op when (nCells, nFaces, lowAddr, uppAddr) are unchanged, exactly as
+ * the reference (B.C:469-477), which reuses stale addressing after an
+ * in-place rewrite. */
 int32_t hipfoam_set_topology(hipfoam_context* ctx, int32_t nCells,
                              int32_t nFaces, const int32_t* lowAddr,
                              const int32_t* uppAddr, const int32_t* ownStart,
                              const int32_t* losort,
                              const int32_t* losortStart);
+
+/* Selects how hipfoam_set_topology decides that the mesh is unchanged:
+ * HIPFOAM_TOPO_CONTENT (default) or HIPFOAM_TOPO_POINTERS (the reference's
+ * key, B.C:469-477, for callers that replace the addressing arrays whenever
+ * the mesh changes and want no per-call hash).  No reference counterpart. */
+int32_t hipfoam_set_topology_check(hipfoam_context* ctx, int32_t mode);
 
 /* metalFoamBackend::setCoeffs (B.C:522-565): FP64 -> FP32 operator of the
  * inner solve, rD = float(1/diag).  lower == NULL selects the symmetric

@@ -18,6 +18,7 @@ HEADER = os.path.join(ROOT, "include", "hipfoam.h")
 OK, ERR_ARG, ERR_DEVICE, ERR_STATE, ERR_SIZE = 0, 1, 2, 3, 4
 PCG, PBICGSTAB = 0, 1
 FP32, FP64 = 0, 1
+TOPO_POINTERS, TOPO_CONTENT = 0, 1
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
@@ -83,6 +84,7 @@ SIGNATURES = {
     "hipfoam_last_error": (C.c_char_p, [_P]),
     "hipfoam_set_topology": (C.c_int32, [_P, C.c_int32, C.c_int32, _i32p, _i32p,
                                          _i32p, _i32p, _i32p]),
+    "hipfoam_set_topology_check": (C.c_int32, [_P, C.c_int32]),
     "hipfoam_set_coeffs": (C.c_int32, [_P, _f64p, _f64p, _f64p, _f64p]),
     "hipfoam_pcg": (C.c_int32, [_P, _f64p, C.c_double, C.c_int32, C.c_int32,
                                 C.c_int32, C.c_double, C.c_double,

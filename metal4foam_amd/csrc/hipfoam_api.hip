@@ -157,7 +157,8 @@ struct hipfoam_context {
   Buf<double> slotCoef64;
   const int32_t* lastLow = nullptr;
   const int32_t* lastUpp = nullptr;
-  uint64_t topoPrint = 0;  // addressing_fingerprint of the uploaded mesh
+  uint64_t topoHash = 0;  // addressing_hash of the uploaded mesh
+  bool topoHashValid = false;
   int gen = 0;  // bumps whenever device buffers are reallocated
   Buf<int32_t> ownStart, losortStart, uppAddr, lowAddr, losort, losortLow;
 
@@ -191,8 +192,13 @@ struct hipfoam_context {
   hipfoam_schedule sched{2, 0, 2, -1, 0, 0};
   std::map<GraphKey, hipGraphExec_t> graphs;
   KernelTimer timer;
+  // host helper threads (created on first use): staged copies, addressing hash
+  std::unique_ptr<CopyPool> pool;
   // pinned staging of large host arrays (created on first use)
   std::unique_ptr<HostStager> stager;
+  // topology check of hipfoam_set_topology: HIPFOAM_TOPO_CONTENT (default) or
+  // HIPFOAM_TOPO_POINTERS
+  int topoCheck = HIPFOAM_TOPO_CONTENT;
   // device address of this device's scratch array (hipGetSymbolAddress at
   // create), used by hipfoam_solve_device_noalloc only
   char* scratchDev = nullptr;
@@ -323,6 +329,15 @@ void launch_fin(hipfoam_context* c, int nblk, bool checkStatus, int runMask, F f
 // returns once the host array has been read, device to host once the host
 // array holds the data (after the work already on the stream).  Other copies
 // are hipMemcpyAsync.
+// The context's host helper threads: up to 7 besides the caller's.
+CopyPool& host_pool(hipfoam_context* c) {
+  if (!c->pool) {
+    const int hw = (int)std::thread::hardware_concurrency();
+    c->pool = std::make_unique<CopyPool>(std::max(0, std::min(7, hw - 1)));
+  }
+  return *c->pool;
+}
+
 int xfer(hipfoam_context* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
   const bool staged = (kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToHost) &&
                       bytes >= HostStager::kMin;
@@ -332,8 +347,7 @@ int xfer(hipfoam_context* c, void* dst, const void* src, size_t bytes, hipMemcpy
   }
   if (!c->stager) {
     auto st = std::make_unique<HostStager>();
-    const int hw = (int)std::thread::hardware_concurrency();
-    CK(st->init(std::max(0, std::min(7, hw - 1))));
+    CK(st->init(&host_pool(c)));
     c->stager = std::move(st);
   }
   if (kind == hipMemcpyHostToDevice)
@@ -1660,9 +1674,69 @@ const char* hipfoam_last_error(const hipfoam_context* c) {
 // Fingerprint of a mesh's addressing: the sizes and 1024 evenly spaced
 // entries (the first and last included) of each addressing array, mixed
 // with splitmix64.  A few thousand loads, so it is taken on every call.
-uint64_t addressing_fingerprint(int32_t n, int32_t nf, const int32_t* low, const int32_t* upp,
-                                const int32_t* ownStart, const int32_t* losort,
-                                const int32_t* losortStart) {
+// 64-bit hash of a byte range: four independent multiply-rotate lanes over
+// 8-byte words (the accumulate step of the published xxHash64 round, written
+// here from its description), a tail loop, and the murmur3 64-bit finaliser.
+uint64_t hash_bytes(const unsigned char* p, size_t len, uint64_t seed) {
+  constexpr uint64_t P1 = 0x9e3779b185ebca87ull, P2 = 0xc2b2ae3d27d4eb4full;
+  auto rotl = [](uint64_t x, int r) { return (x << r) | (x >> (64 - r)); };
+  uint64_t a0 = seed + P1, a1 = seed ^ P2, a2 = seed * P1 + 1, a3 = seed - P1;
+  size_t i = 0;
+  for (; i + 32 <= len; i += 32) {
+    uint64_t w[4];
+    std::memcpy(w, p + i, 32);
+    a0 = rotl(a0 + w[0] * P2, 31) * P1;
+    a1 = rotl(a1 + w[1] * P2, 31) * P1;
+    a2 = rotl(a2 + w[2] * P2, 31) * P1;
+    a3 = rotl(a3 + w[3] * P2, 31) * P1;
+  }
+  uint64_t h = rotl(a0, 1) + rotl(a1, 7) + rotl(a2, 12) + rotl(a3, 18) + len;
+  for (; i < len; ++i) h = (h ^ p[i]) * P1;
+  h ^= h >> 33;
+  h *= 0xff51afd7ed558ccdull;
+  h ^= h >> 33;
+  h *= 0xc4ceb9fe1a85ec53ull;
+  return h ^ (h >> 33);
+}
+
+// Hash of the whole addressing content (every entry of the five arrays and
+// the sizes), so any rewrite of it is detected, up to 64-bit collisions.
+// The arrays are cut into 1 MiB chunks hashed independently (on the
+// context's host threads when the addressing is large) and combined in
+// order: the result does not depend on the thread count.  Measured on the
+// MI355X host at 128^3 (92 MB of addressing): see hipfoam_set_topology_check.
+uint64_t addressing_hash(hipfoam_context* c, int32_t n, int32_t nf, const int32_t* low,
+                         const int32_t* upp, const int32_t* ownStart, const int32_t* losort,
+                         const int32_t* losortStart) {
+  constexpr size_t kChunk = size_t(1) << 20;
+  constexpr size_t kThreadedMin = size_t(4) << 20;
+  const size_t fb = sizeof(int32_t) * (size_t)std::max(nf, 0);
+  const size_t cb = sizeof(int32_t) * ((size_t)std::max(n, 0) + 1);
+  const unsigned char* arr[5] = {reinterpret_cast<const unsigned char*>(low),
+                                 reinterpret_cast<const unsigned char*>(upp),
+                                 reinterpret_cast<const unsigned char*>(losort),
+                                 reinterpret_cast<const unsigned char*>(ownStart),
+                                 reinterpret_cast<const unsigned char*>(losortStart)};
+  const size_t bytes[5] = {fb, fb, fb, cb, cb};
+  std::vector<std::pair<int, size_t>> jobs;
+  size_t total = 0;
+  for (int k = 0; k < 5; ++k) {
+    if (!arr[k]) continue;
+    total += bytes[k];
+    for (size_t o = 0; o < bytes[k]; o += kChunk) jobs.push_back({k, o});
+  }
+  std::vector<uint64_t> part(jobs.size());
+  auto work = [&](int k, int parts) {
+    for (size_t j = (size_t)k; j < jobs.size(); j += (size_t)parts) {
+      const int a = jobs[j].first;
+      const size_t o = jobs[j].second, len = std::min(kChunk, bytes[a] - o);
+      part[j] = hash_bytes(arr[a] + o, len, ((uint64_t)a << 40) ^ o);
+    }
+  };
+  if (total >= kThreadedMin)
+    host_pool(c).run(work);
+  else
+    work(0, 1);
   auto mix = [](uint64_t x) {
     x += 0x9e3779b97f4a7c15ull;
     x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -1670,19 +1744,7 @@ uint64_t addressing_fingerprint(int32_t n, int32_t nf, const int32_t* low, const
     return x ^ (x >> 31);
   };
   uint64_t h = mix(((uint64_t)(uint32_t)n << 32) | (uint32_t)nf);
-  auto add = [&](const int32_t* a, int64_t len) {
-    if (!a || len <= 0) return;
-    const int64_t k = std::min<int64_t>(len, 1024);
-    for (int64_t j = 0; j < k; ++j) {
-      const int64_t i = k > 1 ? j * (len - 1) / (k - 1) : 0;
-      h = mix(h ^ (((uint64_t)i << 32) | (uint32_t)a[i]));
-    }
-  };
-  add(low, nf);
-  add(upp, nf);
-  add(losort, nf);
-  add(ownStart, (int64_t)n + 1);
-  add(losortStart, (int64_t)n + 1);
+  for (size_t j = 0; j < part.size(); ++j) h = mix(h ^ part[j]) + j;
   return h;
 }
 
@@ -1698,14 +1760,24 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
     return fail(c, HIPFOAM_ERR_ARG, "NULL addressing");
   if (!ownStart || !losortStart) return fail(c, HIPFOAM_ERR_ARG, "NULL start arrays");
   CK(hipSetDevice(c->device));
-  // no-op when unchanged: the reference keys on the sizes and the addressing
-  // pointers (metalFoamBackend.C:469-477); the content fingerprint also
-  // catches addressing rewritten in place at the same addresses
-  const uint64_t print = addressing_fingerprint(nCells, nFaces, lowAddr, uppAddr, ownStart,
-                                                losort, losortStart);
-  if (c->ownStart.p && c->n == nCells && c->nf == nFaces && c->lastLow == lowAddr &&
-      c->lastUpp == uppAddr && c->topoPrint == print)
+  // no-op when unchanged.  HIPFOAM_TOPO_POINTERS keys on the sizes and the
+  // addressing pointers, as the reference does (metalFoamBackend.C:469-477);
+  // HIPFOAM_TOPO_CONTENT (the default) on the sizes and a hash of every
+  // addressing entry, so addressing rewritten in place is uploaded again and
+  // the same mesh at new addresses is not
+  uint64_t hash = 0;
+  if (c->topoCheck == HIPFOAM_TOPO_CONTENT) {
+    hash = addressing_hash(c, nCells, nFaces, lowAddr, uppAddr, ownStart, losort, losortStart);
+    if (c->ownStart.p && c->topoHashValid && c->n == nCells && c->nf == nFaces &&
+        c->topoHash == hash) {
+      c->lastLow = lowAddr;
+      c->lastUpp = uppAddr;
+      return HIPFOAM_OK;
+    }
+  } else if (c->ownStart.p && c->n == nCells && c->nf == nFaces && c->lastLow == lowAddr &&
+             c->lastUpp == uppAddr) {
     return HIPFOAM_OK;
+  }
   CK(hipStreamSynchronize(c->stream));
   clear_graphs(c);
   c->gen += 1;
@@ -1723,6 +1795,7 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
   c->strideMinor = c->slotOK ? lay.strideMinor : 0;
   // the cache key is set once the upload has succeeded
   c->lastLow = c->lastUpp = nullptr;
+  c->topoHashValid = false;
   c->have32 = c->have64 = false;
   const size_t nf = (size_t)nFaces, n1 = (size_t)nCells + 1;
   CK(c->ownStart.ensure(n1));
@@ -1769,7 +1842,18 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
   CK(hipStreamSynchronize(c->stream));
   c->lastLow = lowAddr;
   c->lastUpp = uppAddr;
-  c->topoPrint = print;
+  c->topoHash = hash;
+  c->topoHashValid = c->topoCheck == HIPFOAM_TOPO_CONTENT;
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_set_topology_check(hipfoam_context* c, int32_t mode) {
+  FpTrapGuard fpe;
+  if (!c) return HIPFOAM_ERR_ARG;
+  if (mode != HIPFOAM_TOPO_POINTERS && mode != HIPFOAM_TOPO_CONTENT)
+    return fail(c, HIPFOAM_ERR_ARG, "topology check must be HIPFOAM_TOPO_POINTERS (0) or "
+                                    "HIPFOAM_TOPO_CONTENT (1)");
+  c->topoCheck = mode;
   return HIPFOAM_OK;
 }
 

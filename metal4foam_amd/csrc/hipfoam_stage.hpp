@@ -15,14 +15,17 @@
 #include <condition_variable>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include <vector>
 
 namespace hipfoam {
 
-// A fixed pool of helper threads that copies one buffer in equal parts; the
-// calling thread copies the first part itself.
+// A fixed pool of helper threads that runs one job in parts: the calling
+// thread runs part 0 itself, the helpers the others.  Used for the host side
+// of staged copies (one buffer in equal parts) and for hashing the mesh
+// addressing.
 class CopyPool {
  public:
   explicit CopyPool(int helpers) {
@@ -39,46 +42,60 @@ class CopyPool {
   CopyPool(const CopyPool&) = delete;
   CopyPool& operator=(const CopyPool&) = delete;
 
-  // memcpy(dst, src, bytes), returning when every part is copied
-  void copy(void* dst, const void* src, size_t bytes) {
+  int parts() const { return (int)th_.size() + 1; }
+
+  // fn(k, parts) for k = 0 .. parts-1, returning when every part is done
+  void run(const std::function<void(int, int)>& fn) {
     const int parts = (int)th_.size() + 1;
-    if (parts == 1 || bytes < kSplitMin) {
-      std::memcpy(dst, src, bytes);
+    if (parts == 1) {
+      fn(0, 1);
       return;
     }
     {
       std::lock_guard<std::mutex> lk(m_);
-      dst_ = static_cast<char*>(dst);
-      src_ = static_cast<const char*>(src);
-      bytes_ = bytes;
+      fn_ = &fn;
       parts_ = parts;
       pending_ = parts - 1;
       ++job_;
     }
     go_.notify_all();
-    part(0);
+    fn(0, parts);
     std::unique_lock<std::mutex> lk(m_);
     done_.wait(lk, [this] { return pending_ == 0; });
+    fn_ = nullptr;
+  }
+
+  // memcpy(dst, src, bytes), returning when every part is copied
+  void copy(void* dst, const void* src, size_t bytes) {
+    if (th_.empty() || bytes < kSplitMin) {
+      std::memcpy(dst, src, bytes);
+      return;
+    }
+    char* d = static_cast<char*>(dst);
+    const char* s = static_cast<const char*>(src);
+    run([&](int k, int parts) {
+      const size_t a = bytes * k / parts, e = bytes * (k + 1) / parts;
+      std::memcpy(d + a, s + a, e - a);
+    });
   }
 
  private:
   static constexpr size_t kSplitMin = 256u << 10;
 
-  void part(int k) const {
-    const size_t a = bytes_ * k / parts_, e = bytes_ * (k + 1) / parts_;
-    std::memcpy(dst_ + a, src_ + a, e - a);
-  }
-
   void work(int k) {
     uint64_t seen = 0;
     for (;;) {
+      const std::function<void(int, int)>* fn = nullptr;
+      int parts = 1;
       {
         std::unique_lock<std::mutex> lk(m_);
         go_.wait(lk, [&] { return stop_ || job_ != seen; });
         if (stop_) return;
         seen = job_;
+        fn = fn_;
+        parts = parts_;
       }
-      part(k);
+      (*fn)(k, parts);
       std::lock_guard<std::mutex> lk(m_);
       if (--pending_ == 0) done_.notify_one();
     }
@@ -87,9 +104,7 @@ class CopyPool {
   std::vector<std::thread> th_;
   std::mutex m_;
   std::condition_variable go_, done_;
-  char* dst_ = nullptr;
-  const char* src_ = nullptr;
-  size_t bytes_ = 0;
+  const std::function<void(int, int)>* fn_ = nullptr;
   int parts_ = 1, pending_ = 0;
   uint64_t job_ = 0;
   bool stop_ = false;
@@ -104,15 +119,15 @@ class HostStager {
   static constexpr int kBuf = 3;
   static constexpr size_t kMin = 4u << 20;  // smaller copies: hipMemcpyAsync
 
-  // helpers: threads besides the caller's
-  hipError_t init(int helpers) {
+  // pool: the owner's copy pool (outlives the stager)
+  hipError_t init(CopyPool* pool) {
     for (int b = 0; b < kBuf; ++b) {
       hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&buf_[b]), kChunk, 0);
       if (e != hipSuccess) return e;
       e = hipEventCreateWithFlags(&ev_[b], hipEventDisableTiming);
       if (e != hipSuccess) return e;
     }
-    pool_ = new CopyPool(helpers);
+    pool_ = pool;
     return hipSuccess;
   }
 
@@ -124,7 +139,6 @@ class HostStager {
       }
       if (buf_[b]) (void)hipHostFree(buf_[b]);
     }
-    delete pool_;
   }
 
   // Host to device.  Returns once the host array has been read; the DMA of

@@ -20,6 +20,7 @@ from ._lib import Controls, Performance, Schedule, Stats
 
 PCG, PBICGSTAB = _lib.PCG, _lib.PBICGSTAB
 FP32, FP64 = _lib.FP32, _lib.FP64
+TOPO_POINTERS, TOPO_CONTENT = _lib.TOPO_POINTERS, _lib.TOPO_CONTENT
 
 
 class HipFoamError(RuntimeError):
@@ -155,6 +156,13 @@ class Context:
             self.h, m.n, lo.shape[0], _ptr(lo, i32p), _ptr(up, i32p),
             _ptr(os_, i32p), _ptr(ls, i32p), _ptr(lss, i32p)), "set_topology")
         self.n = m.n
+
+    def set_topology_check(self, mode):
+        """TOPO_CONTENT (default): set_topology re-uploads whenever any
+        addressing entry changed; TOPO_POINTERS: only when the sizes or the
+        lowAddr/uppAddr pointers changed (the reference's key)."""
+        self._check(self.lib.hipfoam_set_topology_check(self.h, int(mode)),
+                    "set_topology_check")
 
     def set_coeffs(self, diag, upper, lower=None):
         f64p = C.POINTER(C.c_double)

@@ -1,5 +1,12 @@
 /*---------------------------------------------------------------------------*\
     hipFoam: MI355X (gfx950) linear solvers for OpenFOAM
+
+    The OpenFOAM solver-class structure of this file (run-time selection,
+    constructor, solve/scalarSolve, solveImpl) follows metal4Foam
+    (scotCW/metal4Foam, src/metal4Foam/solvers/metalFoamSolver.C), Copyright (C) 2026
+    metal4Foam authors, and OpenFOAM's lduMatrix::solver API.  Both are
+    licensed GPL-3.0-or-later; this file is distributed under the same terms.
+    SPDX-License-Identifier: GPL-3.0-or-later
 \*---------------------------------------------------------------------------*/
 
 #include "hipFoamSolver.H"
@@ -90,6 +97,20 @@ hipfoam_context* Foam::hipFoamSolver::context() const
         const label device = hipDict_.getOrDefault<label>("device", 0);
         hipfoam_create(device, &slot->ctx);
         Info<< "    " << hipfoam_description(slot->ctx) << nl;
+        // "content" (default): any rewrite of the addressing is detected;
+        // "pointers": the reference's key (sizes and addressing pointers)
+        const word check = hipDict_.getOrDefault<word>("topologyCheck", "content");
+        if (check != "content" && check != "pointers")
+        {
+            FatalErrorInFunction
+                << "hip.topologyCheck must be content or pointers, not "
+                << check << exit(FatalError);
+        }
+        hipfoam_set_topology_check
+        (
+            slot->ctx,
+            check == "pointers" ? HIPFOAM_TOPO_POINTERS : HIPFOAM_TOPO_CONTENT
+        );
     }
     return slot->ctx;
 }

@@ -1,6 +1,13 @@
 /*---------------------------------------------------------------------------*\
     hipFoam: MI355X (gfx950) linear solvers for OpenFOAM
 
+    The OpenFOAM solver-class structure of this file (run-time selection,
+    constructor, solve/scalarSolve, solveImpl) follows metal4Foam
+    (scotCW/metal4Foam, src/metal4Foam/solvers/metalPCG.H), Copyright (C) 2026
+    metal4Foam authors, and OpenFOAM's lduMatrix::solver API.  Both are
+    licensed GPL-3.0-or-later; this file is distributed under the same terms.
+    SPDX-License-Identifier: GPL-3.0-or-later
+
 Class
     Foam::hipPCG
 

@@ -112,6 +112,9 @@ def main():
     ap.add_argument("--patterned-rows", type=int, default=0)
     ap.add_argument("--traffic-json")
     ap.add_argument("--traffic-kernel", default="k_cheb_step<float, false>")
+    ap.add_argument("--traffic", action="append", default=[],
+                    help="PATH=KERNEL[|KERNEL...]: write the calibrated traffic of these "
+                         "kernels (their mean, as the benchmark averages them) to PATH")
     ap.add_argument("--workload", default="")
     a = ap.parse_args()
     slots = a.layout == "slots"
@@ -177,15 +180,27 @@ def main():
               f"{'' if (tr is None or b is None) else f'{tr / b:.2f}'} |")
 
 
-    if a.traffic_json and fscale:
+    jobs = list(a.traffic)
+    if a.traffic_json:
+        jobs.append(f"{a.traffic_json}={a.traffic_kernel}")
+    if jobs and fscale:
         import json
 
-        names = [k for k in fetch if a.traffic_kernel in k and k in write]
-        if names:
-            k = names[0]
-            f_kib, w_kib = mean(fetch[k]), mean(write[k])
+        for job in jobs:
+            path, kernels = job.split("=", 1)
+            wanted = kernels.split("|")
+            names = []
+            for w in wanted:
+                hit = [k for k in fetch if w in k and k in write]
+                if hit:
+                    names.append(hit[0])
+            if len(names) != len(wanted):
+                continue
+            f_kib = mean([mean(fetch[k]) for k in names])
+            w_kib = mean([mean(write[k]) for k in names])
+            alg = [_bytes(k, a.cells, a.faces, a.asym, slots, a.patterned_rows) for k in names]
             out = {
-                "kernel": a.traffic_kernel,
+                "kernel": " / ".join(wanted),
                 "workload": a.workload,
                 "layout": "row-slot" if slots else "LDU face",
                 "patterned_rows": a.patterned_rows if slots else None,
@@ -195,11 +210,11 @@ def main():
                                 "bytes_written": 8 * a.cells, "fetch_scale": round(fscale, 4),
                                 "write_scale": round(wscale, 4)},
                 "bytes_per_launch": int(round((f_kib * fscale + w_kib * wscale) * 1024.0)),
-                "algorithmic_bytes_per_launch": _bytes(k, a.cells, a.faces, a.asym, slots,
-                                                       a.patterned_rows),
+                "algorithmic_bytes_per_launch": (int(round(mean(alg))) if None not in alg
+                                                 else None),
                 "commands": a.command,
             }
-            with open(a.traffic_json, "w") as fh:
+            with open(path, "w") as fh:
                 json.dump(out, fh, indent=1)
                 fh.write("\n")
 

@@ -99,6 +99,8 @@ def declare(L):
                                 C.c_double, C.c_double, C.c_int, C.c_int,
                                 _f64p, _f64p, C.POINTER(Perf)]
     L.orc_adjoint_grads.argtypes = [A, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_pcg_diag_par.argtypes = [A, _f64p, _f64p, C.c_double, C.c_double, C.c_int,
+                                   C.c_int, C.c_int, _f64p, _f64p, C.POINTER(Perf)]
     return L
 
 
@@ -109,6 +111,20 @@ def lib():
             build()
         _lib = declare(C.CDLL(_LIB))
     return _lib
+
+
+def build_native(outdir):
+    """Builds the CPU-baseline variant of the oracle on this host
+    (-O3 -march=native, OpenMP) into outdir and returns it as a declared
+    CDLL, or None when the build fails (no compiler)."""
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "libldu_oracle_native.so")
+    try:
+        subprocess.run(["make", "-s", "-C", _HERE, "native", f"NATIVE_OUT={out}"], check=True,
+                       capture_output=True, timeout=300)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    return declare(C.CDLL(out))
 
 
 def swap_library(L):
@@ -316,6 +332,19 @@ def pbicgstab(m, precon=PRE_DIC, tol=1e-6, relTol=0.0, maxIter=100000,
     lib().orc_pbicgstab(h.ref, _ptr64(_c64(m.diag)), _ptr64(up), _ptr64(lo),
                         precon, tol, relTol, maxIter, minIter, _ptr64(psi),
                         _ptr64(b), C.byref(perf))
+    return psi, perf
+
+
+def pcg_diag_par(m, tol=1e-6, relTol=0.0, maxIter=100000, minIter=0, threads=0, psi=None,
+                 source=None):
+    """OpenFOAM PCG(diagonal) in FP64 over OpenMP threads (0: the runtime's
+    default); one thread when the library was built without OpenMP."""
+    h = _AddrHolder(m)
+    psi = np.zeros(m.n) if psi is None else _c64(psi).copy()
+    b = _c64(m.source if source is None else source)
+    perf = Perf()
+    lib().orc_pcg_diag_par(h.ref, _ptr64(_c64(m.diag)), _ptr64(_c64(m.upper)), tol, relTol,
+                           maxIter, minIter, threads, _ptr64(psi), _ptr64(b), C.byref(perf))
     return psi, perf
 
 

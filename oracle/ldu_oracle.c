@@ -23,6 +23,8 @@
  *   orc_pcg / orc_pbicgstab           OpenFOAM PCG / PBiCGStab (DIC, DILU, diagonal, or the
  *                                     Chebyshev-Jacobi preconditioner in FP64)
  *   orc_adjoint_grads                 dL/dupper, dL/dlower, dL/ddiag of L(x(A)), A x = b
+ *   orc_pcg_diag_par                  OpenFOAM PCG(diagonal) over OpenMP threads (CPU
+ *                                     baseline on all host cores)
  *
  * Reductions are accumulated in double and rounded once (the reference's
  * Metal simd_sum order cannot be reproduced bit-for-bit; parity with the FP32
@@ -34,6 +36,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORC_VSMALL 1.0e-300 /* OpenFOAM doubleScalarVSMALL */
 #define ORC_SMALL 1.0e-20   /* solverPerformance small_ */
@@ -1072,4 +1077,149 @@ void orc_adjoint_grads(const orc_addr* a, const double* x, const double* lam,
         dUpper[f] = -lam[l] * x[u];
         dLower[f] = -lam[u] * x[l];
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* CPU baseline on all host cores: OpenFOAM PCG(diagonal) in FP64, OpenMP    */
+/* ------------------------------------------------------------------------ */
+
+/* y = A x by rows (each row gathers its owner faces through ownStart and its
+ * neighbour faces through losortStart/losort), so the rows are independent
+ * and split over threads without atomics.  Same terms as orc_amul; the sum
+ * order within a row differs from the face loop's. */
+static void amul_rows_par(const orc_addr* a, const double* diag,
+                          const double* upper, const double* lower,
+                          const double* x, double* y)
+{
+    if (!lower) lower = upper;
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < a->n; ++i)
+    {
+        double s = diag[i] * x[i];
+        for (int32_t f = a->ownStart[i]; f < a->ownStart[i + 1]; ++f)
+            s += upper[f] * x[a->upp[f]];
+        for (int32_t k = a->losortStart[i]; k < a->losortStart[i + 1]; ++k)
+        {
+            const int32_t f = a->losort[k];
+            s += lower[f] * x[a->low[f]];
+        }
+        y[i] = s;
+    }
+}
+
+static double dot_par(int32_t n, const double* x, const double* y)
+{
+    double s = 0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
+    for (int32_t i = 0; i < n; ++i) s += x[i] * y[i];
+    return s;
+}
+
+static double summag_par(int32_t n, const double* x)
+{
+    double s = 0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
+    for (int32_t i = 0; i < n; ++i) s += fabs(x[i]);
+    return s;
+}
+
+/* OpenFOAM PCG::scalarSolve with the diagonal preconditioner (the loop of
+ * orc_pcg, precon 0), every O(n) step split over `nthreads` OpenMP threads
+ * (0: the runtime's default).  Built without OpenMP it runs on one thread. */
+int orc_pcg_diag_par(const orc_addr* a, const double* diag,
+                     const double* upper, double tol, double relTol,
+                     int maxIter, int minIter, int nthreads, double* psi,
+                     const double* source, orc_perf* perf)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const int32_t n = a->n;
+    const size_t nb = sizeof(double) * (size_t)(n > 0 ? n : 1);
+    memset(perf, 0, sizeof(*perf));
+    double* wA = (double*)malloc(nb);
+    double* rA = (double*)malloc(nb);
+    double* pA = (double*)malloc(nb);
+    double* rD = (double*)malloc(nb);
+    double* u = (double*)malloc(nb);
+    double* ones = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    /* wA = A psi, rA = b - wA; normFactor = sum(|A psi - xbar u| + |b - xbar u|)
+     * + small with u = A 1 and xbar = mean(psi) */
+    amul_rows_par(a, diag, upper, NULL, psi, wA);
+    double xbar = 0;
+#pragma omp parallel for schedule(static) reduction(+ : xbar)
+    for (int32_t i = 0; i < n; ++i)
+    {
+        rA[i] = source[i] - wA[i];
+        ones[i] = 1.0;
+        xbar += psi[i];
+    }
+    xbar = n > 0 ? xbar / n : 0.0;
+    amul_rows_par(a, diag, upper, NULL, ones, u);
+    double nfs = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nfs)
+    for (int32_t i = 0; i < n; ++i)
+        nfs += fabs(wA[i] - xbar * u[i]) + fabs(source[i] - xbar * u[i]);
+    const double nf = nfs + ORC_SMALL;
+    perf->normFactor = nf;
+    perf->initialResidual = summag_par(n, rA) / nf;
+    perf->finalResidual = perf->initialResidual;
+    int nIter = 0;
+    if (minIter > 0 || !check_convergence(perf->finalResidual,
+                                          perf->initialResidual, tol, relTol))
+    {
+#pragma omp parallel for schedule(static)
+        for (int32_t i = 0; i < n; ++i) rD[i] = 1.0 / diag[i];
+        double wArA = 1e300, wArAold;
+        do
+        {
+            wArAold = wArA;
+#pragma omp parallel for schedule(static)
+            for (int32_t i = 0; i < n; ++i) wA[i] = rD[i] * rA[i];
+            wArA = dot_par(n, wA, rA);
+            if (nIter == 0)
+            {
+#pragma omp parallel for schedule(static)
+                for (int32_t i = 0; i < n; ++i) pA[i] = wA[i];
+            }
+            else
+            {
+                const double b = wArA / wArAold;
+#pragma omp parallel for schedule(static)
+                for (int32_t i = 0; i < n; ++i) pA[i] = wA[i] + b * pA[i];
+            }
+            amul_rows_par(a, diag, upper, NULL, pA, wA);
+            const double wApA = dot_par(n, wA, pA);
+            if (fabs(wApA) / nf < ORC_VSMALL)
+            {
+                perf->singular = 1;
+                break;
+            }
+            const double alpha = wArA / wApA;
+            double rs = 0;
+#pragma omp parallel for schedule(static) reduction(+ : rs)
+            for (int32_t i = 0; i < n; ++i)
+            {
+                psi[i] += alpha * pA[i];
+                rA[i] -= alpha * wA[i];
+                rs += fabs(rA[i]);
+            }
+            perf->finalResidual = rs / nf;
+        } while (((++nIter < maxIter) &&
+                  !check_convergence(perf->finalResidual,
+                                     perf->initialResidual, tol, relTol)) ||
+                 nIter < minIter);
+    }
+    perf->nIterations = nIter;
+    perf->converged = check_convergence(perf->finalResidual,
+                                        perf->initialResidual, tol, relTol);
+    free(wA);
+    free(rA);
+    free(pA);
+    free(rD);
+    free(u);
+    free(ones);
+    return 0;
 }

@@ -155,27 +155,98 @@ def test_host_array_solve_matches_device_solve(ctx):
         a.free()
 
 
-def test_topology_rewritten_in_place_is_uploaded_again(ctx):
-    """The topology cache keys on the sizes, the addressing pointers and a
-    fingerprint of the addressing content.  Another mesh of the same sizes
-    written into the same buffers is uploaded again: the solve then equals a
-    solve on that mesh uploaded from its own arrays."""
-    a = mg.poisson(8, 6, 5, seed=1, lognormal=True)
-    b = mg.poisson(6, 8, 5, seed=2, lognormal=True)
-    assert a.n == b.n and a.nfaces == b.nfaces
-    ctx.set_topology(a)
-    psi = np.zeros(a.n)
-    assert ctx.solve(H.PCG, a.diag, a.upper, None, psi, a.source).converged
-    names = ("lower_addr", "upper_addr", "owner_start", "losort", "losort_start")
-    bufs = [getattr(a, k) for k in names]
-    for buf, k in zip(bufs, names):
-        np.copyto(buf, getattr(b, k))
-    ctx.set_topology(a)  # the same buffers, now holding b's addressing
-    assert all(H._i32(getattr(a, k)) is buf for buf, k in zip(bufs, names))
-    psi = np.zeros(b.n)
-    perf = ctx.solve(H.PCG, b.diag, b.upper, None, psi, b.source)
-    ctx.set_topology(b)
-    ref = np.zeros(b.n)
-    perf_ref = ctx.solve(H.PCG, b.diag, b.upper, None, ref, b.source)
-    assert perf.converged and perf.nIterations == perf_ref.nIterations
-    np.testing.assert_array_equal(psi, ref)
+def _sampled(length, k=1024):
+    """Indices a fingerprint of k evenly spaced entries (first and last
+    included) reads: the in-place rewrite below avoids all of them."""
+    if length <= k:
+        return set(range(length))
+    return {j * (length - 1) // (k - 1) for j in range(k)}
+
+
+def _rewrite_unsampled(m):
+    """Swap the neighbours of two owner faces of one row, in place, choosing
+    a row whose changed upperAddr and losort entries are all unsampled.
+    Returns the positions changed in upperAddr and losort."""
+    su, sl = _sampled(m.nfaces), _sampled(m.nfaces)
+    for r in range(m.n // 2, m.n):
+        f0, f1 = m.owner_start[r], m.owner_start[r + 1]
+        if f1 - f0 < 3:
+            continue
+        p1, p2 = f0 + 1, f0 + 2
+        upp = m.upper_addr.copy()
+        upp[p1], upp[p2] = upp[p2], upp[p1]
+        own, los, loss = mg.derived_addressing(m.n, m.lower_addr, upp)
+        ql = np.nonzero(los != m.losort)[0]
+        if {p1, p2} & su or set(ql.tolist()) & sl:
+            continue
+        assert np.array_equal(own, m.owner_start) and np.array_equal(loss, m.losort_start)
+        m.upper_addr[:] = upp  # same buffers
+        m.losort[:] = los
+        return [p1, p2], ql
+    raise AssertionError("no row with unsampled faces")
+
+
+def test_topology_rewritten_in_place_is_detected(ctx, orc):
+    """The default topology check hashes every addressing entry: a mesh
+    rewritten in place (same buffers, same sizes) at entries that a sampled
+    fingerprint never reads is uploaded again, and the solve matches the
+    oracle on the rewritten mesh.  The reference's key (sizes and pointers,
+    HIPFOAM_TOPO_POINTERS) keeps the stale addressing, which this shows."""
+    m = mg.poisson(20, 18, 16, seed=11, lognormal=True)
+    ctx.set_topology_check(H.TOPO_CONTENT)
+    ctx.set_topology(m)
+    ctl = H.default_controls(maxRestarts=8)
+    psi_a = np.zeros(m.n)
+    assert ctx.solve(H.PCG, m.diag, m.upper, None, psi_a, m.source, ctl).converged
+    bufs = [m.lower_addr, m.upper_addr, m.owner_start, m.losort, m.losort_start]
+    old = [b.copy() for b in bufs]
+    pu, pl = _rewrite_unsampled(m)
+    assert all(b is H._i32(b) for b in bufs)  # the arrays passed are the same buffers
+    print(f"[topology] rewrote upperAddr at {pu} and losort at {pl.tolist()}", flush=True)
+    ctx.set_topology(m)
+    psi_b = np.zeros(m.n)
+    perf = ctx.solve(H.PCG, m.diag, m.upper, None, psi_b, m.source, ctl)
+    ref_psi, ref = orc.refined_solve(m, ctl=orc.controls(maxRestarts=8))
+    x = np.linalg.solve(mg.dense(m), m.source)
+    scale = np.abs(x).max()
+    print(f"[topology] gpu it={perf.nIterations} oracle it={ref.nIterations}; "
+          f"|psi_b - x|/|x| = {np.abs(psi_b - x).max() / scale:.2e}, "
+          f"|psi_a - x|/|x| = {np.abs(psi_a - x).max() / scale:.2e}", flush=True)
+    assert perf.converged and perf.nIterations == ref.nIterations
+    assert perf.normFactor == pytest.approx(ref.normFactor, rel=1e-12)
+    assert np.abs(psi_b - ref_psi).max() <= 1e-5 * scale
+    # a 1e-6 residual leaves the pinned matrix's constant mode loose: the GPU's
+    # error against the dense solution is bounded by the oracle's own
+    assert np.abs(psi_b - x).max() <= max(5 * np.abs(ref_psi - x).max(), 1e-6 * scale)
+    assert np.abs(psi_a - x).max() > 1e-3 * scale  # the rewrite changed the solution
+    # the reference's key: the rewrite back to the first mesh goes unseen
+    ctx.set_topology_check(H.TOPO_POINTERS)
+    for b, o in zip(bufs, old):
+        b[:] = o
+    ctx.set_topology(m)
+    psi_stale = np.zeros(m.n)
+    ctx.solve(H.PCG, m.diag, m.upper, None, psi_stale, m.source, ctl)
+    assert np.abs(psi_stale - psi_a).max() > 1e-3 * scale
+    # back to the content check: the same call now uploads the first mesh
+    ctx.set_topology_check(H.TOPO_CONTENT)
+    ctx.set_topology(m)
+    psi_again = np.zeros(m.n)
+    ctx.solve(H.PCG, m.diag, m.upper, None, psi_again, m.source, ctl)
+    np.testing.assert_array_equal(psi_again, psi_a)
+
+
+def test_same_mesh_at_new_addresses_is_not_uploaded_again(ctx):
+    """The content key ignores where the arrays live: the same mesh passed in
+    fresh copies gives the same solve (and the check itself must not fail)."""
+    m = mg.poisson(9, 8, 7, seed=12, lognormal=True)
+    ctx.set_topology(m)
+    psi = np.zeros(m.n)
+    ctx.solve(H.PCG, m.diag, m.upper, None, psi, m.source)
+    m2 = mg.LduMatrix(m.n, m.lower_addr.copy(), m.upper_addr.copy(), m.owner_start.copy(),
+                      m.losort.copy(), m.losort_start.copy(), m.diag, m.upper, None, m.source)
+    ctx.set_topology(m2)
+    psi2 = np.zeros(m.n)
+    ctx.solve(H.PCG, m.diag, m.upper, None, psi2, m.source)
+    np.testing.assert_array_equal(psi2, psi)
+    with pytest.raises(H.HipFoamError):
+        ctx.set_topology_check(7)

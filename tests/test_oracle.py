@@ -332,3 +332,17 @@ def test_refined_solve_pinned_projection(orc):
     _, pz2 = orc.refined_solve(mz, ctl=orc.controls(deflate=2, **ctl))
     assert pz3.converged and pz2.converged and not pz3.polished
     assert abs(pz3.nIterations - pz2.nIterations) <= max(3, pz2.nIterations // 10)
+
+
+def test_pcg_diag_par_matches_serial_loop(orc):
+    """The all-cores CPU baseline (OpenFOAM PCG with the diagonal
+    preconditioner, row-gather SpMV over threads) follows the serial loop:
+    same normFactor, iteration count within one, same solution."""
+    m = mg.poisson(11, 9, 8, seed=31, lognormal=True)
+    x1, p1 = orc.pcg(m, precon=orc.PRE_DIAGONAL, tol=1e-10)
+    x2, p2 = orc.pcg_diag_par(m, tol=1e-10)
+    assert p2.converged and abs(p1.nIterations - p2.nIterations) <= 1
+    assert p2.normFactor == pytest.approx(p1.normFactor, rel=1e-13)
+    assert np.abs(x1 - x2).max() <= 1e-8 * np.abs(x1).max()
+    r = m.source - mg.dense(m) @ x2
+    assert np.abs(r).sum() / p2.normFactor == pytest.approx(p2.finalResidual, rel=1e-6)

@@ -30,8 +30,10 @@ MUTANTS = [
      "beta = first ? 0.f : (t3 / rA0rAold);",
      kp.pin_inner_bicgstab_two_eigenvalues),
     ("openfoam_pcg_beta_zero",
-     "const double b = wArA / wArAold;",
-     "const double b = 0.0 * (wArA / wArAold);",
+     "const double b = wArA / wArAold;\n                for (int32_t i = 0; i < n; ++i) "
+     "pA[i] = wA[i] + b * pA[i];",
+     "const double b = 0.0 * (wArA / wArAold);\n                for (int32_t i = 0; i < n; "
+     "++i) pA[i] = wA[i] + b * pA[i];",
      kp.pin_pcg_two_eigenvalues),
     ("openfoam_pbicgstab_beta_without_alpha_over_omega",
      "const double beta = (rA0rA / rA0rAold) * (alpha / omega);",
@@ -54,8 +56,10 @@ MUTANTS = [
      "omega = tAsA / (tAtA + tAsA);",
      kp.pin_inner_bicgstab_iterates_match_scipy),
     ("openfoam_pcg_unpreconditioned_direction",
-     "for (int32_t i = 0; i < n; ++i) pA[i] = wA[i] + b * pA[i];",
-     "for (int32_t i = 0; i < n; ++i) pA[i] = rA[i] + b * pA[i];",
+     "const double b = wArA / wArAold;\n                for (int32_t i = 0; i < n; ++i) "
+     "pA[i] = wA[i] + b * pA[i];",
+     "const double b = wArA / wArAold;\n                for (int32_t i = 0; i < n; ++i) "
+     "pA[i] = rA[i] + b * pA[i];",
      kp.pin_pcg_iterates_match_scipy),
     ("openfoam_pbicgstab_omega_from_s",
      "omega = dotd(n, tA, sA) / tAtA;",
