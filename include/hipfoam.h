@@ -347,10 +347,12 @@ int32_t hipfoam_synchronize(hipfoam_context* ctx);
  *       coefficients in slot-major arrays (a symmetric matrix stores each
  *       face once, on its owner side), columns from per-256-row-tile bases
  *       (a 1-byte presence mask per row where every face sits at its base,
- *       int16 deltas elsewhere), no row-start indirection; used when every
- *       row has <= 3 owner and <= 3 neighbour faces and each tile's column
- *       offsets fit the bases, otherwise schedule 0 runs.  Same terms in the
- *       same order as 0 and 1, so all three give identical results,
+ *       int16 deltas elsewhere), no row-start indirection.  A tile with a
+ *       row of more than 3 owner or 3 neighbour faces, or column offsets
+ *       beyond the deltas' range, is a face tile: its rows run the face
+ *       loops of schedule 0 (hipfoam_slot_tiles counts them); when every
+ *       tile would be one, schedule 0 runs.  Same terms in the same order as
+ *       0 and 1, so all three give identical results,
  * and the iteration driver:
  *   graphChunk = iterations captured per hipGraph launch (even), 0 = eager
  *                launches, -1 = auto (the default): eager from 524,288 rows
@@ -399,6 +401,15 @@ int32_t hipfoam_active_schedule(const hipfoam_context* ctx,
  * 0 when the mesh does not allow row slots.  No reference counterpart. */
 int32_t hipfoam_slot_layout(const hipfoam_context* ctx, int64_t* patternedRows,
                             int32_t* symmetricPlanes);
+/* How the current mesh's rows split over the tile kinds of the row-slot
+ * layout: patterned tiles (1 byte of addressing per row), explicit tiles
+ * (int16 deltas) and face tiles, whose rows run the LDU face loops because
+ * they do not fit the slots (a row with more than 3 owner or 3 neighbour
+ * faces, or column offsets scattered beyond the deltas' range).  Without a
+ * row-slot layout every row is counted in *faceRows.  No reference
+ * counterpart. */
+int32_t hipfoam_slot_tiles(const hipfoam_context* ctx, int64_t* patternedRows,
+                           int64_t* explicitRows, int64_t* faceRows);
 int32_t hipfoam_set_schedule(hipfoam_context* ctx, const hipfoam_schedule* s);
 /* Times the candidate SpMV schedules on the staged operator and keeps the
  * fastest; *best (may be NULL) receives it. */

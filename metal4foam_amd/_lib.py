@@ -134,6 +134,8 @@ SIGNATURES = {
     "hipfoam_get_schedule": (C.c_int32, [_P, C.POINTER(Schedule)]),
     "hipfoam_active_schedule": (C.c_int32, [_P, C.POINTER(C.c_int32)]),
     "hipfoam_slot_layout": (C.c_int32, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "hipfoam_slot_tiles": (C.c_int32, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64)]),
     "hipfoam_set_schedule": (C.c_int32, [_P, C.POINTER(Schedule)]),
     "hipfoam_autotune": (C.c_int32, [_P, C.POINTER(Schedule)]),
     "hipfoam_set_kernel_timing": (C.c_int32, [_P, C.c_int32, C.c_int32]),

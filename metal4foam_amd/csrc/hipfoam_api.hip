@@ -149,6 +149,7 @@ struct hipfoam_context {
   int maxOwn = 0, maxNbr = 0;  // longest owner / neighbour face run of a row
   bool slotOK = false;  // row-slot layout possible (see build_slots)
   int64_t slotPatterned = 0;  // rows in patterned tiles
+  int64_t slotFaceRows = 0;   // rows in face tiles (LDU face loops)
   int64_t strideMajor = 0, strideMinor = 0;  // dominant offsets (SlotLayout)
   Buf<uint32_t> slotOff;
   Buf<int32_t> slotTile;
@@ -1791,6 +1792,7 @@ int32_t hipfoam_set_topology(hipfoam_context* c, int32_t nCells, int32_t nFaces,
   SlotLayout lay;
   c->slotOK = build_slots(nCells, ownStart, uppAddr, losortStart, losort, lowAddr, lay);
   c->slotPatterned = c->slotOK ? lay.patternedRows : 0;
+  c->slotFaceRows = c->slotOK ? lay.faceRows : 0;
   c->strideMajor = c->slotOK ? lay.strideMajor : 0;
   c->strideMinor = c->slotOK ? lay.strideMinor : 0;
   // the cache key is set once the upload has succeeded
@@ -2163,6 +2165,17 @@ int32_t hipfoam_slot_layout(const hipfoam_context* c, int64_t* patternedRows,
   if (!c || !patternedRows || !symmetricPlanes) return HIPFOAM_ERR_ARG;
   *patternedRows = c->slotOK ? c->slotPatterned : 0;
   *symmetricPlanes = c->slotOK ? kShortRow : 0;
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_slot_tiles(const hipfoam_context* c, int64_t* patternedRows,
+                           int64_t* explicitRows, int64_t* faceRows) {
+  if (!c) return HIPFOAM_ERR_ARG;
+  const int64_t pat = c->slotOK ? c->slotPatterned : 0;
+  const int64_t fac = c->slotOK ? c->slotFaceRows : (int64_t)c->n;
+  if (patternedRows) *patternedRows = pat;
+  if (faceRows) *faceRows = fac;
+  if (explicitRows) *explicitRows = (int64_t)c->n - pat - fac;
   return HIPFOAM_OK;
 }
 

@@ -143,8 +143,8 @@ struct RhsPartials {
 template <typename XF>
 __device__ __forceinline__ float batch_row(const LduOp<float>& A, const RowSlots<float>& rs,
                                            long doff, int i, float xi, XF xf) {
-  if (A.slots) return rs.apply(A.diag + doff, i, xi, xf);
-  return ldu_row(seg_op(A, doff), i, xi, xf);
+  if (A.slots && !rs.faces) return rs.apply(A.diag + doff, i, xi, xf);
+  return ldu_row_faces(seg_op(A, doff), i, xi, xf);
 }
 
 // y = A x for every RHS, with each segment's diagonal plane: the timing

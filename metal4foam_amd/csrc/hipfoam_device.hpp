@@ -339,10 +339,12 @@ struct RowSlots {
   T c[2 * kShortRow];
   int col[2 * kShortRow];  // a valid row index for every slot (the row itself if empty)
   unsigned m;              // bit q: slot q holds a face
+  bool faces;              // the row is in a face tile: it runs the face loops
   // row i of slot tile t (i / kSlotTile, or the batch's work tile)
   __device__ __forceinline__ void load(const LduOp<T>& A, int i, int t) {
     const size_t n = (size_t)A.n;
-    // the row's own coefficients first: independent coalesced loads
+    // the row's own coefficients first: independent coalesced loads (in
+    // bounds for every row; a face tile's are not used)
 #pragma unroll
     for (int q = 0; q < 2 * kShortRow; ++q)
       if (q < kShortRow || !A.slotSym) c[q] = A.slotCoef[q * n + i];
@@ -357,6 +359,15 @@ struct RowSlots {
                                          int i) {
     const size_t n = (size_t)A.n;
     const int flags = d[6];
+    faces = (flags & kSlotFaces) != 0;
+    if (faces) return;
+    // symmetric: a neighbour face's coefficient is stored once, in its owner's
+    // row, except in tiles whose neighbour planes hold them
+    const bool gather = A.slotSym && !(flags & kSlotNbrPlanes);
+    if (A.slotSym && !gather) {
+#pragma unroll
+      for (int q = kShortRow; q < 2 * kShortRow; ++q) c[q] = A.slotCoef[q * n + i];
+    }
     unsigned own;
     if (flags & kSlotPatterned) {
       // columns from the tile bases alone: the gathers need not wait for the
@@ -375,8 +386,7 @@ struct RowSlots {
         m |= (o[q] != 0 ? 1u : 0u) << q;
       }
     }
-    if (A.slotSym) {
-      // a neighbour face's coefficient is stored once, in its owner's row
+    if (gather) {
 #pragma unroll
       for (int q = kShortRow; q < 2 * kShortRow; ++q) {
         const unsigned sq = (own >> (2 * (q - kShortRow))) & 3u;
@@ -401,16 +411,10 @@ struct RowSlots {
   }
 };
 
-// Row i of A x; t is row i's slot tile (ldu_row below: i / kSlotTile).
+// Row i of A x through the LDU face arrays: the schedules without row slots,
+// and the rows of face tiles.
 template <typename T, typename XF>
-__device__ __forceinline__ T ldu_row_t(const LduOp<T>& A, int i, int t, T xi, XF xf) {
-  if (A.slots) {
-    // coefficients and offsets are independent coalesced loads (no row-start
-    // indirection)
-    RowSlots<T> rs;
-    rs.load(A, i, t);
-    return rs.apply(A.diag, i, xi, xf);
-  }
+__device__ __forceinline__ T ldu_row_faces(const LduOp<T>& A, int i, T xi, XF xf) {
   const int f0 = A.ownStart[i], f1 = A.ownStart[i + 1];
   const int k0 = A.losortStart[i], k1 = A.losortStart[i + 1];
   T s = A.diag[i] * xi;
@@ -452,6 +456,19 @@ __device__ __forceinline__ T ldu_row_t(const LduOp<T>& A, int i, int t, T xi, XF
     }
   }
   return s;
+}
+
+// Row i of A x; t is row i's slot tile (ldu_row below: i / kSlotTile).
+template <typename T, typename XF>
+__device__ __forceinline__ T ldu_row_t(const LduOp<T>& A, int i, int t, T xi, XF xf) {
+  if (A.slots) {
+    // coefficients and offsets are independent coalesced loads (no row-start
+    // indirection)
+    RowSlots<T> rs;
+    rs.load(A, i, t);
+    if (!rs.faces) return rs.apply(A.diag, i, xi, xf);
+  }
+  return ldu_row_faces(A, i, xi, xf);
 }
 
 template <typename T, typename XF>
@@ -504,12 +521,13 @@ __device__ __forceinline__ T slot_row(const LduOp<T>& A, const TileDesc& d, int 
     // columns from the tile bases alone (an absent slot is clamped to a valid
     // row); the neighbour faces' owner slots are the tile's (flags >> 8)
     const unsigned own = (unsigned)d.flags >> 8;
+    const bool gather = sym && !(d.flags & kSlotNbrPlanes);
 #pragma unroll
     for (int q = 0; q < S; ++q) col[q] = min(max(i + d.base[q], 0), A.n - 1);
 #pragma unroll
     for (int q = K; q < S; ++q) {
-      const int plane = sym ? (int)((own >> (2 * (q - K))) & 3u) : q;
-      c[q] = C[(size_t)plane * n + (sym ? col[q] : i)];
+      const int plane = gather ? (int)((own >> (2 * (q - K))) & 3u) : q;
+      c[q] = C[(size_t)plane * n + (gather ? col[q] : i)];
     }
     m = rb;
   } else {
@@ -527,10 +545,13 @@ __device__ __forceinline__ T slot_row(const LduOp<T>& A, const TileDesc& d, int 
       m |= (o0 != 0 ? 1u : 0u) << (2 * p);
       m |= (o1 != 0 ? 1u : 0u) << (2 * p + 1);
     }
+    // symmetric tiles flagged kSlotNbrPlanes hold their neighbour
+    // coefficients in their own planes 3..5, as asymmetric operators do
+    const bool gather = sym && !(d.flags & kSlotNbrPlanes);
 #pragma unroll
     for (int q = K; q < S; ++q) {
-      const size_t plane = sym ? ((rb >> (2 * (q - K))) & 3u) : (unsigned)q;
-      c[q] = C[plane * n + (sym ? col[q] : i)];
+      const size_t plane = gather ? ((rb >> (2 * (q - K))) & 3u) : (unsigned)q;
+      c[q] = C[plane * n + (gather ? col[q] : i)];
     }
   }
   T s = A.diag[i] * xi;
@@ -561,6 +582,8 @@ __device__ __forceinline__ void for_rows(const LduOp<T>& A, int r0, int r1, Fn f
       if (i >= r1) continue;
       if (d.flags & kSlotPatterned)
         fn(i, [&](T xi, auto xf) { return slot_row<T, true>(A, d, i, xi, xf); });
+      else if (d.flags & kSlotFaces)
+        fn(i, [&](T xi, auto xf) { return ldu_row_faces(A, i, xi, xf); });
       else
         fn(i, [&](T xi, auto xf) { return slot_row<T, false>(A, d, i, xi, xf); });
     }
@@ -1221,15 +1244,20 @@ __global__ __launch_bounds__(256) void k_build_slot_coeffs(
     if (i >= n) return;
   }
   const size_t nn = (size_t)n;
+  const int32_t* d = slotTile + (size_t)t * kSlotDesc;
+  // face tiles have no slots; symmetric tiles flagged kSlotNbrPlanes store
+  // their neighbour coefficients too
+  if (d[6] & kSlotFaces) return;
+  const bool nbr = !sym || (d[6] & kSlotNbrPlanes);
   int o[2 * kShortRow];
   unsigned own;
-  slot_decode(slotTile + (size_t)t * kSlotDesc, slotRow[i], slotOff, nn, i, o, own);
+  slot_decode(d, slotRow[i], slotOff, nn, i, o, own);
   int f = ownStart[i], k = losortStart[i];
 #pragma unroll
   for (int q = 0; q < 2 * kShortRow; ++q) {
     T v = T(0);
     if (o[q] != 0) v = q < kShortRow ? upper[f++] : lowerL[k++];
-    if (q < kShortRow || !sym) slotCoef[q * nn + i] = v;
+    if (q < kShortRow || nbr) slotCoef[q * nn + i] = v;
   }
 }
 

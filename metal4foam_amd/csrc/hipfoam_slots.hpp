@@ -22,6 +22,15 @@ constexpr int kSlotTile = 256;
 constexpr int kSlotEmpty = -32768;
 constexpr int kSlotDesc = 8;       // int32 words per tile descriptor
 constexpr int kSlotPatterned = 1;  // descriptor flag: every face at its slot base
+// descriptor flag: a face tile, whose rows do not fit the slots (a row with
+// more than kShortRow faces on a side, or offsets beyond the int16 deltas);
+// its rows run the LDU face loops
+constexpr int kSlotFaces = 2;
+// descriptor flag: a neighbour face of a row of this tile is owned by a row of
+// a face tile, which has no owner slots; symmetric operators then store this
+// tile's neighbour coefficients in planes 3..5 of its own rows, as
+// asymmetric ones do
+constexpr int kSlotNbrPlanes = 4;
 
 // Column offsets of row i's slots (0: an empty slot) from its tile's
 // descriptor d and its slot byte rb, and the packed owner slots of the row's
@@ -32,7 +41,11 @@ __host__ __device__ __forceinline__ void slot_decode(const int32_t* d, unsigned 
                                                      size_t n, int i,
                                                      int (&o)[2 * kShortRow], unsigned& own) {
   const int flags = d[6];
-  if (flags & kSlotPatterned) {
+  if (flags & kSlotFaces) {
+#pragma unroll
+    for (int q = 0; q < 2 * kShortRow; ++q) o[q] = 0;
+    own = 0;
+  } else if (flags & kSlotPatterned) {
 #pragma unroll
     for (int q = 0; q < 2 * kShortRow; ++q) o[q] = ((rb >> q) & 1u) ? d[q] : 0;
     own = (unsigned)flags >> 8;
@@ -57,6 +70,8 @@ struct SlotLayout {
   std::vector<uint32_t> words;  // kShortRow * n: int16 delta pairs
   std::vector<uint8_t> row;     // n: presence mask or packed owner slots
   int64_t patternedRows = 0;    // rows in patterned tiles
+  int64_t faceRows = 0;         // rows in face tiles (LDU face loops)
+  int64_t nbrPlaneRows = 0;     // rows in tiles flagged kSlotNbrPlanes
   // the most common largest and second-largest owner offsets of the
   // patterned tiles (the z and y strides of a structured block), 0 if none
   int64_t strideMajor = 0, strideMinor = 0;
@@ -77,8 +92,12 @@ struct SlotLayout {
 // over the three slots in increasing order.  A row takes the
 // placement of its faces that keeps its largest |delta| smallest.  A tile
 // whose deltas are all zero, and whose neighbour faces of each slot all sit
-// in the same owner slot of their owner rows, is patterned.  False when a row
-// has more than kShortRow faces on a side or no candidate fits a tile.
+// in the same owner slot of their owner rows, is patterned.  A tile with a
+// row of more than kShortRow faces on a side, or where no candidate fits,
+// becomes a face tile (its rows run the face loops); a slot tile with a
+// neighbour face owned by a face-tile row is flagged kSlotNbrPlanes.  False
+// when every tile is a face tile, or when a losort face is not among its
+// owner's faces (inconsistent addressing).
 // Tiles are kSlotTile-row runs from row 0, or, with tileStart, the ntiles
 // runs [tileStart[t], tileStart[t + 1]) of at most kSlotTile rows (a batch's
 // work tiles, which never straddle two systems).
@@ -101,7 +120,7 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
   L.tile.assign((size_t)nt * kSlotDesc, 0);
   L.words.assign((size_t)K * n, 0);
   L.row.assign((size_t)n, 0);
-  L.patternedRows = 0;
+  L.patternedRows = L.faceRows = L.nbrPlaneRows = 0;
   L.strideMajor = L.strideMinor = 0;
   std::vector<int16_t> delta((size_t)S * n, (int16_t)kSlotEmpty);
   // offsets of row i on one side (owner: side 0, neighbour: side 1), in face order
@@ -156,18 +175,22 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
     }
     return true;
   };
-  std::vector<uint8_t> zeroTile((size_t)nt, 1);
+  std::vector<uint8_t> zeroTile((size_t)nt, 1), faceTile((size_t)nt, 0);
   for (int t = 0; t < nt; ++t) {
     int i0, i1;
     tile_rows(t, i0, i1);
-    for (int side = 0; side < 2; ++side) {
+    bool faces = false;
+    for (int side = 0; side < 2 && !faces; ++side) {
       for (int q = 0; q < K; ++q) most[q].clear();
       all.clear();
       int mcnt = 0;
-      for (int i = i0; i < i1; ++i) {
+      for (int i = i0; i < i1 && !faces; ++i) {
         int64_t off[K];
         const int cnt = offsets(i, side, off);
-        if (cnt > K) return false;
+        if (cnt > K) {
+          faces = true;
+          break;
+        }
         if (cnt > mcnt) {
           mcnt = cnt;
           for (int q = 0; q < K; ++q) most[q].clear();
@@ -214,6 +237,7 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
       } else if (nc == 3) {
         cand.push_back({centre[0], centre[1], centre[2]});
       }
+      if (faces) break;
       int chosen = -1;
       bool chosenZero = false;
       for (size_t c = 0; c < cand.size(); ++c) {
@@ -225,17 +249,40 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
         }
         if (zero) break;
       }
-      if (chosen < 0) return false;
+      if (chosen < 0) {
+        faces = true;
+        break;
+      }
+      for (int q = 0; q < K; ++q) {
+        const int64_t b = cand[chosen][q];
+        if (b < INT32_MIN || b > INT32_MAX) faces = true;
+      }
+      if (faces) break;
       // the rows hold the last placement tried: place them on the chosen bases again
       bool zero = false;
       try_bases(i0, i1, side, cand[chosen].data(), zero);
-      for (int q = 0; q < K; ++q) {
-        const int64_t b = cand[chosen][q];
-        if (b < INT32_MIN || b > INT32_MAX) return false;
-        L.tile[(size_t)t * kSlotDesc + side * K + q] = (int32_t)b;
-      }
+      for (int q = 0; q < K; ++q)
+        L.tile[(size_t)t * kSlotDesc + side * K + q] = (int32_t)cand[chosen][q];
       zeroTile[t] = zeroTile[t] && chosenZero;
     }
+    if (faces) {
+      // no slots in this tile: its rows run the face loops
+      faceTile[t] = 1;
+      zeroTile[t] = 0;
+      for (int q = 0; q < kSlotDesc; ++q) L.tile[(size_t)t * kSlotDesc + q] = 0;
+      for (int i = i0; i < i1; ++i)
+        for (int q = 0; q < S; ++q) delta[(size_t)i * S + q] = (int16_t)kSlotEmpty;
+    }
+  }
+  int64_t nFaceTiles = 0;
+  for (int t = 0; t < nt; ++t) nFaceTiles += faceTile[t];
+  if (nt > 0 && nFaceTiles == nt) return false;
+  // slot tile of every row
+  std::vector<int32_t> tileOf((size_t)n, 0);
+  for (int t = 0; t < nt; ++t) {
+    int i0, i1;
+    tile_rows(t, i0, i1);
+    for (int i = i0; i < i1; ++i) tileOf[i] = t;
   }
   // owner slot of every face, then each row's packed owner slots of its
   // neighbour faces (2 bits per neighbour slot)
@@ -245,7 +292,7 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
     for (int q = 0; q < K; ++q)
       if (delta[(size_t)i * S + q] != kSlotEmpty) ownSlot[f++] = (int8_t)q;
   }
-  std::vector<uint8_t> packed((size_t)n, 0);
+  std::vector<uint8_t> packed((size_t)n, 0), nbrPlanes((size_t)nt, 0);
   std::map<std::pair<int64_t, int64_t>, int64_t> strides;
   for (int i = 0; i < n; ++i) {
     int k = losortStart[i];
@@ -254,7 +301,12 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
       if (delta[(size_t)i * S + q] == kSlotEmpty) continue;
       const int f = losort[k++];
       const int col = lowAddr[f];
-      if (f < ownStart[col] || f >= ownStart[col + 1] || ownSlot[f] < 0) return false;
+      if (col < 0 || col >= n || f < ownStart[col] || f >= ownStart[col + 1]) return false;
+      if (ownSlot[f] < 0) {
+        // owned by a face-tile row: the coefficient is stored in this row
+        nbrPlanes[tileOf[i]] = 1;
+        continue;
+      }
       pk |= (unsigned)ownSlot[f] << (2 * (q - K));
     }
     packed[i] = (uint8_t)pk;
@@ -262,20 +314,29 @@ inline bool build_slots(int n, const int32_t* ownStart, const int32_t* uppAddr,
   for (int t = 0; t < nt; ++t) {
     int i0, i1;
     tile_rows(t, i0, i1);
-    // patterned: zero deltas, and one owner slot per neighbour slot
+    if (faceTile[t]) {
+      L.tile[(size_t)t * kSlotDesc + 6] = kSlotFaces;
+      L.faceRows += i1 - i0;
+      continue;
+    }
+    if (nbrPlanes[t]) L.nbrPlaneRows += i1 - i0;
+    // patterned: zero deltas, and one owner slot per neighbour slot (the
+    // latter only where symmetric operators gather from the owner rows)
     bool pat = zeroTile[t];
     int ownQ[K] = {-1, -1, -1};
-    for (int i = i0; i < i1 && pat; ++i)
+    for (int i = i0; i < i1 && pat && !nbrPlanes[t]; ++i)
       for (int q = K; q < S && pat; ++q) {
         if (delta[(size_t)i * S + q] == kSlotEmpty) continue;
         const int s = (packed[i] >> (2 * (q - K))) & 3;
         if (ownQ[q - K] < 0) ownQ[q - K] = s;
         pat = ownQ[q - K] == s;
       }
+    if (!pat) L.tile[(size_t)t * kSlotDesc + 6] = nbrPlanes[t] ? kSlotNbrPlanes : 0;
     if (pat) {
       unsigned pk = 0;
       for (int q = 0; q < K; ++q) pk |= (unsigned)std::max(ownQ[q], 0) << (2 * q);
-      L.tile[(size_t)t * kSlotDesc + 6] = kSlotPatterned | (int32_t)(pk << 8);
+      L.tile[(size_t)t * kSlotDesc + 6] =
+          kSlotPatterned | (nbrPlanes[t] ? kSlotNbrPlanes : 0) | (int32_t)(pk << 8);
       L.patternedRows += i1 - i0;
       // the tile's two largest owner offsets
       int64_t o[K];

@@ -311,6 +311,14 @@ class Context:
                     "slot_layout")
         return rows.value, planes.value
 
+    def slot_tiles(self) -> tuple:
+        """(patterned, explicit, face-loop) rows of the current mesh's row-slot
+        layout (every row a face-loop row without one)."""
+        p, e, f = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.lib.hipfoam_slot_tiles(self.h, C.byref(p), C.byref(e), C.byref(f)),
+                    "slot_tiles")
+        return p.value, e.value, f.value
+
     def set_schedule(self, **kw):
         s = self.get_schedule()
         for k, v in kw.items():

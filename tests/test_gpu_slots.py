@@ -2,8 +2,8 @@
 order as the face-loop schedules 0 and 1, so SpMV results, inner solves and
 refined solves must agree bit for bit; meshes that do not allow the layout
 (a row with more than three owner or neighbour faces, or one slot's column
-offsets spanning more than an int16 delta within a 256-row tile) fall back to
-schedule 0."""
+offsets spanning more than an int16 delta within a 256-row tile) run those
+tiles on the face loops, and fall back to schedule 0 when no tile fits."""
 
 import numpy as np
 import pytest
@@ -131,12 +131,17 @@ def test_spmv_schedules_bitwise_equal(ctx, case):
         assert 0 < m.n - patterned < m.n // 10
     elif case == "face_order":
         assert patterned == 0 and planes == 3
+    elif case == "scattered":
+        # offsets beyond int16 in most tiles: those run the face loops
+        assert patterned == 0 and planes == 3
+        _, explicit, faces = ctx.slot_tiles()
+        assert explicit > 0 and faces > m.n // 2
     else:
         assert (patterned, planes) == (0, 0)
     x = np.random.default_rng(9).standard_normal(m.n).astype(np.float32)
     for lower in (None, m.lower):
         ys = spmv_all_schedules(ctx, m, x, lower)
-        assert ctx.active_schedule() == (0 if case in ("irregular", "scattered") else 2)
+        assert ctx.active_schedule() == (0 if case == "irregular" else 2)
         np.testing.assert_array_equal(ys[2], ys[0])
         np.testing.assert_array_equal(ys[1], ys[0])
         # and against an FP64 product, to the FP32 rounding bound

@@ -110,3 +110,24 @@ def test_generators_are_deterministic():
     np.testing.assert_array_equal(a.source, b.source)
     c, d = mg.poisson(5, 5, 5, 7), mg.poisson(5, 5, 5, 7)
     np.testing.assert_array_equal(c.upper, d.upper)
+
+
+def test_renumbering_is_a_symmetric_permutation():
+    """renumber() gives P A P^T with the right-hand side permuted alike, in
+    OpenFOAM's face order; RCM restores short rows after a random order."""
+    m = mg.convection_diffusion(7, 6, 5, seed=3)
+    perm = np.random.default_rng(5).permutation(m.n)
+    r = mg.renumber(m, perm)
+    P = np.zeros((m.n, m.n))
+    P[perm, np.arange(m.n)] = 1.0
+    np.testing.assert_allclose(mg.dense(r), P @ mg.dense(m) @ P.T, rtol=0, atol=0)
+    np.testing.assert_array_equal(r.source[perm], m.source)
+    assert np.all(r.lower_addr < r.upper_addr)
+    assert np.all(np.diff(r.lower_addr) >= 0)
+    x = np.linalg.solve(mg.dense(m), m.source)
+    np.testing.assert_allclose(np.linalg.solve(mg.dense(r), r.source)[perm], x, rtol=1e-12)
+    rnd = mg.random_renumbering(mg.poisson(12, 11, 10, seed=1), 2)
+    rows = lambda q: np.diff(q.owner_start)  # noqa: E731
+    assert (rows(rnd) > 3).any()
+    rcm = mg.rcm_renumbering(rnd)
+    assert (rows(rcm) <= 3).all() and (np.diff(rcm.losort_start) <= 3).all()

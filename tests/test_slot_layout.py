@@ -5,8 +5,11 @@ its per-row decode (the function the kernels call) over each mesh and
 counts rows whose decoded slots differ from the face loops: owner faces in
 face order, neighbour faces in losort order, and each neighbour slot's packed
 owner slot pointing at the same face.  Structured blocks must come out fully
-patterned (1 byte of addressing per row); meshes with rows of more than three
-faces a side must be refused (the library then runs the face loops)."""
+patterned (1 byte of addressing per row).  A tile holding a row of more than
+three faces a side, or offsets no base reaches, becomes a face tile whose
+rows run the face loops, with the rest of the mesh kept on slots; only a
+mesh whose every tile is a face tile is refused (the library then runs the
+face loops everywhere)."""
 
 import os
 import shutil
@@ -92,6 +95,20 @@ def scattered(n, seed):
     return from_pairs(n, pairs)
 
 
+def block_with_long_rows(nx, ny, nz, extra, seed):
+    """A structured block plus `extra` faces between random far-apart cells:
+    their rows have more than three faces a side (face tiles) and their
+    neighbours' tiles refer to owners without slots."""
+    n, low, upp, *_ = block(nx, ny, nz)
+    rng = np.random.default_rng(seed)
+    pairs = list(zip(low.tolist(), upp.tolist()))
+    for _ in range(extra):
+        a = int(rng.integers(0, n))
+        for d in (1, 2, 3):
+            pairs.append((a, (a + d * 4099) % n))
+    return from_pairs(n, pairs)
+
+
 def face_order_shuffled(nx, ny, nz, seed):
     """A structured block whose faces of one owner are not in neighbour order."""
     n, low, upp, *_ = block(nx, ny, nz)
@@ -119,6 +136,13 @@ CASES = {
     "renumbered": (lambda: renumbered(12, 11, 10, 4), "any"),
     "scattered": (lambda: scattered(3000, 5), "any"),
     "irregular": (lambda: irregular(2000, 6), "refused"),
+    # random cell numbering of a hex mesh: owner rows of up to six faces and
+    # offsets across the whole mesh: every tile holds a long row, so the whole
+    # mesh runs the face loops
+    "renumbered_hex": (lambda: renumbered(40, 40, 40, 7), "refused"),
+    # a structured block with a few long rows: a few face tiles, the rest on
+    # slots (patterned where nothing refers to a face tile)
+    "long_rows": (lambda: block_with_long_rows(44, 42, 40, 4, 8), "some_faces"),
     "one_cell": (lambda: block(1, 1, 1), "all"),
 }
 
@@ -172,11 +196,19 @@ def test_slot_layout_matches_face_loops(checker, tmp_path):
         if expect == "mixed":
             assert 0 < res["explicit"] < n // 10, name
             assert res["patterned"] + res["explicit"] == n
+        if expect in ("all", "mixed"):
+            assert res["faces"] == 0, name
+        if expect == "faces":
+            assert res["faces"] > n // 2, name
+        if expect == "some_faces":
+            assert 0 < res["faces"] < n // 10, name
+            assert res["nbrplanes"] > 0 and res["patterned"] > 0.8 * n, name
+            assert res["patterned"] + res["explicit"] + res["faces"] == n
 
 
 def test_slot_layout_empty_mesh(checker, tmp_path):
     z = np.zeros(0, dtype=np.int32)
     res = run_checker(checker, tmp_path, [(0, z, z, np.zeros(1, np.int32), z,
                                            np.zeros(1, np.int32))])
-    assert res == [{"slots": 1, "patterned": 0, "explicit": 0, "errors": 0, "major": 0,
-                    "minor": 0}]
+    assert res == [{"slots": 1, "patterned": 0, "explicit": 0, "faces": 0, "nbrplanes": 0,
+                    "errors": 0, "major": 0, "minor": 0}]
