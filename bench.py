@@ -295,7 +295,7 @@ def extra_mesh(H, mg, args, gen_name, gen_args, solver, tag, what, schedule=None
     """Kernel rooflines (and the time to 1e-6) of another mesh, on a context
     of its own: the past-MALL mesh, the forced LDU face schedule, the
     convection-diffusion BiCGStab kernels."""
-    m = getattr(mg, gen_name)(*gen_args)
+    m = gen_name(mg) if callable(gen_name) else getattr(mg, gen_name)(*gen_args)
     ctx = H.Context(0)
     ctx.set_topology(m)
     if schedule is not None:
@@ -322,9 +322,11 @@ def extra_mesh(H, mg, args, gen_name, gen_args, solver, tag, what, schedule=None
     ms = 1e3 * (time.perf_counter() - t0)
     _, solve_k, spmv = measure_kernels(H, ctx, m, solver, args.spmv_reps, tag, solve)
     rsum, _ = numpy_residual(m, psi.download())
+    pat, expl, frows = ctx.slot_tiles()
     out = {"what": what, "mesh": m.name, "cells": m.n, "faces": m.nfaces,
            "schedule": ctx.get_schedule().as_dict(),
            "active_spmv_schedule": ctx.active_schedule(),
+           "rows_patterned_explicit_facetile": [pat, expl, frows],
            "solve_ms": round(ms, 3), "iterations": p.nIterations,
            "converged": bool(p.converged and rsum / p.normFactor < 1e-6),
            "solve_kernel": solve_k, "spmv": spmv}
@@ -563,6 +565,16 @@ def main():
             "convdiff96_bicgstab": extra_mesh(H, mg, args, "config2", (), "bicg", "convdiff96",
                                               "96^3 convection-diffusion, refined "
                                               "hipPBiCGStab: the BiCGStab SpMV kernels"),
+            "random_order128": extra_mesh(
+                H, mg, args, lambda g: g.random_renumbering(g.config1(), 11), (), "pcg",
+                "poisson128_random", "the 128^3 Poisson with its cells in a random order: "
+                "every tile holds a row of more than 3 owner faces, so the whole mesh runs "
+                "the LDU face loops (and its gathers have no locality)"),
+            "rcm_order128": extra_mesh(
+                H, mg, args, lambda g: g.rcm_renumbering(g.random_renumbering(g.config1(), 11)),
+                (), "pcg", "poisson128_rcm", "the random order renumbered by reverse "
+                "Cuthill-McKee (as OpenFOAM's renumberMesh can): rows of at most 3 faces a "
+                "side again, row slots in explicit tiles"),
         }
 
     cpu = None

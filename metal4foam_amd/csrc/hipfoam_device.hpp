@@ -993,7 +993,36 @@ __global__ __launch_bounds__(1024) void k_bicg_update_rho(int n, int xcdRemap,
   const T omega = (T)st->omega;
   const bool copy = st->copyBest != 0;
   double v[2] = {0.0, 0.0};
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+  // four rows per thread at a time: every row's loads issue before any row's
+  // arithmetic (one memory round trip instead of four); the rows and their
+  // sums go in the order of the plain loop
+  constexpr int U = 4;
+  const int bd = blockDim.x;
+  int i0 = r0 + threadIdx.x;
+  for (; i0 + (U - 1) * bd < r1; i0 += U * bd) {
+    T d[U], y[U], z[U], s[U], t[U], a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * bd;
+      d[u] = delta[i];
+      y[u] = yA[i];
+      z[u] = zA[i];
+      s[u] = sA[i];
+      t[u] = tA[i];
+      a[u] = rA0[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * bd;
+      if (copy) deltaBest[i] = d[u];
+      delta[i] = d[u] + (alpha * y[u] + omega * z[u]);
+      const T ri = s[u] - omega * t[u];
+      r[i] = ri;
+      v[0] += (double)fabs(ri);
+      v[1] += (double)(a[u] * ri);
+    }
+  }
+  for (int i = i0; i < r1; i += bd) {
     const T d = delta[i];
     if (copy) deltaBest[i] = d;
     delta[i] = d + (alpha * yA[i] + omega * zA[i]);
