@@ -250,7 +250,7 @@ inline int row_grid(const hipfoam_context* c) {
 
 // Launch shape of the row kernels that reduce nothing (the interior
 // Chebyshev steps, the plain SpMV): no partials to fold, so the automatic
-// schedule keeps them at full occupancy, 256-thread workgroups, 8 per CU.
+// schedule is free to size them for occupancy.
 // Their results do not depend on the launch shape.
 // Automatic shape: 256-thread workgroups, 16 per CU (two rounds of resident
 // workgroups, 2 passes of rows each at 128^3), and from kStreamBigTiles

@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--write")
     ap.add_argument("--fetch-scale", type=float, default=1.998)
     ap.add_argument("--write-scale", type=float, default=1.0)
+    ap.add_argument("--note", action="append", default=[],
+                    help="a paragraph printed after the tables")
     args = ap.parse_args()
 
     geo, nrhs = batch_geometry()
@@ -166,6 +168,9 @@ def main():
           "planes + RHS x per-RHS vector bytes, per row: " + "; ".join(
               f"`{k}` {v[1]} B/RHS{' + slots' if v[2] else ''}{f' + {v[3]} B diag' if v[3] else ''}"
               for k, v in KERNELS.items()) + ".")
+    for n in args.note:
+        print()
+        print(n)
 
 
 if __name__ == "__main__":
