@@ -358,11 +358,14 @@ int32_t hipfoam_synchronize(hipfoam_context* ctx);
  *                launches, -1 = auto (the default): eager from 524,288 rows
  *                on, where the kernels outlast their launches, graphs of 4
  *                iterations below.  Results do not depend on it.
- * blockSize 0 (auto, the default) picks the launch shape from the mesh: one
- * 256-thread workgroup per 256-row tile up to 512 tiles, 512 workgroups of
- * 512 threads up to 49,152 tiles (every reduction's second stage then runs
- * inside the next row kernel), and 8 workgroups of 256 threads per CU
- * beyond.  An explicit blockSize launches min(ceil(n / blockSize),
+ * blockSize 0 (auto, the default) picks the launch shape from the mesh.
+ * Kernels that reduce: one 256-thread workgroup per 256-row tile up to 512
+ * tiles, 512 workgroups of 512 threads up to 49,152 tiles (every
+ * reduction's second stage then runs inside the next row kernel), and 16
+ * workgroups of 512 threads per CU beyond.  Kernels that reduce nothing (the
+ * interior Chebyshev steps, the plain SpMV): 16 workgroups of 256 threads
+ * per CU up to 32,768 tiles, 8 of 512 beyond.  An explicit blockSize
+ * launches min(ceil(n / blockSize),
  * gridLimit) workgroups; each walks a contiguous run of blockSize-row tiles,
  * and second stages run inside the next kernel when there are at most 512
  * workgroups of at least 256 threads.  Results are deterministic for a given

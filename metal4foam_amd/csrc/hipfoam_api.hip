@@ -213,15 +213,20 @@ namespace {
 //  - at most kAutoSmallTiles: one 256-thread workgroup per tile;
 //  - up to kAutoMidTiles: kAutoMidGrid workgroups of kAutoMidBlock threads,
 //    each walking a contiguous run of tiles;
-//  - beyond: 8 workgroups of 256 threads per CU.
+//  - beyond: kAutoBigPerCU workgroups of kAutoMidBlock threads per CU, with
+//    separate finalise launches.
 // The first two stay within kMergeMaxBlocks, so every reduction's second
 // stage runs inside the next row kernel.  Against 8 x 256 per CU (same box,
 // lognormal Poisson PCG unless noted): 128^3 16.2 -> 15.7 ms, 160^3 44.0 ->
 // 36.5 ms, 192^3 88.6 -> 83.3 ms, 224^3 170.3 -> 167.0 ms, 128^3 BiCGStab
-// 4.16 -> 3.86 ms, 96^3 BiCGStab 2.91 -> 2.80 ms; 96^3 7.26 vs 7.28 ms; at
-// 256^3 (65,536 tiles) 8 x 256 per CU is 3-6 % faster, where every pass
-// streams from HBM and full occupancy pays.  An explicit blockSize launches
-// min(ceil(n / blockSize), gridLimit) workgroups (gridLimit 0: 8 per CU).
+// 4.16 -> 3.86 ms, 96^3 BiCGStab 2.91 -> 2.80 ms; 96^3 7.26 vs 7.28 ms.  At
+// 256^3 (65,536 tiles) the merged grid is 3-6 % slower than full occupancy,
+// where every pass streams from HBM; there 16 x 512 per CU beats 8 x 256
+// per CU: 240^3 229.9 -> 225.4 ms, 256^3 273.1 -> 265.0 ms, 288^3 454.2 ->
+// 447.0 ms, 320^3 820.7 -> 791.8 ms (the same shape measured within 0.5 %
+// at 160^3 and 224^3 and 4 % slower at 192^3, so the mid rule stays).  An
+// explicit blockSize launches min(ceil(n / blockSize), gridLimit)
+// workgroups (gridLimit 0: 8 per CU).
 // Rows from which the automatic graph chunk (-1) launches eagerly.
 constexpr int kEagerRows = 1 << 19;
 constexpr int kMergeMaxBlocks = 512;
@@ -229,22 +234,21 @@ constexpr int kAutoSmallTiles = 512;
 constexpr int kAutoMidTiles = 49152;
 constexpr int kAutoMidBlock = 512;
 constexpr int kAutoMidGrid = 512;
+constexpr int kAutoBigPerCU = 16;
 
 inline int64_t tiles256(const hipfoam_context* c) { return ((int64_t)c->n + 255) / 256; }
 
 inline int row_block(const hipfoam_context* c) {
   if (c->sched.blockSize > 0) return c->sched.blockSize;
-  const int64_t t = tiles256(c);
-  return (t <= kAutoSmallTiles || t > kAutoMidTiles) ? 256 : kAutoMidBlock;
+  return tiles256(c) <= kAutoSmallTiles ? 256 : kAutoMidBlock;
 }
 
 inline int row_grid(const hipfoam_context* c) {
   const int bs = row_block(c);
   const int tiles = std::max(1, (c->n + bs - 1) / bs);
   int cap = c->sched.gridLimit > 0 ? c->sched.gridLimit : 8 * std::max(c->numCUs, 1);
-  if (c->sched.blockSize == 0 && c->sched.gridLimit <= 0 && tiles256(c) > kAutoSmallTiles &&
-      tiles256(c) <= kAutoMidTiles)
-    cap = kAutoMidGrid;
+  if (c->sched.blockSize == 0 && c->sched.gridLimit <= 0 && tiles256(c) > kAutoSmallTiles)
+    cap = tiles256(c) <= kAutoMidTiles ? kAutoMidGrid : kAutoBigPerCU * std::max(c->numCUs, 1);
   return std::min(tiles, cap);
 }
 
