@@ -971,10 +971,13 @@ int check_ready(hipfoam_context* c) {
 // metalFoamSolver::solveImpl with the FP64 steps on the device
 // -------------------------------------------------------------------------
 
+// kind: where psi, source and residual live; matKind: where diag, upper and
+// lower live (default: the same)
 int refined_solve(hipfoam_context* c, int solver, const double* diag, const double* upper,
                   const double* lower, double* psi, const double* source,
                   const hipfoam_controls* ctl, hipfoam_performance* perf, double* residual,
-                  hipMemcpyKind kind) {
+                  hipMemcpyKind kind, hipMemcpyKind matKind = hipMemcpyDefault) {
+  if (matKind == hipMemcpyDefault) matKind = kind;
   const double t0 = now_seconds();
   int rc = check_ready(c);
   if (rc) return rc;
@@ -989,7 +992,7 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
   rc = ensure_vectors(c);
   if (rc) return rc;
   // lower is used only by PBiCGStab on asymmetric matrices (metalFoamSolver.C:214-219)
-  rc = stage_matrix(c, diag, upper, lower, kind);
+  rc = stage_matrix(c, diag, upper, lower, matKind);
   if (rc) return rc;
   const double tStage = now_seconds() - t0;
   XF(c->psi0.p, psi, sizeof(double) * n, kind);
@@ -1123,7 +1126,8 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
 int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double* upper,
                const double* lower, double* psi, const double* source,
                const hipfoam_controls* ctl, hipfoam_performance* perf, double* residual,
-               hipMemcpyKind kind) {
+               hipMemcpyKind kind, hipMemcpyKind matKind = hipMemcpyDefault) {
+  if (matKind == hipMemcpyDefault) matKind = kind;
   const double t0 = now_seconds();
   int rc = check_ready(c);
   if (rc) return rc;
@@ -1136,7 +1140,7 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
   rc = ensure_vectors(c);
   if (rc) return rc;
   CK(c->w64.ensure(n));
-  rc = stage_matrix(c, diag, upper, useBiCG ? lower : nullptr, kind);
+  rc = stage_matrix(c, diag, upper, useBiCG ? lower : nullptr, matKind);
   if (rc) return rc;
   XF(c->psi0.p, psi, sizeof(double) * n, kind);
   XF(c->b.p, source, sizeof(double) * n, kind);
@@ -1282,10 +1286,13 @@ int set_matrix_impl(hipfoam_context* c, const double* diag, const double* upper,
 // A^T lambda = g runs through the same refined path with upper and lower
 // swapped on the same addressing (lduMatrix::Tmul semantics), then the face,
 // diagonal and source gradients are formed from x and lambda.
+// x, g, lambda and the gradients on the device; diag, upper and lower where
+// matKind says (hipMemcpyHostToDevice: staged from host memory by the solve)
 int adjoint_device(hipfoam_context* c, int solver, int precision, const double* diag,
                    const double* upper, const double* lower, const double* x, const double* g,
                    double* lambda, double* dUpper, double* dLower, double* dDiag,
-                   double* dSource, const hipfoam_controls* ctl, hipfoam_performance* perf) {
+                   double* dSource, const hipfoam_controls* ctl, hipfoam_performance* perf,
+                   hipMemcpyKind matKind = hipMemcpyDeviceToDevice) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (!x || !g || !lambda || !ctl) return fail(c, HIPFOAM_ERR_ARG, "NULL x/g/lambda/controls");
@@ -1298,9 +1305,9 @@ int adjoint_device(hipfoam_context* c, int solver, int precision, const double* 
     return fail(c, HIPFOAM_ERR_ARG, "precision must be HIPFOAM_FP32 or HIPFOAM_FP64");
   rc = precision == HIPFOAM_FP64
            ? fp64_solve(c, solver, diag, upT, loT, lambda, g, ctl, perf, nullptr,
-                        hipMemcpyDeviceToDevice)
+                        hipMemcpyDeviceToDevice, matKind)
            : refined_solve(c, solver, diag, upT, loT, lambda, g, ctl, perf, nullptr,
-                           hipMemcpyDeviceToDevice);
+                           hipMemcpyDeviceToDevice, matKind);
   if (rc) return rc;
   if (nf > 0 && (dUpper || dLower)) {
     hipLaunchKernelGGL(k_adjoint_faces, dim3(blocks_for(nf, 256)), dim3(256), 0, c->stream, nf,
@@ -1333,25 +1340,15 @@ int adjoint_host(hipfoam_context* c, int solver, int precision, const double* di
   CK(c->adjDB.ensure(n));
   CK(c->adjDU.ensure(nf));
   CK(c->adjDL.ensure(nf));
-  // matrix staged into the library's own buffers, then used as device input
   rc = ensure_vectors(c);
   if (rc) return rc;
-  CK(c->diag64.ensure(n));
   XF(c->adjX.p, x, sizeof(double) * n, h2d);
   XF(c->adjG.p, g, sizeof(double) * n, h2d);
-  // device copies of the coefficients (stage_matrix inside the solve copies again)
-  Buf<double> d, u, l;
-  CK(d.ensure(n));
-  CK(u.ensure(nf));
-  if (lower) CK(l.ensure(nf));
-  XF(d.p, diag, sizeof(double) * n, h2d);
-  XF(u.p, upper, sizeof(double) * nf, h2d);
-  if (lower) XF(l.p, lower, sizeof(double) * nf, h2d);
-  rc = adjoint_device(c, solver, precision, d.p, u.p, lower ? l.p : nullptr, c->adjX.p,
-                      c->adjG.p,
+  // the coefficients go from host memory straight into the staged matrix
+  rc = adjoint_device(c, solver, precision, diag, upper, lower, c->adjX.p, c->adjG.p,
                       c->adjLam.p, dUpper ? c->adjDU.p : nullptr,
                       (dLower && lower) ? c->adjDL.p : nullptr, dDiag ? c->adjDD.p : nullptr,
-                      dSource ? c->adjDB.p : nullptr, ctl, perf);
+                      dSource ? c->adjDB.p : nullptr, ctl, perf, h2d);
   if (rc == HIPFOAM_OK) {
     XF(lambda, c->adjLam.p, sizeof(double) * n, d2h);
     if (dUpper) XF(dUpper, c->adjDU.p, sizeof(double) * nf, d2h);
@@ -1361,9 +1358,6 @@ int adjoint_host(hipfoam_context* c, int solver, int precision, const double* di
     if (dSource) XF(dSource, c->adjDB.p, sizeof(double) * n, d2h);
   }
   CK(hipStreamSynchronize(c->stream));
-  d.release();
-  u.release();
-  l.release();
   return rc;
 }
 
