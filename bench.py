@@ -247,8 +247,8 @@ def measure_kernels(H, ctx, m, solver, spmv_reps, tag, solve_fn):
     PCG: the interior Chebyshev step k_cheb_step<float, false> (class 0);
     BiCGStab: the two SpMVs fused with their vector updates (class 2).
     Bytes are the kernel's own algorithmic bytes on the layout it runs (the
-    row-slot layout, or the LDU face arrays of SURVEY-style pricing when the
-    schedule is 0 or the mesh falls back to it)."""
+    row-slot layout, or the LDU face arrays, priced on the reference's own
+    format, when the schedule is 0 or the mesh falls back to it)."""
     sym = m.lower is None
     D = H.DeviceArray
     ctx.set_kernel_timing(True, 0 if solver == "pcg" else 2)
@@ -378,8 +378,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU: the LDU solve is not "
-                          "sharded across GPUs (see DESIGN.md for how it would shard); "
-                          "run with --gpus 1"}))
+                          "sharded across GPUs; run with --gpus 1"}))
         sys.exit(2)
 
     from metal4foam_amd import hipfoam as H
