@@ -143,6 +143,16 @@ def test_uncoupled_patches_are_accepted_before_the_device(driver, tmp_path):
     assert "HIP backend unavailable" in fatal[0] and "no HIP device" in fatal[0]
 
 
+def test_unknown_topology_check_is_fatal(driver, tmp_path):
+    """hip { topologyCheck ...; } accepts content (the default) and pointers
+    (the reference's key) only."""
+    m = small_poisson()
+    r, res, fatal, _ = run_driver(driver, tmp_path, m, np.zeros(m.n), "hipPCG",
+                                  hip={"topologyCheck": "sometimes"})
+    assert r.returncode == 3 and not res
+    assert "hip.topologyCheck must be content or pointers, not sometimes" in fatal[0]
+
+
 # ---------------------------------------------------------------------------
 # GPU: the plugin solve
 # ---------------------------------------------------------------------------
@@ -254,3 +264,25 @@ def test_plugin_deflate_pinned_keyword(driver, tmp_path):
     assert "restarts=1" in r.stdout or "restarts=0" in r.stdout
     x = np.linalg.solve(mg.dense(m), m.source)
     assert np.abs(arrays[0] - x).max() <= 1e-6 * np.abs(x).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("check", ["content", "pointers"])
+def test_plugin_topology_check_keyword(driver, tmp_path, ctx, check):
+    """hip { topologyCheck content|pointers; }: both keys give the direct
+    C-ABI solves' psi bit for bit on an unchanged mesh, over two solves of
+    one context (the second starting from the first's psi)."""
+    from metal4foam_amd import hipfoam as H
+
+    m = small_poisson()
+    psi0 = np.random.default_rng(7).standard_normal(m.n)
+    r, res, _, arrays = run_driver(driver, tmp_path, m, psi0, "hipPCG", n_solves=2,
+                                   tolerance=1e-7, rel_tol=0.0, maxIter=500,
+                                   hip=dict(topologyCheck=check, maxRestarts=6,
+                                            chebyshevOrder=6))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert len(res) == 2 and all(x["converged"] for x in res)
+    ctl = dict(tolerance=1e-7, relTol=0.0, maxIter=500, maxRestarts=6, chebyshevOrder=6)
+    psi1, _, _ = _direct_solve(ctx, m, H.PCG, psi0, **ctl)
+    psi2, _, _ = _direct_solve(ctx, m, H.PCG, psi1, **ctl)
+    assert np.array_equal(arrays[0], psi2)
