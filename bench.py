@@ -110,6 +110,9 @@ def _relmax(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
+CPU_RUNS = 3  # each CPU time is the median of this many runs
+
+
 def cpu_baseline(m, solver, sample_iters):
     """CPU solves on this host, in FP64, built here with -O3 -march=native:
 
@@ -120,7 +123,8 @@ def cpu_baseline(m, solver, sample_iters):
     * the CPU-vs-CPU floor: how far the diagonal-preconditioned CPU solve
       lands from the DIC (DILU) one at the same 1e-6 tolerance.
 
-    sample_iters == 0: every solve runs to 1e-6 and is timed whole, and the
+    Each time is the median of CPU_RUNS runs.  sample_iters == 0: every
+    solve runs to 1e-6 and is timed whole, and the
     DIC (DILU) solution is returned so the GPU results can state their error
     against it.  Otherwise a bounded number of iterations of the baseline is
     timed and scaled to the iteration count of the full solve (committed
@@ -139,17 +143,22 @@ def cpu_baseline(m, solver, sample_iters):
         fpath = os.path.join(ROOT, "tests", "golden", "cpu_baseline_iterations.json")
         full = json.load(open(fpath)) if os.path.exists(fpath) else {}
         max_iter = sample_iters if sample_iters > 0 else 20000
-        t0 = time.perf_counter()
-        if solver == "pcg":
-            x, perf = oracle.pcg(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=max_iter)
-        else:
-            x, perf = oracle.pbicgstab(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=max_iter)
-        dt = time.perf_counter() - t0
+        runs = []
+        for _ in range(CPU_RUNS):
+            t0 = time.perf_counter()
+            if solver == "pcg":
+                x, perf = oracle.pcg(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=max_iter)
+            else:
+                x, perf = oracle.pbicgstab(m, precon=oracle.PRE_DIC, tol=1e-6,
+                                           maxIter=max_iter)
+            runs.append(time.perf_counter() - t0)
+        dt = float(np.median(runs))
         name = "PCG(DIC)" if solver == "pcg" else "PBiCGStab(DILU)"
         done = perf.nIterations
         if perf.converged:
-            est, sample = dt, (f"{name} FP64, 1 core, the full solve to 1e-6 timed "
-                               f"({done} iterations, final residual {perf.finalResidual:.3e})")
+            est, sample = dt, (f"{name} FP64, 1 core, the full solve to 1e-6 timed, median "
+                               f"of {CPU_RUNS} runs ({done} iterations, final residual "
+                               f"{perf.finalResidual:.3e})")
         else:
             iters_full = full.get(m.name, {}).get("iterations") or done
             est = dt / max(done, 1) * iters_full
@@ -170,13 +179,16 @@ def cpu_baseline(m, solver, sample_iters):
         if x is not None:
             if solver == "pcg":
                 th = _omp_threads()
-                t0 = time.perf_counter()
-                xd, pd = oracle.pcg_diag_par(m, tol=1e-6, maxIter=20000, threads=th)
-                dtd = time.perf_counter() - t0
+                druns = []
+                for _ in range(CPU_RUNS):
+                    t0 = time.perf_counter()
+                    xd, pd = oracle.pcg_diag_par(m, tol=1e-6, maxIter=20000, threads=th)
+                    druns.append(time.perf_counter() - t0)
                 out["all_cores"] = {
-                    "value": round(1e3 * dtd, 3), "unit": "ms",
+                    "value": round(1e3 * float(np.median(druns)), 3), "unit": "ms",
                     "cores": th if native is not None else 1,
                     "solver": "PCG(diagonal) FP64, OpenMP row-gather SpMV",
+                    "runs": CPU_RUNS,
                     "iterations": pd.nIterations, "converged": bool(pd.converged)}
                 other = "PCG(diagonal)"
             else:
