@@ -1,0 +1,58 @@
+"""bench.py's one-line JSON contract, on a small workload in a child process:
+the required keys and types, a time-like value with higher_is_better false,
+a roofline whose fraction is achieved / peak and below 1 and whose achieved
+rate is its bytes per launch over its average launch time, a CPU baseline
+timed on one core with its sample stated, and --gpus above 1 refused with a
+message."""
+
+import json
+import pathlib
+import subprocess
+import sys
+
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+pytestmark = pytest.mark.gpu
+
+
+def _bench(*args, timeout=300):
+    print(f"[bench-contract] bench.py {' '.join(args)}", flush=True)
+    r = subprocess.run([sys.executable, "bench.py", *args], cwd=ROOT, capture_output=True,
+                       text=True, timeout=timeout)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    return r, (json.loads(lines[-1]) if lines else None)
+
+
+def test_bench_line_contract():
+    r, out = _bench("--workload", "poisson2d100", "--steps", "2", "--warmup", "1",
+                    "--fp64-steps", "1", "--host-steps", "1", "--spmv-reps", "5",
+                    "--spmv-tune", "0", "--cold-reps", "0", "--extras", "0")
+    assert r.returncode == 0, r.stderr[-3000:]
+    for key in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
+                "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config",
+                "roofline", "cpu_baseline"):
+        assert key in out, key
+    assert out["n_gpus"] == 1 and out["steps"] == 2 and out["warmup"] == 1
+    assert out["unit"] == "ms" and out["higher_is_better"] is False
+    assert out["value"] > 0 and out["value"] == pytest.approx(out["ms_per_step"])
+    assert out["vs_baseline"] is None and out["data"] == "synthetic"
+    assert "workload" in out["config"] and "model" not in out["config"]
+    rf = out["roofline"]
+    assert rf["bound"] == "hbm" and rf["unit"] == "GB/s" and rf["peak"] == 8000.0
+    assert 0 < rf["frac"] < 1
+    # the line rounds frac to 4 decimals and achieved to 0.1 GB/s
+    assert rf["frac"] == pytest.approx(rf["achieved"] / rf["peak"], abs=6e-5)
+    assert rf["achieved"] == pytest.approx(
+        rf["bytes_per_launch"] / (rf["avg_launch_us"] * 1e-6) / 1e9, rel=2e-3)
+    cpu = out["cpu_baseline"]
+    assert cpu["cores"] == 1 and cpu["kind"] in ("port", "reference")
+    assert cpu["value"] > 0 and cpu["unit"] == "ms" and cpu["sample"]
+    assert out["solve"]["converged"] and out["solve"]["numpy_residual"] < 1e-6
+
+
+def test_bench_refuses_several_gpus():
+    r, out = _bench("--gpus", "2", timeout=120)
+    assert r.returncode != 0
+    assert out is not None and "one GPU" in out["error"]
