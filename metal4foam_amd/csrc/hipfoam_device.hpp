@@ -593,6 +593,32 @@ __device__ __forceinline__ void for_rows(const LduOp<T>& A, int r0, int r1, Fn f
   }
 }
 
+// for_rows over the tiles of logical workgroup lb in an XCD march: on a grid
+// whose XCD mapping gives each XCD one contiguous run of workgroups
+// (xcdRemap 1, a multiple of 8 workgroups), the XCD's share of the mesh is
+// one contiguous slab of tiles, and its q workgroups walk the slab together:
+// pass p of local workgroup j is tile slab + p q + j.  The XCD's workgroups
+// then work on one window of q tiles at a time instead of q windows spread
+// over the slab, so the neighbour planes a pass gathers were loaded by the
+// pass before, in the same L2.  The rows and their terms are those of
+// for_rows; only which workgroup sums which rows, and so the grouping of
+// the FP64 partial sums, changes.  With one tile per workgroup (the
+// per-tile launch shape) every workgroup gets the tile it has contiguously.
+template <typename T, typename Fn>
+__device__ __forceinline__ void for_rows_xcd(const LduOp<T>& A, int lb, int r0, int r1, Fn fn) {
+  const int G = (int)gridDim.x;
+  if ((G & 7) != 0 || A.xcdRemap != 1) {
+    for_rows(A, r0, r1, fn);
+    return;
+  }
+  const int bs = (int)blockDim.x, q = G >> 3;
+  const int ntiles = (A.n + bs - 1) / bs;
+  const int xcd = lb / q, j = lb - xcd * q;
+  const int per = (ntiles + 7) >> 3;
+  const int s0 = xcd * per, s1 = min(s0 + per, ntiles);
+  for (int t = s0 + j; t < s1; t += q) for_rows(A, t * bs, min((t + 1) * bs, A.n), fn);
+}
+
 // ---------------------------------------------------------------------------
 // scalar finalisation (runs in thread 0 of the last workgroup)
 // ---------------------------------------------------------------------------
@@ -684,7 +710,7 @@ __global__ __launch_bounds__(1024) void k_cheb_step(LduOp<T> A, const T* __restr
   const T c2 = (T)st->chebC2[step - 1];
   const T c3 = (T)st->chebC3[step - 1];
   double dot = 0.0;
-  for_rows(A, r0, r1, [&](int i, auto row) {
+  auto row_step = [&](int i, auto row) {
     // the row's own operands load before the gathers, in one batch
     const T ri = r[i], rDi = rD[i];
     const T zi = zin[i], wold = w[i];
@@ -694,7 +720,12 @@ __global__ __launch_bounds__(1024) void k_cheb_step(LduOp<T> A, const T* __restr
     if (!LAST) w[i] = wi;
     zout[i] = zi_new;
     if (LAST) dot += (double)(zi_new * ri);
-  });
+  };
+  // the last step runs on the reducing launch shape: an XCD march
+  if (LAST)
+    for_rows_xcd(A, lb, r0, r1, row_step);
+  else
+    for_rows(A, r0, r1, row_step);
   if (LAST) {
     double v[1] = {dot};
     block_partials<1>(v, partials, pstride, lb);
@@ -809,7 +840,7 @@ __global__ __launch_bounds__(1024) void k_pcg_p_spmv(LduOp<T> A, const T* __rest
   block_rows(A.n, lb, r0, r1);
   const T beta = (T)st->beta;
   double dot = 0.0;
-  for_rows(A, r0, r1, [&](int i, auto row) {
+  for_rows_xcd(A, lb, r0, r1, [&](int i, auto row) {
     const T pi = wA[i] + beta * pold[i];
     const T y = row(pi, [&](int j) { return wA[j] + beta * pold[j]; });
     pnew[i] = pi;
