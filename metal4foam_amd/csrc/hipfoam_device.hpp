@@ -937,7 +937,7 @@ __global__ __launch_bounds__(1024) void k_bicg_py_spmv(LduOp<T> A, const T* __re
   block_rows(A.n, lb, r0, r1);
   const T beta = (T)st->beta, omega = (T)st->omega;
   double d = 0.0;
-  for_rows(A, r0, r1, [&](int i, auto row) {
+  for_rows_xcd(A, lb, r0, r1, [&](int i, auto row) {
     const T a0 = rA0[i];
     const T pi = r[i] + beta * (pold[i] - omega * AyAold[i]);
     const T yi = rD[i] * pi;
@@ -974,7 +974,7 @@ __global__ __launch_bounds__(1024) void k_bicg_sz_spmv(LduOp<T> A, const T* __re
   block_rows(A.n, lb, r0, r1);
   const T alpha = (T)st->alpha;
   double d[3] = {0.0, 0.0, 0.0};
-  for_rows(A, r0, r1, [&](int i, auto row) {
+  for_rows_xcd(A, lb, r0, r1, [&](int i, auto row) {
     const T si = r[i] - alpha * AyA[i];
     const T zi = rD[i] * si;
     const T t = row(zi, [&](int j) {
