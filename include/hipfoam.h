@@ -388,6 +388,13 @@ typedef struct
     int32_t rowForm;      /* 0 = unrolled predicated rows when every row has
                              <= 3 owner and <= 3 neighbour faces, else loops;
                              1 = always loops */
+    int32_t persistent;   /* Chebyshev PCG inner loop as one persistent
+                             launch on one XCD, on the per-tile launch shape
+                             (one 256-thread workgroup per 256-row tile):
+                             -1 = auto (default): up to 64 tiles (16,384
+                             rows); 1 = up to 512 tiles; 0 = never.  The
+                             same rows, partials and fold order as the
+                             launched loop: identical results */
 } hipfoam_schedule;
 
 int32_t hipfoam_get_schedule(const hipfoam_context* ctx, hipfoam_schedule* s);
@@ -414,6 +421,14 @@ int32_t hipfoam_slot_layout(const hipfoam_context* ctx, int64_t* patternedRows,
 int32_t hipfoam_slot_tiles(const hipfoam_context* ctx, int64_t* patternedRows,
                            int64_t* explicitRows, int64_t* faceRows);
 int32_t hipfoam_set_schedule(hipfoam_context* ctx, const hipfoam_schedule* s);
+/* Inner loops this context ran as one persistent launch (schedule field
+ * persistent), how many of those were abandoned part-way and run again on
+ * the launched loop, and why the last one was: 0x10000 | workgroup << 4 |
+ * XCD id when that workgroup found a participant on another XCD (the id),
+ * 0x20000 | barrier index for a barrier wait that timed out (0: none
+ * abandoned).  Any pointer may be NULL.  No reference counterpart. */
+int32_t hipfoam_persistent_stats(const hipfoam_context* ctx, int64_t* launches,
+                                 int64_t* abandoned, int64_t* lastAbort);
 /* Times the candidate SpMV schedules on the staged operator and keeps the
  * fastest; *best (may be NULL) receives it. */
 int32_t hipfoam_autotune(hipfoam_context* ctx, hipfoam_schedule* best);

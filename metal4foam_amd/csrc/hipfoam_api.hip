@@ -27,6 +27,7 @@
 #include "../../include/hipfoam.h"
 #include "hipfoam_batched.hpp"
 #include "hipfoam_device.hpp"
+#include "hipfoam_persist.hpp"
 #include "hipfoam_slots.hpp"
 #include "hipfoam_stage.hpp"
 
@@ -186,11 +187,16 @@ struct hipfoam_context {
   Buf<OState> ost;
   KState* hk = nullptr;  // pinned
   OState* ho = nullptr;  // pinned
-  int32_t* hstatus = nullptr;  // pinned, 2 slots
+  int32_t* hstatus = nullptr;  // pinned: 2 poll slots, the persistent loop's abort word
+  // barrier words of the persistent inner loop (XcdSync: flags, XCD ids, abort, epoch)
+  Buf<uint32_t> xsync;
+  int64_t persistLaunches = 0;   // persistent inner loops launched
+  int64_t persistAbandoned = 0;  // ... of which abandoned (the launched loop ran instead)
+  int64_t persistLastAbort = 0;  // abort word of the last abandoned launch (XcdSync)
   hipEvent_t evPoll[2] = {nullptr, nullptr};
   hipEvent_t evT0 = nullptr, evT1 = nullptr;
 
-  hipfoam_schedule sched{2, 0, 2, -1, 0, 0};
+  hipfoam_schedule sched{2, 0, 2, -1, 0, 0, -1};
   std::map<GraphKey, hipGraphExec_t> graphs;
   KernelTimer timer;
   // host helper threads (created on first use): staged copies, addressing hash
@@ -789,6 +795,83 @@ int run_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int 
   return HIPFOAM_OK;
 }
 
+// Meshes of at most this many 256-row tiles run the Chebyshev PCG loop as one
+// persistent launch on one XCD by default (schedule.persistent -1).
+constexpr int kPersistAutoTiles = 64;
+
+// Whether the inner loop runs as one persistent launch (k_pcg_persistent):
+// Chebyshev PCG on the per-tile launch shape (one 256-thread workgroup per
+// tile, second stages merged), whose rows, partials and fold order it
+// reproduces, up to kPersistAutoTiles tiles (persistent -1) or every
+// per-tile mesh (1); never under kernel timing, whose per-launch events need
+// the launched kernels.
+inline bool persistent_ok(const hipfoam_context* c, Kind kind, int order) {
+  if (c->sched.persistent == 0 || !c->xsync.p || c->timer.on) return false;
+  if (kind != KIND_PCG || order < 2) return false;
+  const int ntiles = (c->n + kSlotTile - 1) / kSlotTile;
+  if (row_block(c) != kSlotTile || row_grid(c) != ntiles || !merge_stages(c)) return false;
+  return ntiles <= (c->sched.persistent > 0 ? kAutoSmallTiles : kPersistAutoTiles);
+}
+
+// One persistent launch of the whole Chebyshev PCG loop; vectors and *kst2
+// must be initialised.  *abandoned: a participant timed out at a barrier or
+// ran off XCD 0, so the launch gave up part-way; the barrier words are reset
+// and the caller starts the solve again on the launched loop.
+template <typename T>
+int persistent_loop(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order,
+                    int maxIter, double* gpuSeconds, bool* abandoned) {
+  Work<T>& w = work<T>(c);
+  const int ntiles = (c->n + kSlotTile - 1) / kSlotTile;
+  const int K = std::min(ntiles, kPersistMaxGroups);
+  const PcgVectors<T> V{w.r.p,  w.r1.p,    w.p0.p,        w.p1.p, w.wA.p, w.Ap.p,
+                        w.delta.p, w.deltaBest.p, w.z0.p, w.z1.p, w.w.p};
+  uint32_t* xs = c->xsync.p;
+  const XcdSync S{xs, xs + kPersistMaxGroups, xs + 2 * kPersistMaxGroups,
+                  xs + 2 * kPersistMaxGroups + 1};
+  c->persistLaunches += 1;
+  CK(hipEventRecord(c->evT0, c->stream));
+  hipLaunchKernelGGL((k_pcg_persistent<T>), dim3(8 * K), dim3(kSlotTile), 0, c->stream, A, rD,
+                     V, c->kst.p, c->kst2.p, c->partials.p, c->partials2.p, c->pstride, order,
+                     K, std::max(maxIter, 1) + 4, S);
+  CKL();
+  CK(hipEventRecord(c->evT1, c->stream));
+  CK(hipEventSynchronize(c->evT1));
+  float ms = 0.f;
+  CK(hipEventElapsedTime(&ms, c->evT0, c->evT1));
+  if (gpuSeconds) *gpuSeconds += 1e-3 * ms;
+  CK(hipMemcpy(c->hk, c->kst.p, sizeof(KState), hipMemcpyDeviceToHost));
+  if (c->hk->status != ST_RUNNING) return HIPFOAM_OK;
+  // the state was left running: abandoned, or out of iterations
+  int32_t* ab = &c->hstatus[2];
+  CK(hipMemcpy(ab, S.abortw, sizeof(int32_t), hipMemcpyDeviceToHost));
+  CK(hipMemsetAsync(xs, 0, sizeof(uint32_t) * kXcdSyncWords, c->stream));
+  CK(hipStreamSynchronize(c->stream));
+  if (*ab != 0) {
+    c->persistAbandoned += 1;
+    c->persistLastAbort = (int64_t)(uint32_t)*ab;
+    *abandoned = true;
+    return HIPFOAM_OK;
+  }
+  return fail(c, HIPFOAM_ERR_STATE, "inner loop did not terminate within maxIter iterations");
+}
+
+// The iteration loop of an initialised solve: one persistent launch where
+// persistent_ok allows, else launched iterations (run_loop).  If the
+// persistent launch was abandoned, reinit() restores the vectors and the
+// state the loop starts from, and the launched loop runs.
+template <typename T, typename Reinit>
+int iterate(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
+            int maxIter, double* gpuSeconds, Reinit reinit) {
+  if (persistent_ok(c, kind, order)) {
+    bool abandoned = false;
+    int rc = persistent_loop<T>(c, A, rD, order, maxIter, gpuSeconds, &abandoned);
+    if (rc || !abandoned) return rc;
+    rc = reinit();
+    if (rc) return rc;
+  }
+  return run_loop<T>(c, kind, A, rD, order, maxIter, gpuSeconds);
+}
+
 // Inner correction solve in reference-backend semantics
 // (metalFoamBackend::pcg / ::bicgstab): r64 -> FP32 residual (minus *mean,
 // times proj when given, when mean is given), delta0 = 0, device loop,
@@ -806,11 +889,6 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
   if (order > kMaxCheb + 1) return fail(c, HIPFOAM_ERR_ARG, "chebyshevOrder > 33");
   Work<float>& w = c->w32;
   const int n = c->n;
-  const dim3 g1(blocks_for(n, 256)), b1(256);
-  hipLaunchKernelGGL((k_inner_init<float>), g1, b1, 0, c->stream, n, r64, mean, proj, w.r.p,
-                     kind == KIND_BICG ? w.rA0.p : nullptr, w.delta.p, w.p0.p, w.p1.p,
-                     kind == KIND_BICG ? w.AyA.p : nullptr);
-  CKL();
   KState s{};
   s.mode = MODE_INNER;
   s.target = target;
@@ -821,10 +899,21 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
   s.bestRes = 1.0e300;
   s.status = maxIter > 0 ? ST_RUNNING : ST_MAXITER;
   if (kind == KIND_PCG && order > 1) cheb_constants(s, order, beta, ratio, true);
-  *c->hk = s;
-  // both state buffers: merged stages read the first iteration's state from kst2
-  CK(hipMemcpyAsync(c->kst.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
-  CK(hipMemcpyAsync(c->kst2.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+  // vectors and both state buffers (merged stages read the first iteration's
+  // state from kst2)
+  auto init = [&]() -> int {
+    const dim3 g1(blocks_for(n, 256)), b1(256);
+    hipLaunchKernelGGL((k_inner_init<float>), g1, b1, 0, c->stream, n, r64, mean, proj, w.r.p,
+                       kind == KIND_BICG ? w.rA0.p : nullptr, w.delta.p, w.p0.p, w.p1.p,
+                       kind == KIND_BICG ? w.AyA.p : nullptr);
+    CKL();
+    *c->hk = s;
+    CK(hipMemcpyAsync(c->kst.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+    CK(hipMemcpyAsync(c->kst2.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+    return HIPFOAM_OK;
+  };
+  int rc0 = init();
+  if (rc0) return rc0;
   double gpu = 0;
   if (s.status == ST_RUNNING) {
     const LduOp<float> A = op32(c);
@@ -832,7 +921,8 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
       int rc = enqueue_bicg_start<float>(c, A);
       if (rc) return rc;
     }
-    int rc = run_loop<float>(c, kind, A, c->rD32.p, kind == KIND_PCG ? order : 1, maxIter, &gpu);
+    int rc = iterate<float>(c, kind, A, c->rD32.p, kind == KIND_PCG ? order : 1, maxIter, &gpu,
+                            init);
     if (rc) return rc;
   } else {
     CK(hipStreamSynchronize(c->stream));
@@ -1170,12 +1260,6 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
     }
     pf.chebBeta = beta;
     hipLaunchKernelGGL(k_recip, g1, b1, 0, c->stream, n, c->diag64.p, c->rD64.p);
-    hipLaunchKernelGGL((k_inner_init<double>), g1, b1, 0, c->stream, n, c->r64.p,
-                       (const double*)nullptr, (const double*)nullptr, w.r.p,
-                       useBiCG ? w.rA0.p : nullptr, (double*)nullptr, w.p0.p, w.p1.p,
-                       useBiCG ? w.AyA.p : nullptr);
-    hipLaunchKernelGGL((k_copy<double>), g1, b1, 0, c->stream, n, c->psi0.p, w.delta.p);
-    CKL();
     KState s{};
     s.mode = MODE_OPENFOAM;
     s.normFactor = pf.normFactor;
@@ -1186,9 +1270,21 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
     s.minIter = ctl->minIter;
     s.status = ST_RUNNING;
     if (order > 1) cheb_constants(s, order, beta, ctl->chebyshevRatio, false);
-    *c->hk = s;
-    CK(hipMemcpyAsync(c->kst.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
-    CK(hipMemcpyAsync(c->kst2.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+    // vectors (delta starts from psi) and both state buffers
+    auto init = [&]() -> int {
+      hipLaunchKernelGGL((k_inner_init<double>), g1, b1, 0, c->stream, n, c->r64.p,
+                         (const double*)nullptr, (const double*)nullptr, w.r.p,
+                         useBiCG ? w.rA0.p : nullptr, (double*)nullptr, w.p0.p, w.p1.p,
+                         useBiCG ? w.AyA.p : nullptr);
+      hipLaunchKernelGGL((k_copy<double>), g1, b1, 0, c->stream, n, c->psi0.p, w.delta.p);
+      CKL();
+      *c->hk = s;
+      CK(hipMemcpyAsync(c->kst.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+      CK(hipMemcpyAsync(c->kst2.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+      return HIPFOAM_OK;
+    };
+    rc = init();
+    if (rc) return rc;
     const LduOp<double> A = op64(c);
     double gpu = 0;
     const double tw = now_seconds();
@@ -1196,8 +1292,8 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
       rc = enqueue_bicg_start<double>(c, A);
       if (rc) return rc;
     }
-    rc = run_loop<double>(c, useBiCG ? KIND_BICG : KIND_PCG, A, c->rD64.p, order,
-                          std::max(ctl->maxIter, ctl->minIter), &gpu);
+    rc = iterate<double>(c, useBiCG ? KIND_BICG : KIND_PCG, A, c->rD64.p, order,
+                         std::max(ctl->maxIter, ctl->minIter), &gpu, init);
     if (rc) return rc;
     pf.tInner = now_seconds() - tw;
     pf.tGpu = gpu;
@@ -1615,7 +1711,7 @@ int32_t hipfoam_create(int32_t device, hipfoam_context** out) {
       hipHostMalloc(reinterpret_cast<void**>(&c->ho), sizeof(OState)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&c->hstatus), 4 * sizeof(int32_t)) != hipSuccess ||
       c->kst.ensure(1) != hipSuccess || c->kst2.ensure(1) != hipSuccess ||
-      c->ost.ensure(1) != hipSuccess ||
+      c->ost.ensure(1) != hipSuccess || c->xsync.ensure(kXcdSyncWords) != hipSuccess ||
       hipEventCreate(&c->evPoll[0]) != hipSuccess || hipEventCreate(&c->evPoll[1]) != hipSuccess ||
       hipEventCreate(&c->evT0) != hipSuccess || hipEventCreate(&c->evT1) != hipSuccess) {
     c->desc = "hipfoam: device allocation failed";
@@ -1625,6 +1721,7 @@ int32_t hipfoam_create(int32_t device, hipfoam_context** out) {
   (void)hipMemset(c->kst.p, 0, sizeof(KState));
   (void)hipMemset(c->kst2.p, 0, sizeof(KState));
   (void)hipMemset(c->ost.p, 0, sizeof(OState));
+  (void)hipMemset(c->xsync.p, 0, sizeof(uint32_t) * kXcdSyncWords);
   (void)hipDeviceSynchronize();
   // the device address of this device's scratch array (allocation-free solve)
   if (scratch_address(device, &c->scratchDev) != hipSuccess) c->scratchDev = nullptr;
@@ -2199,6 +2296,8 @@ int32_t hipfoam_set_schedule(hipfoam_context* c, const hipfoam_schedule* s) {
     return fail(c, HIPFOAM_ERR_ARG, "graphChunk must be -1 (auto), 0 (eager) or even in [2, 4096]");
   if (s->gridLimit < 0) return fail(c, HIPFOAM_ERR_ARG, "gridLimit must be >= 0");
   if (s->rowForm < 0 || s->rowForm > 1) return fail(c, HIPFOAM_ERR_ARG, "rowForm must be 0 or 1");
+  if (s->persistent < -1 || s->persistent > 1)
+    return fail(c, HIPFOAM_ERR_ARG, "persistent must be -1 (auto), 0 (off) or 1 (on)");
   if (c->good) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
@@ -2207,6 +2306,15 @@ int32_t hipfoam_set_schedule(hipfoam_context* c, const hipfoam_schedule* s) {
     return fail(c, HIPFOAM_ERR_ARG, "xcdRemap must be 0, 1 or 2");
   c->sched = *s;
   clear_graphs(c);
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_persistent_stats(const hipfoam_context* c, int64_t* launches,
+                                 int64_t* abandoned, int64_t* lastAbort) {
+  if (!c) return HIPFOAM_ERR_ARG;
+  if (launches) *launches = c->persistLaunches;
+  if (abandoned) *abandoned = c->persistAbandoned;
+  if (lastAbort) *lastAbort = c->persistLastAbort;
   return HIPFOAM_OK;
 }
 

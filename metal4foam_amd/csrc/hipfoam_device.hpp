@@ -149,6 +149,19 @@ __device__ __forceinline__ void block_rows(int n, int lb, int& r0, int& r1) {
 // reductions
 // ---------------------------------------------------------------------------
 
+// Element i of a vector the current launch also writes.  COH (the persistent
+// loop, where other workgroups of the same launch wrote it before a barrier):
+// a device-scope load, which bypasses the CU's L1 and reads the XCD's L2.
+// Otherwise a plain load: a kernel boundary separates writer and reader.
+template <bool COH, typename T>
+__device__ __forceinline__ T vld(const T* __restrict__ p, int i) {
+  if constexpr (COH)
+    return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    return p[i];
+}
+
+
 // Sum over the 64 lanes of a wavefront, every lane receiving the total: the
 // butterfly v += v[lane ^ m] for m = 32, 16, 8, 4, 2, 1, on VALU lane moves
 // rather than __shfl_xor, which lowers to ds_bpermute (two LDS operations per
@@ -227,7 +240,7 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* partials
 // kernel of any workgroup size folds exactly as k_finalize does.
 constexpr int kFoldThreads = 256;
 
-template <int NV, bool LASTMAX = false>
+template <int NV, bool LASTMAX = false, bool COH = false>
 __device__ __forceinline__ void fold_partials(const double* partials, int pstride, int nblk,
                                               double (&out)[NV]) {
   __shared__ double sh[NV][kFoldThreads / 64];
@@ -244,7 +257,7 @@ __device__ __forceinline__ void fold_partials(const double* partials, int pstrid
         const int b = b0 + u * kFoldThreads;
 #pragma unroll
         for (int j = 0; j < NV; ++j)
-          t[u][j] = b < nblk ? partials[j * pstride + b]
+          t[u][j] = b < nblk ? vld<COH>(partials, j * pstride + b)
                              : ((LASTMAX && j == NV - 1) ? -1.0e308 : 0.0);
       }
 #pragma unroll
@@ -688,12 +701,114 @@ __device__ __forceinline__ void finalize_beta(KState* st, double t) {
 }
 
 // ---------------------------------------------------------------------------
-// PCG kernels
+// PCG row bodies: one row of each PCG kernel, shared by the launched kernels
+// and the persistent loop (hipfoam_persist.hpp), so both evaluate the same
+// float expressions in the same order.
 // ---------------------------------------------------------------------------
 
 // One Chebyshev step after the first, fused with its SpMV (ldu_spmv +
 // cheb_step of metalFoamKernels.metal:112-194):
 //   v = A z_{s-1};  w_s = c2 w_{s-1} + c3 rD (r - v);  z_s = z_{s-1} + w_s
+// LAST:  z_s is the preconditioned residual wA; *dot += wA . rA.
+template <typename T, bool LAST, bool COH>
+struct ChebStepRow {
+  const T* __restrict__ r;
+  const T* __restrict__ rD;
+  const T* __restrict__ zin;
+  T* __restrict__ w;
+  T* __restrict__ zout;
+  T c2, c3;
+  double* dot;
+  template <typename Row>
+  __device__ __forceinline__ void operator()(int i, Row row) const {
+    // the row's own operands load before the gathers, in one batch
+    const T ri = vld<COH>(r, i), rDi = rD[i];
+    const T zi = vld<COH>(zin, i), wold = vld<COH>(w, i);
+    const T v = row(zi, [&](int j) { return vld<COH>(zin, j); });
+    const T wi = c2 * wold + c3 * rDi * (ri - v);
+    const T zi_new = zi + wi;
+    if (!LAST) w[i] = wi;
+    zout[i] = zi_new;
+    if (LAST) *dot += (double)(zi_new * ri);
+  }
+};
+
+// First Chebyshev step of a PCG iteration (k_cheb_first below).  PEND: the
+// previous iteration's update is applied first: rn = r - alpha Ap (gathered
+// on the fly at the neighbours), delta += alpha p, deltaBest = delta when
+// copy; v[1] += |rn|.  Then z_0 = c1 rD rn and the first step; LAST: v[0] +=
+// wA . rn.
+template <typename T, bool LAST, bool COH, bool PEND>
+struct ChebFirstRow {
+  const T* __restrict__ rin;
+  T* __restrict__ rout;
+  const T* __restrict__ rD;
+  const T* __restrict__ p;
+  const T* __restrict__ Ap;
+  T* __restrict__ delta;
+  T* __restrict__ deltaBest;
+  T* __restrict__ w;
+  T* __restrict__ zout;
+  T c1, c2, c3, alpha;
+  bool copy;
+  double* v;  // [2]: rho, sum|rn|
+  template <typename Row>
+  __device__ __forceinline__ void operator()(int i, Row row) const {
+    if constexpr (PEND) {
+      const T ri = vld<COH>(rin, i) - alpha * vld<COH>(Ap, i);
+      const T di = vld<COH>(delta, i), pi = vld<COH>(p, i), rDi = rD[i];
+      const T z0 = c1 * rDi * ri;
+      const T y = row(z0, [&](int j) {
+        return c1 * rD[j] * (vld<COH>(rin, j) - alpha * vld<COH>(Ap, j));
+      });
+      const T wi = c2 * z0 + c3 * rDi * (ri - y);
+      const T zn = z0 + wi;
+      if (copy) deltaBest[i] = di;
+      delta[i] = di + alpha * pi;
+      rout[i] = ri;
+      if (!LAST) w[i] = wi;
+      zout[i] = zn;
+      v[1] += (double)fabs(ri);
+      if (LAST) v[0] += (double)(zn * ri);
+    } else {
+      const T ri = vld<COH>(rin, i), rDi = rD[i];
+      const T z0 = c1 * rDi * ri;
+      const T y = row(z0, [&](int j) { return c1 * rD[j] * vld<COH>(rin, j); });
+      const T wi = c2 * z0 + c3 * rDi * (ri - y);
+      const T zn = z0 + wi;
+      rout[i] = ri;
+      if (!LAST) w[i] = wi;
+      zout[i] = zn;
+      if (LAST) v[0] += (double)(zn * ri);
+    }
+  }
+};
+
+// p = wA + beta p_old formed at the gathers, Ap = A p, *dot += p . Ap
+// (k_pcg_p_spmv below).
+template <typename T, bool COH>
+struct PcgPRow {
+  const T* __restrict__ wA;
+  const T* __restrict__ pold;
+  T* __restrict__ pnew;
+  T* __restrict__ Ap;
+  T beta;
+  double* dot;
+  template <typename Row>
+  __device__ __forceinline__ void operator()(int i, Row row) const {
+    const T pi = vld<COH>(wA, i) + beta * vld<COH>(pold, i);
+    const T y = row(pi, [&](int j) { return vld<COH>(wA, j) + beta * vld<COH>(pold, j); });
+    pnew[i] = pi;
+    Ap[i] = y;
+    *dot += (double)(pi * y);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// PCG kernels
+// ---------------------------------------------------------------------------
+
+// One Chebyshev step after the first, fused with its SpMV (ChebStepRow).
 // LAST:  z_s is the preconditioned residual wA; fuses dot(wA, rA) and beta.
 // (The first step, z_0 = w_0 = c1 rD r, is k_cheb_first.)
 template <typename T, bool LAST>
@@ -710,17 +825,7 @@ __global__ __launch_bounds__(1024) void k_cheb_step(LduOp<T> A, const T* __restr
   const T c2 = (T)st->chebC2[step - 1];
   const T c3 = (T)st->chebC3[step - 1];
   double dot = 0.0;
-  auto row_step = [&](int i, auto row) {
-    // the row's own operands load before the gathers, in one batch
-    const T ri = r[i], rDi = rD[i];
-    const T zi = zin[i], wold = w[i];
-    const T v = row(zi, [&](int j) { return zin[j]; });
-    const T wi = c2 * wold + c3 * rDi * (ri - v);
-    const T zi_new = zi + wi;
-    if (!LAST) w[i] = wi;
-    zout[i] = zi_new;
-    if (LAST) dot += (double)(zi_new * ri);
-  };
+  const ChebStepRow<T, LAST, false> row_step{r, rD, zin, w, zout, c2, c3, &dot};
   // the last step runs on the reducing launch shape: an XCD march
   if (LAST)
     for_rows_xcd(A, lb, r0, r1, row_step);
@@ -765,33 +870,13 @@ __global__ __launch_bounds__(1024) void k_cheb_first(
   if (st->pend) {
     const T alpha = (T)st->alpha;
     const bool copy = st->copyBest != 0;
-    for_rows(A, r0, r1, [&](int i, auto row) {
-      const T ri = rin[i] - alpha * Ap[i];
-      const T di = delta[i], pi = p[i], rDi = rD[i];
-      const T z0 = c1 * rDi * ri;
-      const T y = row(z0, [&](int j) { return c1 * rD[j] * (rin[j] - alpha * Ap[j]); });
-      const T wi = c2 * z0 + c3 * rDi * (ri - y);
-      const T zn = z0 + wi;
-      if (copy) deltaBest[i] = di;
-      delta[i] = di + alpha * pi;
-      rout[i] = ri;
-      if (!LAST) w[i] = wi;
-      zout[i] = zn;
-      v[1] += (double)fabs(ri);
-      if (LAST) v[0] += (double)(zn * ri);
-    });
+    for_rows(A, r0, r1,
+             ChebFirstRow<T, LAST, false, true>{rin, rout, rD, p, Ap, delta, deltaBest, w, zout,
+                                                c1, c2, c3, alpha, copy, v});
   } else {
-    for_rows(A, r0, r1, [&](int i, auto row) {
-      const T ri = rin[i], rDi = rD[i];
-      const T z0 = c1 * rDi * ri;
-      const T y = row(z0, [&](int j) { return c1 * rD[j] * rin[j]; });
-      const T wi = c2 * z0 + c3 * rDi * (ri - y);
-      const T zn = z0 + wi;
-      rout[i] = ri;
-      if (!LAST) w[i] = wi;
-      zout[i] = zn;
-      if (LAST) v[0] += (double)(zn * ri);
-    });
+    for_rows(A, r0, r1,
+             ChebFirstRow<T, LAST, false, false>{rin, rout, rD, p, Ap, delta, deltaBest, w,
+                                                 zout, c1, c2, c3, (T)0, false, v});
   }
   block_partials<2>(v, partials, pstride, lb);
 }
@@ -840,13 +925,7 @@ __global__ __launch_bounds__(1024) void k_pcg_p_spmv(LduOp<T> A, const T* __rest
   block_rows(A.n, lb, r0, r1);
   const T beta = (T)st->beta;
   double dot = 0.0;
-  for_rows_xcd(A, lb, r0, r1, [&](int i, auto row) {
-    const T pi = wA[i] + beta * pold[i];
-    const T y = row(pi, [&](int j) { return wA[j] + beta * pold[j]; });
-    pnew[i] = pi;
-    Ap[i] = y;
-    dot += (double)(pi * y);
-  });
+  for_rows_xcd(A, lb, r0, r1, PcgPRow<T, false>{wA, pold, pnew, Ap, beta, &dot});
   double v[1] = {dot};
   block_partials<1>(v, partials, pstride, lb);
 }

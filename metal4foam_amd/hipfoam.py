@@ -331,6 +331,19 @@ class Context:
         self._check(self.lib.hipfoam_autotune(self.h, C.byref(s)), "autotune")
         return s
 
+    def persistent_stats(self) -> tuple:
+        """(inner loops run as one persistent launch, of which abandoned and
+        run again on the launched loop) since the context was created."""
+        return self.persistent_abort()[:2]
+
+    def persistent_abort(self) -> tuple:
+        """persistent_stats() and the abort word of the last abandoned launch
+        (0x10000 | workgroup << 4 | XCD id, 0x20000 | barrier index, 0)."""
+        a, b, w = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.lib.hipfoam_persistent_stats(self.h, C.byref(a), C.byref(b),
+                                                      C.byref(w)), "persistent_stats")
+        return a.value, b.value, w.value
+
     def set_kernel_timing(self, enable, kclass=0):
         self._check(self.lib.hipfoam_set_kernel_timing(self.h, int(enable), kclass),
                     "set_kernel_timing")

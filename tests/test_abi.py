@@ -70,6 +70,10 @@ def test_schedule_validation_without_device():
     assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(bad)) == _lib.ERR_ARG
     bad = _lib.Schedule(0, 256, 3, 8, 0)
     assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(bad)) == _lib.ERR_ARG
+    for p in (-2, 2):
+        bad = _lib.Schedule(0, 256, 1, 8, 0, 0, p)
+        assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(bad)) == _lib.ERR_ARG
+    assert s.persistent == -1
     ok = _lib.Schedule(1, 512, 0, 16, 1024)
     assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(ok)) == _lib.OK
     assert ctx.get_schedule().as_dict() == ok.as_dict()
