@@ -391,8 +391,10 @@ typedef struct
     int32_t persistent;   /* Chebyshev PCG inner loop as one persistent
                              launch on one XCD, on the per-tile launch shape
                              (one 256-thread workgroup per 256-row tile):
-                             -1 = auto (default): up to 64 tiles (16,384
-                             rows); 1 = up to 512 tiles; 0 = never.  The
+                             -1 = auto (default): up to 160 tiles (40,960
+                             rows) for the FP32 inner loop, 96 for the
+                             plain-FP64 loop; 1 = up to 512 tiles; 0 =
+                             never.  The
                              same rows, partials and fold order as the
                              launched loop: identical results */
 } hipfoam_schedule;

@@ -796,21 +796,31 @@ int run_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int 
 }
 
 // Meshes of at most this many 256-row tiles run the Chebyshev PCG loop as one
-// persistent launch on one XCD by default (schedule.persistent -1).
-constexpr int kPersistAutoTiles = 64;
+// persistent launch on one XCD by default (schedule.persistent -1), for the
+// FP32 inner loop and the plain-FP64 loop.  Refined / plain-FP64 solve,
+// launched against persistent, lognormal Poisson, same box: 10k cells (40
+// tiles) 2.01 / 1.12 ms and 2.77 / 2.00 ms; 24^3 (54) 1.26 / 0.71 and
+// 1.50 / 1.10; 28^3 (86) 1.46 / 1.01 and 1.74 / 1.55; 32^3 (128) 1.47 / 1.11
+// and 1.96 / 2.01; 36^3 (183) 1.62 / 1.55 and 2.12 / 3.07; 40^3 (250) 1.73 /
+// 1.99 and 2.39 / 4.45.  One XCD's share of the chip bounds the persistent
+// loop; the launched one pays a fixed ~4.4 us per kernel.
+constexpr int kPersistAutoTiles32 = 160;
+constexpr int kPersistAutoTiles64 = 96;
 
 // Whether the inner loop runs as one persistent launch (k_pcg_persistent):
 // Chebyshev PCG on the per-tile launch shape (one 256-thread workgroup per
 // tile, second stages merged), whose rows, partials and fold order it
-// reproduces, up to kPersistAutoTiles tiles (persistent -1) or every
+// reproduces, up to kPersistAutoTiles32/64 tiles (persistent -1) or every
 // per-tile mesh (1); never under kernel timing, whose per-launch events need
 // the launched kernels.
+template <typename T>
 inline bool persistent_ok(const hipfoam_context* c, Kind kind, int order) {
   if (c->sched.persistent == 0 || !c->xsync.p || c->timer.on) return false;
   if (kind != KIND_PCG || order < 2) return false;
   const int ntiles = (c->n + kSlotTile - 1) / kSlotTile;
   if (row_block(c) != kSlotTile || row_grid(c) != ntiles || !merge_stages(c)) return false;
-  return ntiles <= (c->sched.persistent > 0 ? kAutoSmallTiles : kPersistAutoTiles);
+  const int autoTiles = sizeof(T) == sizeof(float) ? kPersistAutoTiles32 : kPersistAutoTiles64;
+  return ntiles <= (c->sched.persistent > 0 ? kAutoSmallTiles : autoTiles);
 }
 
 // One persistent launch of the whole Chebyshev PCG loop; vectors and *kst2
@@ -862,7 +872,7 @@ int persistent_loop(hipfoam_context* c, const LduOp<T>& A, const T* rD, int orde
 template <typename T, typename Reinit>
 int iterate(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
             int maxIter, double* gpuSeconds, Reinit reinit) {
-  if (persistent_ok(c, kind, order)) {
+  if (persistent_ok<T>(c, kind, order)) {
     bool abandoned = false;
     int rc = persistent_loop<T>(c, A, rD, order, maxIter, gpuSeconds, &abandoned);
     if (rc || !abandoned) return rc;

@@ -36,6 +36,8 @@ namespace hipfoam {
 
 constexpr int kPersistMaxGroups = 64;         // participants: at most 2 per CU of one XCD
 constexpr long kPersistMaxPolls = 1L << 22;   // one barrier wait: about 1-2 s at most
+// wave 0 of each participant polls the participants' flags one per lane
+static_assert(kPersistMaxGroups <= 64, "the barrier polls one participant per lane");
 
 // Barrier words of a persistent launch (device memory, zeroed at context
 // creation and after an abandoned launch).

@@ -86,15 +86,22 @@ def test_persistent_equals_launched(ctx, name, fp64):
 
 
 def test_persistent_auto_limit(ctx):
-    """Auto (-1) runs the persistent loop up to 64 tiles and the launched loop
-    beyond; the explicit block shape of 512 threads is not the per-tile shape."""
-    small = mg.poisson(16, 16, 64, seed=21, lognormal=True)   # 64 tiles
-    large = mg.poisson(16, 16, 65, seed=21, lognormal=True)   # 65 tiles
+    """Auto (-1) runs the persistent loop up to 160 tiles (FP32 inner loop) or
+    96 (plain FP64) and the launched loop beyond; the explicit block shape of
+    512 threads is not the per-tile shape."""
+    small = mg.poisson(16, 16, 160, seed=21, lognormal=True)   # 160 tiles
+    large = mg.poisson(16, 16, 161, seed=21, lognormal=True)   # 161 tiles
+    small64 = mg.poisson(16, 16, 96, seed=21, lognormal=True)  # 96 tiles
+    large64 = mg.poisson(16, 16, 97, seed=21, lognormal=True)  # 97 tiles
     used = _solve(ctx, small, -1)[3]
     assert used[0] >= 1 and used[1] == 0
     assert _solve(ctx, large, -1)[3] == (0, 0)
     used = _solve(ctx, large, 1)[3]
     assert used[0] >= 1 and used[1] == 0
+    used = _solve(ctx, small64, -1, fp64=True)[3]
+    assert used[0] == 1 and used[1] == 0
+    assert _solve(ctx, large64, -1, fp64=True)[3] == (0, 0)
+    assert _solve(ctx, large64, 1, fp64=True)[3] == (1, 0)
     ctx.set_schedule(blockSize=512)
     assert _solve(ctx, small, 1)[3] == (0, 0)
     ctx.set_schedule(blockSize=0)
