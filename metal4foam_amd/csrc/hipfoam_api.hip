@@ -807,42 +807,51 @@ int run_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int 
 constexpr int kPersistAutoTiles32 = 160;
 constexpr int kPersistAutoTiles64 = 96;
 
-// Whether the inner loop runs as one persistent launch (k_pcg_persistent):
-// Chebyshev PCG on the per-tile launch shape (one 256-thread workgroup per
-// tile, second stages merged), whose rows, partials and fold order it
-// reproduces, up to kPersistAutoTiles32/64 tiles (persistent -1) or every
-// per-tile mesh (1); never under kernel timing, whose per-launch events need
-// the launched kernels.
+// Whether the inner loop runs as one persistent launch (k_pcg_persistent,
+// k_bicg_persistent): Chebyshev PCG or BiCGStab on the per-tile launch shape
+// (one 256-thread workgroup per tile, second stages merged), whose rows,
+// partials and fold order it reproduces, up to kPersistAutoTiles32/64 tiles
+// (persistent -1) or every per-tile mesh (1); never under kernel timing,
+// whose per-launch events need the launched kernels.
 template <typename T>
 inline bool persistent_ok(const hipfoam_context* c, Kind kind, int order) {
   if (c->sched.persistent == 0 || !c->xsync.p || c->timer.on) return false;
-  if (kind != KIND_PCG || order < 2) return false;
+  if (kind == KIND_PCG && order < 2) return false;
   const int ntiles = (c->n + kSlotTile - 1) / kSlotTile;
   if (row_block(c) != kSlotTile || row_grid(c) != ntiles || !merge_stages(c)) return false;
   const int autoTiles = sizeof(T) == sizeof(float) ? kPersistAutoTiles32 : kPersistAutoTiles64;
   return ntiles <= (c->sched.persistent > 0 ? kAutoSmallTiles : autoTiles);
 }
 
-// One persistent launch of the whole Chebyshev PCG loop; vectors and *kst2
-// must be initialised.  *abandoned: a participant timed out at a barrier or
-// ran off XCD 0, so the launch gave up part-way; the barrier words are reset
-// and the caller starts the solve again on the launched loop.
+// One persistent launch of the whole iteration loop; vectors and the state
+// must be initialised (and the BiCGStab start enqueued).  *abandoned: the
+// participants were not on one XCD or a barrier wait timed out, so the launch
+// gave up part-way; the barrier words are reset and the caller starts the
+// solve again on the launched loop.
 template <typename T>
-int persistent_loop(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order,
+int persistent_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
                     int maxIter, double* gpuSeconds, bool* abandoned) {
   Work<T>& w = work<T>(c);
   const int ntiles = (c->n + kSlotTile - 1) / kSlotTile;
   const int K = std::min(ntiles, kPersistMaxGroups);
-  const PcgVectors<T> V{w.r.p,  w.r1.p,    w.p0.p,        w.p1.p, w.wA.p, w.Ap.p,
-                        w.delta.p, w.deltaBest.p, w.z0.p, w.z1.p, w.w.p};
+  const int loops = std::max(maxIter, 1) + 4;
   uint32_t* xs = c->xsync.p;
   const XcdSync S{xs, xs + kPersistMaxGroups, xs + 2 * kPersistMaxGroups,
                   xs + 2 * kPersistMaxGroups + 1};
+  const dim3 grid(8 * K), block(kSlotTile);
   c->persistLaunches += 1;
   CK(hipEventRecord(c->evT0, c->stream));
-  hipLaunchKernelGGL((k_pcg_persistent<T>), dim3(8 * K), dim3(kSlotTile), 0, c->stream, A, rD,
-                     V, c->kst.p, c->kst2.p, c->partials.p, c->partials2.p, c->pstride, order,
-                     K, std::max(maxIter, 1) + 4, S);
+  if (kind == KIND_PCG) {
+    const PcgVectors<T> V{w.r.p,     w.r1.p,        w.p0.p, w.p1.p, w.wA.p, w.Ap.p,
+                          w.delta.p, w.deltaBest.p, w.z0.p, w.z1.p, w.w.p};
+    hipLaunchKernelGGL((k_pcg_persistent<T>), grid, block, 0, c->stream, A, rD, V, c->kst.p,
+                       c->kst2.p, c->partials.p, c->partials2.p, c->pstride, order, K, loops, S);
+  } else {
+    const BicgVectors<T> V{w.r.p,  w.p0.p, w.p1.p, w.AyA.p, w.AyA1.p,  w.yA.p,
+                           w.rA0.p, w.sA.p, w.zA.p, w.tA.p,  w.delta.p, w.deltaBest.p};
+    hipLaunchKernelGGL((k_bicg_persistent<T>), grid, block, 0, c->stream, A, rD, V, c->kst.p,
+                       c->kst2.p, c->partials.p, c->partials2.p, c->pstride, K, loops, S);
+  }
   CKL();
   CK(hipEventRecord(c->evT1, c->stream));
   CK(hipEventSynchronize(c->evT1));
@@ -865,18 +874,27 @@ int persistent_loop(hipfoam_context* c, const LduOp<T>& A, const T* rD, int orde
   return fail(c, HIPFOAM_ERR_STATE, "inner loop did not terminate within maxIter iterations");
 }
 
-// The iteration loop of an initialised solve: one persistent launch where
-// persistent_ok allows, else launched iterations (run_loop).  If the
-// persistent launch was abandoned, reinit() restores the vectors and the
-// state the loop starts from, and the launched loop runs.
+// The iteration loop of an initialised solve (BiCGStab: its start, then the
+// loop): one persistent launch where persistent_ok allows, else launched
+// iterations (run_loop).  If the persistent launch was abandoned, reinit()
+// restores the vectors and the state the loop starts from, and the launched
+// loop runs.
 template <typename T, typename Reinit>
 int iterate(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
             int maxIter, double* gpuSeconds, Reinit reinit) {
   if (persistent_ok<T>(c, kind, order)) {
+    if (kind == KIND_BICG) {
+      int rc = enqueue_bicg_start<T>(c, A);
+      if (rc) return rc;
+    }
     bool abandoned = false;
-    int rc = persistent_loop<T>(c, A, rD, order, maxIter, gpuSeconds, &abandoned);
+    int rc = persistent_loop<T>(c, kind, A, rD, order, maxIter, gpuSeconds, &abandoned);
     if (rc || !abandoned) return rc;
     rc = reinit();
+    if (rc) return rc;
+  }
+  if (kind == KIND_BICG) {
+    int rc = enqueue_bicg_start<T>(c, A);
     if (rc) return rc;
   }
   return run_loop<T>(c, kind, A, rD, order, maxIter, gpuSeconds);
@@ -927,10 +945,6 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
   double gpu = 0;
   if (s.status == ST_RUNNING) {
     const LduOp<float> A = op32(c);
-    if (kind == KIND_BICG) {
-      int rc = enqueue_bicg_start<float>(c, A);
-      if (rc) return rc;
-    }
     int rc = iterate<float>(c, kind, A, c->rD32.p, kind == KIND_PCG ? order : 1, maxIter, &gpu,
                             init);
     if (rc) return rc;
@@ -1298,10 +1312,6 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
     const LduOp<double> A = op64(c);
     double gpu = 0;
     const double tw = now_seconds();
-    if (useBiCG) {
-      rc = enqueue_bicg_start<double>(c, A);
-      if (rc) return rc;
-    }
     rc = iterate<double>(c, useBiCG ? KIND_BICG : KIND_PCG, A, c->rD64.p, order,
                          std::max(ctl->maxIter, ctl->minIter), &gpu, init);
     if (rc) return rc;

@@ -969,6 +969,128 @@ __global__ __launch_bounds__(1024) void k_pcg_update(int n, int xcdRemap, T* __r
 // BiCGStab kernels (metalFoamKernels.metal:279-458; metalFoamBackend.C:784-906)
 // ---------------------------------------------------------------------------
 
+// BiCGStab row bodies, shared by the launched kernels and the persistent
+// loop (COH: vectors the same launch writes are read with device-scope loads).
+
+// p = rA + beta (p_old - omega AyA_old) and y = rD p formed at the gathers,
+// AyA = A y; *d += rA0 . AyA (k_bicg_py_spmv).
+template <typename T, bool COH>
+struct BicgPyRow {
+  const T* __restrict__ r;
+  const T* __restrict__ rD;
+  const T* __restrict__ pold;
+  T* __restrict__ pnew;
+  const T* __restrict__ AyAold;
+  T* __restrict__ AyAnew;
+  T* __restrict__ yA;
+  const T* __restrict__ rA0;
+  T beta, omega;
+  double* d;
+  template <typename Row>
+  __device__ __forceinline__ void operator()(int i, Row row) const {
+    const T a0 = rA0[i];
+    const T pi = vld<COH>(r, i) + beta * (vld<COH>(pold, i) - omega * vld<COH>(AyAold, i));
+    const T yi = rD[i] * pi;
+    const T y = row(yi, [&](int j) {
+      const T pj = vld<COH>(r, j) + beta * (vld<COH>(pold, j) - omega * vld<COH>(AyAold, j));
+      return rD[j] * pj;
+    });
+    pnew[i] = pi;
+    yA[i] = yi;
+    AyAnew[i] = y;
+    *d += (double)(a0 * y);
+  }
+};
+
+// sA = rA - alpha AyA and zA = rD sA formed at the gathers, tA = A zA;
+// d += |sA|, tA . sA, tA . tA (k_bicg_sz_spmv).
+template <typename T, bool COH>
+struct BicgSzRow {
+  const T* __restrict__ r;
+  const T* __restrict__ rD;
+  const T* __restrict__ AyA;
+  T* __restrict__ sA;
+  T* __restrict__ zA;
+  T* __restrict__ tA;
+  T alpha;
+  double* d;  // [3]
+  template <typename Row>
+  __device__ __forceinline__ void operator()(int i, Row row) const {
+    const T si = vld<COH>(r, i) - alpha * vld<COH>(AyA, i);
+    const T zi = rD[i] * si;
+    const T t = row(zi, [&](int j) {
+      const T sj = vld<COH>(r, j) - alpha * vld<COH>(AyA, j);
+      return rD[j] * sj;
+    });
+    sA[i] = si;
+    zA[i] = zi;
+    tA[i] = t;
+    d[0] += (double)fabs(si);
+    d[1] += (double)(si * t);
+    d[2] += (double)(t * t);
+  }
+};
+
+// Rows [r0, r1), stride blockDim, of the BiCGStab update (k_bicg_update_rho):
+// deltaBest = delta when copy, delta += alpha yA + omega zA, rA = sA - omega
+// tA; v[0] += |rA|, v[1] += rA0 . rA.  Four rows per thread at a time:
+// every row's loads issue before any row's arithmetic (one memory round trip
+// instead of four); the rows and their sums go in the order of the plain loop.
+template <typename T, bool COH>
+__device__ __forceinline__ void bicg_update_rows(int r0, int r1, T* __restrict__ delta,
+                                                 T* __restrict__ deltaBest,
+                                                 const T* __restrict__ yA,
+                                                 const T* __restrict__ zA, T* __restrict__ r,
+                                                 const T* __restrict__ sA,
+                                                 const T* __restrict__ tA,
+                                                 const T* __restrict__ rA0, T alpha, T omega,
+                                                 bool copy, double (&v)[2]) {
+  constexpr int U = 4;
+  const int bd = blockDim.x;
+  int i0 = r0 + threadIdx.x;
+  for (; i0 + (U - 1) * bd < r1; i0 += U * bd) {
+    T d[U], y[U], z[U], s[U], t[U], a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * bd;
+      d[u] = vld<COH>(delta, i);
+      y[u] = vld<COH>(yA, i);
+      z[u] = vld<COH>(zA, i);
+      s[u] = vld<COH>(sA, i);
+      t[u] = vld<COH>(tA, i);
+      a[u] = rA0[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * bd;
+      if (copy) deltaBest[i] = d[u];
+      delta[i] = d[u] + (alpha * y[u] + omega * z[u]);
+      const T ri = s[u] - omega * t[u];
+      r[i] = ri;
+      v[0] += (double)fabs(ri);
+      v[1] += (double)(a[u] * ri);
+    }
+  }
+  for (int i = i0; i < r1; i += bd) {
+    const T d = vld<COH>(delta, i);
+    if (copy) deltaBest[i] = d;
+    delta[i] = d + (alpha * vld<COH>(yA, i) + omega * vld<COH>(zA, i));
+    const T ri = vld<COH>(sA, i) - omega * vld<COH>(tA, i);
+    r[i] = ri;
+    v[0] += (double)fabs(ri);
+    v[1] += (double)(rA0[i] * ri);
+  }
+}
+
+// After an |sA| exit only delta += alpha yA is applied (bicg_early_update,
+// metalFoamKernels.metal:415-426).
+template <typename T, bool COH>
+__device__ __forceinline__ void bicg_early_rows(int r0, int r1, T* __restrict__ delta,
+                                                const T* __restrict__ yA, T alpha) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x)
+    delta[i] = vld<COH>(delta, i) + alpha * vld<COH>(yA, i);
+}
+
 // rho = rA0.rA, beta (reduce_scalars mode 3).
 template <typename T>
 __global__ __launch_bounds__(1024) void k_bicg_rho(int n, int xcdRemap, const T* __restrict__ rA0,
@@ -1016,19 +1138,8 @@ __global__ __launch_bounds__(1024) void k_bicg_py_spmv(LduOp<T> A, const T* __re
   block_rows(A.n, lb, r0, r1);
   const T beta = (T)st->beta, omega = (T)st->omega;
   double d = 0.0;
-  for_rows_xcd(A, lb, r0, r1, [&](int i, auto row) {
-    const T a0 = rA0[i];
-    const T pi = r[i] + beta * (pold[i] - omega * AyAold[i]);
-    const T yi = rD[i] * pi;
-    const T y = row(yi, [&](int j) {
-      const T pj = r[j] + beta * (pold[j] - omega * AyAold[j]);
-      return rD[j] * pj;
-    });
-    pnew[i] = pi;
-    yA[i] = yi;
-    AyAnew[i] = y;
-    d += (double)(a0 * y);
-  });
+  for_rows_xcd(A, lb, r0, r1,
+               BicgPyRow<T, false>{r, rD, pold, pnew, AyAold, AyAnew, yA, rA0, beta, omega, &d});
   double v[1] = {d};
   block_partials<1>(v, partials, pstride, lb);
 }
@@ -1053,20 +1164,7 @@ __global__ __launch_bounds__(1024) void k_bicg_sz_spmv(LduOp<T> A, const T* __re
   block_rows(A.n, lb, r0, r1);
   const T alpha = (T)st->alpha;
   double d[3] = {0.0, 0.0, 0.0};
-  for_rows_xcd(A, lb, r0, r1, [&](int i, auto row) {
-    const T si = r[i] - alpha * AyA[i];
-    const T zi = rD[i] * si;
-    const T t = row(zi, [&](int j) {
-      const T sj = r[j] - alpha * AyA[j];
-      return rD[j] * sj;
-    });
-    sA[i] = si;
-    zA[i] = zi;
-    tA[i] = t;
-    d[0] += (double)fabs(si);
-    d[1] += (double)(si * t);
-    d[2] += (double)(t * t);
-  });
+  for_rows_xcd(A, lb, r0, r1, BicgSzRow<T, false>{r, rD, AyA, sA, zA, tA, alpha, d});
   block_partials<3>(d, partials, pstride, lb);
 }
 
@@ -1097,50 +1195,12 @@ __global__ __launch_bounds__(1024) void k_bicg_update_rho(int n, int xcdRemap,
   block_rows(n, lb, r0, r1);
   const T alpha = (T)st->alpha;
   if (status == ST_EARLY) {
-    for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) delta[i] += alpha * yA[i];
+    bicg_early_rows<T, false>(r0, r1, delta, yA, alpha);
     return;  // FinBicgUpdateRho completes the |sA| exit
   }
-  const T omega = (T)st->omega;
-  const bool copy = st->copyBest != 0;
   double v[2] = {0.0, 0.0};
-  // four rows per thread at a time: every row's loads issue before any row's
-  // arithmetic (one memory round trip instead of four); the rows and their
-  // sums go in the order of the plain loop
-  constexpr int U = 4;
-  const int bd = blockDim.x;
-  int i0 = r0 + threadIdx.x;
-  for (; i0 + (U - 1) * bd < r1; i0 += U * bd) {
-    T d[U], y[U], z[U], s[U], t[U], a[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = i0 + u * bd;
-      d[u] = delta[i];
-      y[u] = yA[i];
-      z[u] = zA[i];
-      s[u] = sA[i];
-      t[u] = tA[i];
-      a[u] = rA0[i];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = i0 + u * bd;
-      if (copy) deltaBest[i] = d[u];
-      delta[i] = d[u] + (alpha * y[u] + omega * z[u]);
-      const T ri = s[u] - omega * t[u];
-      r[i] = ri;
-      v[0] += (double)fabs(ri);
-      v[1] += (double)(a[u] * ri);
-    }
-  }
-  for (int i = i0; i < r1; i += bd) {
-    const T d = delta[i];
-    if (copy) deltaBest[i] = d;
-    delta[i] = d + (alpha * yA[i] + omega * zA[i]);
-    const T ri = sA[i] - omega * tA[i];
-    r[i] = ri;
-    v[0] += (double)fabs(ri);
-    v[1] += (double)(rA0[i] * ri);
-  }
+  bicg_update_rows<T, false>(r0, r1, delta, deltaBest, yA, zA, r, sA, tA, rA0, alpha,
+                             (T)st->omega, st->copyBest != 0, v);
   block_partials<2>(v, partials, pstride, lb);
 }
 

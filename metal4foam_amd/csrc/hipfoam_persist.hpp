@@ -105,6 +105,56 @@ __device__ __forceinline__ bool xcd_barrier(const XcdSync& S, int K, int w, uint
   return ok != 0;
 }
 
+// The state a persistent kernel keeps in LDS, and the end of the kernel.
+constexpr int kStateWords = (int)(sizeof(KState) / sizeof(uint32_t));
+static_assert(sizeof(KState) % sizeof(uint32_t) == 0 && kStateWords <= 256, "KState word copy");
+constexpr uint32_t kNoStart = 0xffffffffu;
+
+// Prologue of a persistent kernel (participant wg of K): copies *start into
+// the LDS state, then checks that every participant runs on the same XCD --
+// each reports its XCD id tagged with this launch, then one barrier, then
+// every participant reads all reports.  Returns the epoch the launch's
+// arrival counts start from (its first barrier used epoch + 1), or kNoStart
+// when the launch was abandoned.
+__device__ __forceinline__ uint32_t persistent_start(const XcdSync& S, int K, int wg,
+                                                     const KState* start, KState* ks) {
+  if ((int)threadIdx.x < kStateWords)
+    reinterpret_cast<uint32_t*>(ks)[threadIdx.x] =
+        reinterpret_cast<const uint32_t*>(start)[threadIdx.x];
+  const uint32_t tgt0 = __hip_atomic_load(S.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t mine = ((tgt0 + 1) << 4) | xcc_id();
+  if (threadIdx.x == 0) S.xccs[wg] = mine;
+  if (!xcd_barrier(S, K, wg, tgt0 + 1, 0)) return kNoStart;
+  __shared__ int same;
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    const uint32_t v = lane < K ? vld<true>(S.xccs, lane) : mine;
+    const bool ok = __all(v == mine);
+    if (!ok) {
+      const unsigned long long diff = __ballot(v != mine);
+      const uint32_t other = __shfl(v, __ffsll((long long)diff) - 1);
+      if (lane == 0) abandon(S, kAbortOffXcd | ((uint32_t)wg << 4) | (other & 0xfu));
+    }
+    if (lane == 0) same = ok;
+  }
+  __syncthreads();
+  return same ? tgt0 : kNoStart;
+}
+
+// Every participant left the loop at the same point with the same state:
+// participant 0 stores it in *kst and *kst2 and the last arrival count.
+__device__ __forceinline__ void persistent_end(const XcdSync& S, int wg, const KState* ks,
+                                               KState* kst, KState* kst2, uint32_t tgt) {
+  if (wg != 0) return;
+  if ((int)threadIdx.x < kStateWords) {
+    const uint32_t word = reinterpret_cast<const uint32_t*>(ks)[threadIdx.x];
+    reinterpret_cast<uint32_t*>(kst)[threadIdx.x] = word;
+    reinterpret_cast<uint32_t*>(kst2)[threadIdx.x] = word;
+  }
+  if (threadIdx.x == 0)
+    __hip_atomic_store(S.epoch, tgt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The vectors of one inner PCG solve (the context's Work<T>).
 template <typename T>
 struct PcgVectors {
@@ -137,33 +187,10 @@ __global__ __launch_bounds__(256) void k_pcg_persistent(LduOp<T> A, const T* __r
   if (blockIdx.x & 7) return;
   const int wg = (int)(blockIdx.x >> 3);
   if (wg >= K) return;
-  constexpr int W = (int)(sizeof(KState) / sizeof(uint32_t));
-  static_assert(sizeof(KState) % sizeof(uint32_t) == 0 && W <= 256, "KState word copy");
   __shared__ KState ks;
-  if ((int)threadIdx.x < W)
-    reinterpret_cast<uint32_t*>(&ks)[threadIdx.x] = reinterpret_cast<const uint32_t*>(kst2)[threadIdx.x];
-  const uint32_t tgt0 = __hip_atomic_load(S.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t tgt0 = persistent_start(S, K, wg, kst2, &ks);
+  if (tgt0 == kNoStart) return;
   uint32_t tgt = tgt0 + 1;
-  // placement: every participant reports its XCD, tagged with this launch
-  const uint32_t mine = (tgt << 4) | xcc_id();
-  if (threadIdx.x == 0) S.xccs[wg] = mine;
-  if (!xcd_barrier(S, K, wg, tgt, 0)) return;
-  {
-    __shared__ int same;
-    if (threadIdx.x < 64) {
-      const int lane = threadIdx.x;
-      const uint32_t v = lane < K ? vld<true>(S.xccs, lane) : mine;
-      const bool ok = __all(v == mine);
-      if (!ok) {
-        const unsigned long long diff = __ballot(v != mine);
-        const uint32_t other = __shfl(v, __ffsll((long long)diff) - 1);
-        if (lane == 0) abandon(S, kAbortOffXcd | ((uint32_t)wg << 4) | (other & 0xfu));
-      }
-      if (lane == 0) same = ok;
-    }
-    __syncthreads();
-    if (!same) return;
-  }
   const int n = A.n;
   const int ntiles = (n + kSlotTile - 1) / kSlotTile;
   for (int it = 0; it < maxLoops; ++it) {
@@ -259,16 +286,124 @@ __global__ __launch_bounds__(256) void k_pcg_persistent(LduOp<T> A, const T* __r
       if (!xcd_barrier(S, K, wg, tgt, tgt - tgt0 - 1)) return;
     }
   }
-  // every participant left the loop at the same point with the same state
-  if (wg == 0) {
-    if ((int)threadIdx.x < W) {
-      const uint32_t word = reinterpret_cast<const uint32_t*>(&ks)[threadIdx.x];
-      reinterpret_cast<uint32_t*>(kst)[threadIdx.x] = word;
-      reinterpret_cast<uint32_t*>(kst2)[threadIdx.x] = word;
+  persistent_end(S, wg, &ks, kst, kst2, tgt);
+}
+
+
+// The vectors of one inner BiCGStab solve (the context's Work<T>).
+template <typename T>
+struct BicgVectors {
+  T* r;
+  T* p0;
+  T* p1;
+  T* AyA;
+  T* AyA1;
+  T* yA;
+  T* rA0;
+  T* sA;
+  T* zA;
+  T* tA;
+  T* delta;
+  T* deltaBest;
+};
+
+// The BiCGStab loop of the launched kernels after its start (k_bicg_rho and
+// FinBicgRho, launched before this kernel), iteration by iteration:
+// k_bicg_py_spmv, k_bicg_sz_spmv with FinBicgAlpha merged, k_bicg_update_rho
+// with FinBicgSOmega merged, then the end-of-iteration stage FinBicgUpdateRho
+// (a separate finalise when launched, on RUNNING or EARLY states), here at
+// the start of the next iteration; p and AyA ping-pong on the iteration
+// parity.  The state starts from *kst and ends in *kst and *kst2.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bicg_persistent(LduOp<T> A, const T* __restrict__ rD,
+                                                         BicgVectors<T> V, KState* kst,
+                                                         KState* kst2, double* P, double* P2,
+                                                         int pstride, int K, int maxLoops,
+                                                         XcdSync S) {
+  if (blockIdx.x & 7) return;
+  const int wg = (int)(blockIdx.x >> 3);
+  if (wg >= K) return;
+  __shared__ KState ks;
+  const uint32_t tgt0 = persistent_start(S, K, wg, kst, &ks);
+  if (tgt0 == kNoStart) return;
+  uint32_t tgt = tgt0 + 1;
+  const int n = A.n;
+  const int ntiles = (n + kSlotTile - 1) / kSlotTile;
+  for (int it = 0; it < maxLoops; ++it) {
+    const bool odd = (it & 1) != 0;
+    const T* pold = odd ? V.p1 : V.p0;
+    T* pnew = odd ? V.p0 : V.p1;
+    const T* AyAold = odd ? V.AyA1 : V.AyA;
+    T* AyAnew = odd ? V.AyA : V.AyA1;
+    // the previous iteration's end and the next rho, beta
+    if (it > 0) {
+      double t[2];
+      fold_partials<2, false, true>(P, pstride, ntiles, t);
+      if (threadIdx.x == 0 && (ks.status == ST_RUNNING || ks.status == ST_EARLY))
+        FinBicgUpdateRho<T>{&ks}(t);
+      __syncthreads();
     }
-    if (threadIdx.x == 0)
-      __hip_atomic_store(S.epoch, tgt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (ks.status != ST_RUNNING) break;
+    // k_bicg_py_spmv
+    {
+      const T beta = (T)ks.beta, omega = (T)ks.omega;
+      for (int tile = wg; tile < ntiles; tile += K) {
+        const int r0 = tile * kSlotTile, r1 = min(r0 + kSlotTile, n);
+        double d = 0.0;
+        for_rows(A, r0, r1,
+                 BicgPyRow<T, true>{V.r, rD, pold, pnew, AyAold, AyAnew, V.yA, V.rA0, beta,
+                                    omega, &d});
+        double v[1] = {d};
+        block_partials<1>(v, P, pstride, tile);
+        __syncthreads();
+      }
+      ++tgt;
+      if (!xcd_barrier(S, K, wg, tgt, tgt - tgt0 - 1)) return;
+    }
+    // k_bicg_sz_spmv: alpha's second stage, then sA, zA, tA
+    {
+      double t[1];
+      fold_partials<1, false, true>(P, pstride, ntiles, t);
+      if (threadIdx.x == 0 && ks.status == ST_RUNNING) FinBicgAlpha<T>{&ks}(t);
+      __syncthreads();
+      if (ks.status != ST_RUNNING) break;
+      const T alpha = (T)ks.alpha;
+      for (int tile = wg; tile < ntiles; tile += K) {
+        const int r0 = tile * kSlotTile, r1 = min(r0 + kSlotTile, n);
+        double d[3] = {0.0, 0.0, 0.0};
+        for_rows(A, r0, r1, BicgSzRow<T, true>{V.r, rD, AyAnew, V.sA, V.zA, V.tA, alpha, d});
+        block_partials<3>(d, P2, pstride, tile);
+        __syncthreads();
+      }
+      ++tgt;
+      if (!xcd_barrier(S, K, wg, tgt, tgt - tgt0 - 1)) return;
+    }
+    // k_bicg_update_rho: the |sA| test and omega, then delta and rA
+    {
+      double t[3];
+      fold_partials<3, false, true>(P2, pstride, ntiles, t);
+      if (threadIdx.x == 0 && ks.status == ST_RUNNING) FinBicgSOmega<T>{&ks}(t);
+      __syncthreads();
+      const int status = ks.status;
+      if (status != ST_RUNNING && status != ST_EARLY) break;
+      const T alpha = (T)ks.alpha;
+      for (int tile = wg; tile < ntiles; tile += K) {
+        const int r0 = tile * kSlotTile, r1 = min(r0 + kSlotTile, n);
+        if (status == ST_EARLY) {
+          bicg_early_rows<T, true>(r0, r1, V.delta, V.yA, alpha);
+        } else {
+          double v[2] = {0.0, 0.0};
+          bicg_update_rows<T, true>(r0, r1, V.delta, V.deltaBest, V.yA, V.zA, V.r, V.sA, V.tA,
+                                    V.rA0, alpha, (T)ks.omega, ks.copyBest != 0, v);
+          block_partials<2>(v, P, pstride, tile);
+          __syncthreads();
+        }
+      }
+      ++tgt;
+      if (!xcd_barrier(S, K, wg, tgt, tgt - tgt0 - 1)) return;
+    }
   }
+  persistent_end(S, wg, &ks, kst, kst2, tgt);
 }
 
 }  // namespace hipfoam

@@ -85,6 +85,64 @@ def test_persistent_equals_launched(ctx, name, fp64):
         assert perf1.polished == 1 and used1[0] >= 2
 
 
+ASYM = {
+    "convdiff_20": lambda: mg.convection_diffusion(20, 20, 20, seed=41),
+    "convdiff_13x11x9_peclet8": lambda: mg.convection_diffusion(13, 11, 9, seed=42, umax=8.0),
+    "convdiff_100x100x1": lambda: mg.convection_diffusion(100, 100, 1, seed=43),
+    "convdiff_24_random_numbering": lambda: mg.random_renumbering(
+        mg.convection_diffusion(24, 24, 24, seed=44), seed=45),
+}
+
+
+@pytest.mark.parametrize("name", list(ASYM))
+@pytest.mark.parametrize("fp64", [False, True], ids=["refined", "fp64"])
+def test_persistent_bicgstab_equals_launched(ctx, name, fp64):
+    """hipPBiCGStab on asymmetric convection-diffusion: persistent and
+    launched loops bit for bit."""
+    m = ASYM[name]()
+    print(f"[persistent] {name}: {m.n} cells, bicgstab {'fp64' if fp64 else 'refined'}",
+          flush=True)
+    out = []
+    for p in (0, 1):
+        ctx.set_schedule(persistent=p)
+        ctx.set_topology(m)
+        psi = np.zeros(m.n)
+        before = ctx.persistent_stats()
+        perf, res = ctx.solve(H.PBICGSTAB, m.diag, m.upper, m.lower, psi, m.source,
+                              H.default_controls(maxRestarts=6), residual=True, fp64=fp64)
+        after = ctx.persistent_stats()
+        out.append((psi, res, perf, (after[0] - before[0], after[1] - before[1])))
+    (psi0, res0, perf0, used0), (psi1, res1, perf1, used1) = out
+    assert used0 == (0, 0)
+    assert used1[0] >= 1 and used1[1] == 0, (used1, hex(ctx.persistent_abort()[2]))
+    _same_perf(perf0, perf1)
+    np.testing.assert_array_equal(psi1, psi0)
+    np.testing.assert_array_equal(res1, res0)
+    assert perf1.converged, (perf1.finalResidual, perf1.nIterations)
+
+
+@pytest.mark.parametrize("max_iter", [300, 4])
+def test_persistent_bicgstab_inner_entry(ctx, max_iter):
+    """hipfoam_bicgstab (metalFoamBackend::bicgstab): stats and delta bit for
+    bit, through the |sA| exit (a loose target) and a maxIter exit."""
+    m = mg.convection_diffusion(18, 17, 16, seed=46)
+    ctx.set_topology(m)
+    ctx.set_coeffs(m.diag, m.upper, m.lower)
+    r = m.source.copy()
+    for target in (1e-6 * np.abs(r).sum(), 0.5 * np.abs(r).sum()):
+        out = []
+        for p in (0, 1):
+            ctx.set_schedule(persistent=p)
+            before = ctx.persistent_stats()[0]
+            st = ctx.bicgstab(r, target, max_iter)
+            out.append((st, ctx.delta(), ctx.persistent_stats()[0] - before))
+        (s0, d0, u0), (s1, d1, u1) = out
+        assert (u0, u1) == (0, 1)
+        for k in ("nIterations", "finalResSum", "converged", "singular", "stagnated"):
+            assert getattr(s0, k) == getattr(s1, k), k
+        np.testing.assert_array_equal(d1, d0)
+
+
 def test_persistent_auto_limit(ctx):
     """Auto (-1) runs the persistent loop up to 160 tiles (FP32 inner loop) or
     96 (plain FP64) and the launched loop beyond; the explicit block shape of
