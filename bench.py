@@ -7,6 +7,8 @@ and the same-host CPU PCG(DIC) baseline.
 One step = one complete solve from psi = 0 to the FP64 L1 residual 1e-6
 (refined: FP32 Chebyshev-Jacobi PCG inner iterations, constant-mode deflation,
 FP64 refinement on the GPU).  Inputs are resident in HBM before timing starts.
+The line also carries small meshes (10k-cell Poisson and convection-diffusion,
+24^3) with the inner loop as one persistent launch against the launched loop.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload poisson128]
 """
@@ -353,6 +355,61 @@ def extra_mesh(H, mg, args, gen_name, gen_args, solver, tag, what, schedule=None
     return out
 
 
+def small_meshes(H, mg, reps=20):
+    """Refined solves of small meshes, where a launched kernel costs about as
+    much as the work it does: the whole inner loop as one persistent launch
+    on one XCD (the automatic schedule) against the launched loop (persistent
+    0), median of `reps` device-resident solves each."""
+    cases = [
+        ("poisson2d100", mg.config0(), H.PCG,
+         "2D pressure Poisson 100x100x1 (10,000 cells), hipPCG refined"),
+        ("convdiff2d100", mg.convection_diffusion(100, 100, 1, seed=1002), H.PBICGSTAB,
+         "2D upwind convection-diffusion 100x100x1 (10,000 cells), hipPBiCGStab refined"),
+        ("poisson24", mg.poisson(24, 24, 24, seed=1001, lognormal=True), H.PCG,
+         "3D lognormal Poisson 24^3 (13,824 cells), hipPCG refined"),
+    ]
+    out = {}
+    for tag, m, kind, what in cases:
+        ctx = H.Context(0)
+        ctx.set_topology(m)
+        D = H.DeviceArray
+        arrs = [D.from_host(ctx, a) for a in (m.diag, m.upper)]
+        lo = None if m.lower is None else D.from_host(ctx, m.lower)
+        src = D.from_host(ctx, m.source)
+        psi = D.from_host(ctx, np.zeros(m.n))
+        zero = D.from_host(ctx, np.zeros(m.n))
+        ctl = H.default_controls(maxRestarts=8, maxIter=20000)
+        res = {}
+        for name, p in (("persistent", -1), ("launched", 0)):
+            ctx.set_schedule(persistent=p)
+            ts = []
+            for k in range(reps + 2):
+                psi.copy_from(zero)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                perf = ctx.solve_device(kind, arrs[0], arrs[1], lo, psi, src, ctl)
+                ctx.synchronize()
+                ts.append(time.perf_counter() - t0)
+            rsum, _ = numpy_residual(m, psi.download())
+            res[name] = (1e3 * float(np.median(ts[2:])), perf,
+                         bool(perf.converged and rsum / perf.normFactor < 1e-6))
+        launches, abandoned = ctx.persistent_stats()
+        tp, pp, cp = res["persistent"]
+        tl, pl, cl = res["launched"]
+        out[tag] = {"what": what, "cells": m.n, "solve_ms": round(tp, 4),
+                    "launched_loop_ms": round(tl, 4), "speedup": round(tl / tp, 2),
+                    "iterations": pp.nIterations, "restarts": pp.restarts,
+                    "converged": cp and cl, "same_iterations": pp.nIterations == pl.nIterations,
+                    "persistent_launches": launches, "abandoned": abandoned}
+        log(f"small mesh {tag}: persistent {tp:.3f} ms, launched loop {tl:.3f} ms "
+            f"({pp.nIterations} it., {launches} persistent launches, {abandoned} abandoned)")
+        for a in (*arrs, lo, src, psi, zero):
+            if a is not None:
+                a.free()
+        ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -559,6 +616,11 @@ def main():
     x.free()
     y.free()
 
+    # ---- small meshes: the persistent inner loop against the launched one
+    small = None
+    if args.extras and args.workload in ("poisson128", "poisson2d100"):
+        small = small_meshes(H, mg)
+
     # ---- more rooflines: a mesh past the 256 MB MALL, the LDU face schedule
     # priced on the LDU arrays, the BiCGStab kernels of the 96^3 workload
     extras = None
@@ -638,6 +700,7 @@ def main():
         "roofline": roof,
         "spmv": None if spmv is None else dict(spmv, mall_flushed=cold, autotuned=tuned),
         "rooflines_more": extras,
+        "small_meshes": small,
         "solve": {
             "iterations": perf.nIterations,
             "restarts": perf.restarts,

@@ -52,6 +52,21 @@ def test_bench_line_contract():
     assert out["solve"]["converged"] and out["solve"]["numpy_residual"] < 1e-6
 
 
+def test_bench_small_meshes():
+    """The small-mesh section: every case converged on the persistent inner
+    loop (never abandoned) in the same iterations as the launched loop."""
+    r, out = _bench("--workload", "poisson2d100", "--steps", "1", "--warmup", "1",
+                    "--fp64-steps", "0", "--host-steps", "0", "--spmv-reps", "2",
+                    "--spmv-tune", "0", "--cold-reps", "0", "--no-cpu", "--extras", "1")
+    assert r.returncode == 0, r.stderr[-3000:]
+    small = out["small_meshes"]
+    assert set(small) == {"poisson2d100", "convdiff2d100", "poisson24"}
+    for tag, e in small.items():
+        assert e["converged"] and e["same_iterations"], tag
+        assert e["persistent_launches"] >= 1 and e["abandoned"] == 0, tag
+        assert e["solve_ms"] > 0 and e["launched_loop_ms"] > 0, tag
+
+
 def test_bench_refuses_several_gpus():
     r, out = _bench("--gpus", "2", timeout=120)
     assert r.returncode != 0
