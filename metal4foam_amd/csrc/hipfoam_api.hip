@@ -188,6 +188,10 @@ struct hipfoam_context {
   KState* hk = nullptr;  // pinned
   OState* ho = nullptr;  // pinned
   int32_t* hstatus = nullptr;  // pinned: 2 poll slots, the persistent loop's abort word
+  // pinned, coherent host word the stages writing kst post its status to, and
+  // its device address (nullptr: the host copies the status instead)
+  int32_t* mirrorHost = nullptr;
+  int32_t* mirrorDev = nullptr;
   // barrier words of the persistent inner loop (XcdSync: flags, XCD ids, abort, epoch)
   Buf<uint32_t> xsync;
   int64_t persistLaunches = 0;   // persistent inner loops launched
@@ -307,7 +311,8 @@ template <int NV, bool LASTMAX, typename F>
 void launch_fin(hipfoam_context* c, int nblk, bool checkStatus, int runMask, F f) {
   hipLaunchKernelGGL((k_finalize<NV, LASTMAX, F>), dim3(1), dim3(256), 0, c->stream,
                      c->partials.p, c->pstride, nblk,
-                     checkStatus ? &c->kst.p->status : (const int32_t*)nullptr, runMask, f);
+                     checkStatus ? &c->kst.p->status : (const int32_t*)nullptr, runMask,
+                     checkStatus ? c->mirrorDev : (int32_t*)nullptr, f);
 }
 
 // Second stage of a reduction: one workgroup folds the partials of the
@@ -566,11 +571,11 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
       if (order == 2)
         LAUNCH_CLASSED(3, capturing, (k_cheb_first<T, true, FinAlpha<T>, true>), grid, block, A,
                        rin, rout, rD, pold, w.Ap.p, w.delta.p, w.deltaBest.p, w.w.p, zout, st2,
-                       P, ps, Merge{st, P2, nb}, FinAlpha<T>{nullptr});
+                       P, ps, Merge{st, P2, nb, c->mirrorDev}, FinAlpha<T>{nullptr});
       else
         LAUNCH_CLASSED(3, capturing, (k_cheb_first<T, false, FinAlpha<T>, true>), grid, block,
                        A, rin, rout, rD, pold, w.Ap.p, w.delta.p, w.deltaBest.p, w.w.p, zout,
-                       st2, P, ps, Merge{st, P2, nb}, FinAlpha<T>{nullptr});
+                       st2, P, ps, Merge{st, P2, nb, c->mirrorDev}, FinAlpha<T>{nullptr});
     } else {
       if (order == 2)
         LAUNCH_CLASSED(3, capturing, (k_cheb_first<T, true, FinAlpha<T>, false>), grid, block,
@@ -620,7 +625,7 @@ void enqueue_pcg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int or
                    pold, pnew, w.Ap.p, st, P2, ps, Merge{st2, P, nb}, FinBeta<T>{nullptr});
     hipLaunchKernelGGL((k_pcg_update<T, FinAlpha<T>, true>), grid, block, 0, c->stream, c->n,
                        A.xcdRemap, w.delta.p, w.deltaBest.p, w.r.p, pnew, w.Ap.p, st2, P, ps,
-                       Merge{st, P2, nb}, FinAlpha<T>{nullptr});
+                       Merge{st, P2, nb, c->mirrorDev}, FinAlpha<T>{nullptr});
   } else {
     LAUNCH_CLASSED(1, capturing, (k_pcg_p_spmv<T, FinBeta<T>, false>), grid, block, A, w.wA.p,
                    pold, pnew, w.Ap.p, st, P, ps, none, FinBeta<T>{nullptr});
@@ -662,7 +667,7 @@ void enqueue_bicg_iter(hipfoam_context* c, const LduOp<T>& A, const T* rD, int p
                    FinBicgAlpha<T>{nullptr});
     hipLaunchKernelGGL((k_bicg_update_rho<T, FinBicgSOmega<T>, true>), grid, block, 0,
                        c->stream, n, rm, w.delta.p, w.deltaBest.p, w.yA.p, w.zA.p, w.r.p,
-                       w.sA.p, w.tA.p, w.rA0.p, st2, P, ps, Merge{st, P2, nb},
+                       w.sA.p, w.tA.p, w.rA0.p, st2, P, ps, Merge{st, P2, nb, c->mirrorDev},
                        FinBicgSOmega<T>{nullptr});
   } else {
     launch_fin<1>(c, nb, true, kRunning, FinBicgAlpha<T>{st});
@@ -730,10 +735,13 @@ int get_graph(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int
 
 // Runs the device-resident iteration loop until the device state leaves
 // RUNNING.  Chunks of `chunk` iterations are launched back to back; the host
-// polls the status word of the chunk before the one just launched, so one
-// chunk stays queued behind the GPU.  In kernel-timing mode every iteration is
-// launched eagerly and polled synchronously, so no timed launch is a no-op.
-// Vectors and *kst must be initialised.
+// polls the status of the chunk before the one just launched, so one chunk
+// stays queued behind the GPU.  The status is the one the chunk's kernels
+// posted to the context's pinned mirror word (every iteration has a stage
+// that posts it), read once the chunk's event has completed; without a
+// mirror it is copied after each chunk.  In kernel-timing mode every
+// iteration is launched eagerly and polled synchronously, so no timed launch
+// is a no-op.  Vectors and *kst must be initialised.
 template <typename T>
 int run_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
              int maxIter, double* gpuSeconds) {
@@ -761,21 +769,26 @@ int run_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int 
     } else {
       CK(hipGraphLaunch(ex, c->stream));
     }
-    CK(hipMemcpyAsync(&c->hstatus[slot], &c->kst.p->status, sizeof(int32_t),
-                      hipMemcpyDeviceToHost, c->stream));
+    if (!c->mirrorHost)
+      CK(hipMemcpyAsync(&c->hstatus[slot], &c->kst.p->status, sizeof(int32_t),
+                        hipMemcpyDeviceToHost, c->stream));
     CK(hipEventRecord(c->evPoll[slot], c->stream));
+    // the status as of the chunk whose event completed last (or later)
+    auto polled = [&](int s) {
+      return c->mirrorHost ? __atomic_load_n(c->mirrorHost, __ATOMIC_ACQUIRE) : c->hstatus[s];
+    };
     if (timing) {
       CK(hipEventSynchronize(c->evPoll[slot]));
       if (c->timer.used + 64 > c->timer.ev.size()) {
         int rc = timer_flush(c);
         if (rc) return rc;
       }
-      if (c->hstatus[slot] != ST_RUNNING) break;
+      if (polled(slot) != ST_RUNNING) break;
       continue;
     }
     if (havePrev) {
       CK(hipEventSynchronize(c->evPoll[slot ^ 1]));
-      if (c->hstatus[slot ^ 1] != ST_RUNNING) break;
+      if (polled(slot ^ 1) != ST_RUNNING) break;
     }
     havePrev = true;
     slot ^= 1;
@@ -1732,7 +1745,8 @@ int32_t hipfoam_create(int32_t device, hipfoam_context** out) {
       hipHostMalloc(reinterpret_cast<void**>(&c->hstatus), 4 * sizeof(int32_t)) != hipSuccess ||
       c->kst.ensure(1) != hipSuccess || c->kst2.ensure(1) != hipSuccess ||
       c->ost.ensure(1) != hipSuccess || c->xsync.ensure(kXcdSyncWords) != hipSuccess ||
-      hipEventCreate(&c->evPoll[0]) != hipSuccess || hipEventCreate(&c->evPoll[1]) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evPoll[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evPoll[1], hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&c->evT0) != hipSuccess || hipEventCreate(&c->evT1) != hipSuccess) {
     c->desc = "hipfoam: device allocation failed";
     c->err = c->desc;
@@ -1742,6 +1756,18 @@ int32_t hipfoam_create(int32_t device, hipfoam_context** out) {
   (void)hipMemset(c->kst2.p, 0, sizeof(KState));
   (void)hipMemset(c->ost.p, 0, sizeof(OState));
   (void)hipMemset(c->xsync.p, 0, sizeof(uint32_t) * kXcdSyncWords);
+  // the status mirror: coherent pinned memory the kernels store to directly
+  // (without it the host copies the status after each chunk)
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->mirrorHost), 64, hipHostMallocCoherent) ==
+          hipSuccess &&
+      hipHostGetDevicePointer(reinterpret_cast<void**>(&c->mirrorDev), c->mirrorHost, 0) ==
+          hipSuccess) {
+    *c->mirrorHost = ST_RUNNING;
+  } else {
+    if (c->mirrorHost) (void)hipHostFree(c->mirrorHost);
+    c->mirrorHost = nullptr;
+    c->mirrorDev = nullptr;
+  }
   (void)hipDeviceSynchronize();
   // the device address of this device's scratch array (allocation-free solve)
   if (scratch_address(device, &c->scratchDev) != hipSuccess) c->scratchDev = nullptr;
@@ -1781,6 +1807,7 @@ void hipfoam_destroy(hipfoam_context* c) {
   if (c->hk) (void)hipHostFree(c->hk);
   if (c->ho) (void)hipHostFree(c->ho);
   if (c->hstatus) (void)hipHostFree(c->hstatus);
+  if (c->mirrorHost) (void)hipHostFree(c->mirrorHost);
   for (hipEvent_t e : {c->evPoll[0], c->evPoll[1], c->evT0, c->evT1})
     if (e) (void)hipEventDestroy(e);
   c->stager.reset();  // waits for its DMAs, frees the pinned chunks, joins its threads

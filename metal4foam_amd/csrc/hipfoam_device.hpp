@@ -284,26 +284,44 @@ __device__ __forceinline__ void fold_partials(const double* partials, int pstrid
   }
 }
 
+// Host-visible copy of the Krylov status: pinned host memory the host polls
+// between launches, written with a system-scope store by the stages that
+// write the primary state.
+__device__ __forceinline__ void post_status(int32_t* mirror, int32_t status) {
+  if (mirror) __hip_atomic_store(mirror, status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Second stage of every reduction: one workgroup folds the partials of the
 // previous kernel and thread 0 applies the finaliser f (the device-side
 // scalar update, reduce_scalars of metalFoamKernels.metal:298-367).  It runs
-// only when the Krylov status is in runMask (status == NULL: always).
+// only when the Krylov status is in runMask (status == NULL: always); either
+// way thread 0 posts the status to mirror (NULL: none), so that every chunk
+// of launches posts the current status.
 template <int NV, bool LASTMAX, typename F>
 __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ partials, int pstride,
                                                   int nblk, const int32_t* status, int runMask,
-                                                  F f) {
-  if (status && !((runMask >> *status) & 1)) return;
+                                                  int32_t* mirror, F f) {
+  if (status && !((runMask >> *status) & 1)) {
+    if (threadIdx.x == 0) post_status(mirror, *status);
+    return;
+  }
   double t[NV];
   fold_partials<NV, LASTMAX>(partials, pstride, nblk, t);
-  if (threadIdx.x == 0) f(t);
+  if (threadIdx.x == 0) {
+    f(t);
+    if (status) post_status(mirror, *status);
+  }
 }
 
 // A reduction's second stage merged into the next kernel: the state it
-// writes and the previous kernel's partials (nblk workgroups).
+// writes and the previous kernel's partials (nblk workgroups); mirror: where
+// to post the resulting status (NULL: none; set when stOut is the primary
+// state the host reads).
 struct Merge {
   KState* stOut;
   const double* pin;
   int nblk;
+  int32_t* mirror;
 };
 
 // Every workgroup reads the Krylov state stIn and the partials M.pin, folds
@@ -332,6 +350,7 @@ __device__ __forceinline__ const KState* merged_finalize(const KState* stIn, con
   }
   __syncthreads();
   if (blockIdx.x == 0 && k < W) reinterpret_cast<uint32_t*>(M.stOut)[k] = lds[k];
+  if (blockIdx.x == 0 && k == 0) post_status(M.mirror, ks->status);
   return ks;
 }
 
