@@ -451,22 +451,6 @@ LduOp<double> op64(hipfoam_context* c) {
                          c->sym64 ? c->upper64.p : c->lower64.p, c->slotCoef64.p);
 }
 
-// Host copy of the Chebyshev constants (encCheb, metalFoamBackend.C:349-372).
-void cheb_constants(KState& s, int order, double beta, double ratio, bool round32) {
-  const double alpha = beta / ratio;
-  const double theta = 0.5 * (beta + alpha);
-  const double delta = 0.5 * (beta - alpha);
-  const double sigma = theta / delta;
-  double rho = 1.0 / sigma;
-  auto rd = [&](double x) { return round32 ? (double)(float)x : x; };
-  s.chebC1 = rd(1.0 / theta);
-  for (int i = 1; i < order && i <= kMaxCheb; ++i) {
-    const double rhoNew = 1.0 / (2.0 * sigma - rho);
-    s.chebC2[i - 1] = rd(rhoNew * rho);
-    s.chebC3[i - 1] = rd(2.0 * rhoNew / delta);
-    rho = rhoNew;
-  }
-}
 
 template <typename T>
 Work<T>& work(hipfoam_context* c);

@@ -695,6 +695,25 @@ __device__ __forceinline__ void inner_iteration_end(KState* s, double checkB) {
   s->batchLeft = s->batchSize;
 }
 
+// Chebyshev constants (encCheb, metalFoamBackend.C:349-372), in FP64, rounded
+// to FP32 when round32.
+__host__ __device__ inline void cheb_constants(KState& s, int order, double beta, double ratio,
+                                               bool round32) {
+  const double alpha = beta / ratio;
+  const double theta = 0.5 * (beta + alpha);
+  const double delta = 0.5 * (beta - alpha);
+  const double sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  auto rd = [&](double x) { return round32 ? (double)(float)x : x; };
+  s.chebC1 = rd(1.0 / theta);
+  for (int i = 1; i < order && i <= kMaxCheb; ++i) {
+    const double rhoNew = 1.0 / (2.0 * sigma - rho);
+    s.chebC2[i - 1] = rd(rhoNew * rho);
+    s.chebC3[i - 1] = rd(2.0 * rhoNew / delta);
+    rho = rhoNew;
+  }
+}
+
 // OpenFOAM loop condition: while ((++nIter < maxIter && !converged) || nIter < minIter)
 __device__ __forceinline__ void openfoam_iteration_end(KState* s) {
   s->it += 1;
@@ -1227,6 +1246,23 @@ __global__ __launch_bounds__(1024) void k_bicg_update_rho(int n, int xcdRemap,
 // inner-solve initialisation
 // ---------------------------------------------------------------------------
 
+// Row i of the inner-solve initialisation (k_inner_init); useMean: m applies.
+template <typename T>
+__device__ __forceinline__ void inner_init_row(int i, const double* __restrict__ r64,
+                                               bool useMean, double m,
+                                               const double* __restrict__ proj,
+                                               T* __restrict__ rA, T* __restrict__ rA0,
+                                               T* __restrict__ delta, T* __restrict__ p0,
+                                               T* __restrict__ p1, T* __restrict__ AyA) {
+  const T v = (T)(useMean ? (proj ? r64[i] - m * proj[i] : r64[i] - m) : r64[i]);
+  rA[i] = v;
+  if (rA0) rA0[i] = v;
+  if (delta) delta[i] = (T)0;
+  if (p0) p0[i] = (T)0;
+  if (p1) p1[i] = (T)0;
+  if (AyA) AyA[i] = (T)0;
+}
+
 // rA = T(r64 - m v) with m = *mean and v = proj (or 1 without proj): the
 // zero-mean projection of the deflated inner right-hand side, or with
 // proj = u = A 1 the projection along the pinned rows; delta = p0 = p1 =
@@ -1240,19 +1276,131 @@ __global__ __launch_bounds__(1024) void k_inner_init(int n, const double* __rest
                                                      T* __restrict__ p1, T* __restrict__ AyA) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double m = mean ? *mean : 0.0;
-  const T v = (T)(mean ? (proj ? r64[i] - m * proj[i] : r64[i] - m) : r64[i]);
-  rA[i] = v;
-  if (rA0) rA0[i] = v;
-  if (delta) delta[i] = (T)0;
-  if (p0) p0[i] = (T)0;
-  if (p1) p1[i] = (T)0;
-  if (AyA) AyA[i] = (T)0;
+  inner_init_row<T>(i, r64, mean != nullptr, mean ? *mean : 0.0, proj, rA, rA0, delta, p0, p1,
+                    AyA);
 }
 
 // ---------------------------------------------------------------------------
 // FP64 refinement kernels (metalFoamSolver.C:101-365 on the device)
 // ---------------------------------------------------------------------------
+
+// Rows of the FP64 refinement kernels.
+
+// u = A 1; v += psi, [|u| > 1e-8 |diag|], u.u, u (k_rowsum)
+struct RowsumRow {
+  const double* __restrict__ psi;
+  const double* __restrict__ diag;
+  double* __restrict__ u;
+  double* v;  // [4]
+  template <typename Row>
+  __device__ __forceinline__ void operator()(int i, Row row) const {
+    const double pi = psi[i], di = diag[i];
+    const double ui = row(1.0, [](int) { return 1.0; });
+    u[i] = ui;
+    v[0] += pi;
+    v[1] += (fabs(ui) > 1e-8 * fabs(di)) ? 1.0 : 0.0;
+    v[2] += ui * ui;
+    v[3] += ui;
+  }
+};
+
+// wA = A psi, r = b - wA; v += |r|, |wA - xbar u| + |b - xbar u|, r
+// (k_initial_residual)
+struct InitialResidualRow {
+  const double* __restrict__ psi;
+  const double* __restrict__ b;
+  const double* __restrict__ u;
+  double* __restrict__ r;
+  double xbar;
+  double* v;  // [3]
+  template <typename Row>
+  __device__ __forceinline__ void operator()(int i, Row row) const {
+    const double bi = b[i], t = u[i] * xbar;
+    const double wA = row(psi[i], [&](int j) { return psi[j]; });
+    const double ri = bi - wA;
+    r[i] = ri;
+    v[0] += fabs(ri);
+    v[1] += fabs(wA - t) + fabs(bi - t);
+    v[2] += ri;
+  }
+};
+
+// psiNew = psiOld + (double(delta) - dm), formed at the gathers; r = b - A
+// psiNew; v += u.r, |r|, r (k_refine_residual)
+template <typename T>
+struct RefineRow {
+  const double* __restrict__ psiOld;
+  const T* __restrict__ delta;
+  double* __restrict__ psiNew;
+  const double* __restrict__ b;
+  const double* __restrict__ u;
+  double* __restrict__ r;
+  double dm;
+  double* v;  // [3]
+  template <typename Row>
+  __device__ __forceinline__ void operator()(int i, Row row) const {
+    auto pf = [&](int j) { return psiOld[j] + ((double)delta[j] - dm); };
+    const double bi = b[i], ui = u[i];
+    const double pi = pf(i);
+    const double wA = row(pi, pf);
+    psiNew[i] = pi;
+    const double ri = bi - wA;
+    r[i] = ri;
+    v[0] += ui * ri;
+    v[1] += fabs(ri);
+    v[2] += ri;
+  }
+};
+
+// Rows [r0, r1), stride blockDim, of k_prepare_inner: diag* = deflate ? diag
+// - u : diag, diagT = T(diag*), rDT = T(1 / diag*); v[0] += sign flips,
+// v[1] = max(v[1], 1 + sum |upper| / |diag*|) when wantBeta.
+template <typename T>
+__device__ __forceinline__ void prepare_rows(const LduOp<double>& A, int r0, int r1,
+                                             const double* __restrict__ upperF,
+                                             const int32_t* __restrict__ losort,
+                                             const double* __restrict__ u, int deflate,
+                                             int wantBeta, T* __restrict__ diagT,
+                                             T* __restrict__ rDT, double (&v)[2]) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+    const double d = A.diag[i];
+    double ds = d;
+    if (deflate) {
+      ds = d - u[i];
+      if (ds * d <= kVSmall * fabs(ds)) v[0] += 1.0;
+    }
+    diagT[i] = (T)ds;
+    rDT[i] = (T)(1.0 / ds);
+    if (wantBeta) {
+      double so = 0.0;
+      for (int f = A.ownStart[i]; f < A.ownStart[i + 1]; ++f) so += fabs(upperF[f]);
+      for (int k = A.losortStart[i]; k < A.losortStart[i + 1]; ++k)
+        so += fabs(upperF[losort[k]]);
+      v[1] = fmax(v[1], 1.0 + so / fabs(ds));
+    }
+  }
+}
+
+// Rows [r0, r1), stride blockDim: v += double(delta) (k_delta_mean)
+template <typename T>
+__device__ __forceinline__ void delta_mean_rows(int r0, int r1, const T* __restrict__ delta,
+                                                double (&v)[1]) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) v[0] += (double)delta[i];
+}
+
+// Rows [r0, r1), stride blockDim: psi += c, r -= c u; v += |r|, r (k_const_shift)
+__device__ __forceinline__ void const_shift_rows(int r0, int r1, double* __restrict__ psi,
+                                                 double* __restrict__ r,
+                                                 const double* __restrict__ u, double c,
+                                                 double (&v)[2]) {
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+    psi[i] += c;
+    const double ri = r[i] - c * u[i];
+    r[i] = ri;
+    v[0] += fabs(ri);
+    v[1] += ri;
+  }
+}
 
 // u = A 1 (sumA), partials: sum psi, #{|u_i| > 1e-8 |diag_i|}, u.u
 __global__ __launch_bounds__(1024) void k_rowsum(LduOp<double> A, const double* __restrict__ psi,
@@ -1262,15 +1410,7 @@ __global__ __launch_bounds__(1024) void k_rowsum(LduOp<double> A, const double* 
   int r0, r1;
   block_rows(A.n, lb, r0, r1);
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for_rows(A, r0, r1, [&](int i, auto row) {
-    const double pi = psi[i], di = A.diag[i];
-    const double ui = row(1.0, [](int) { return 1.0; });
-    u[i] = ui;
-    v[0] += pi;
-    v[1] += (fabs(ui) > 1e-8 * fabs(di)) ? 1.0 : 0.0;
-    v[2] += ui * ui;
-    v[3] += ui;
-  });
+  for_rows(A, r0, r1, RowsumRow{psi, A.diag, u, v});
   block_partials<4>(v, partials, pstride, lb);
 }
 
@@ -1285,17 +1425,8 @@ __global__ __launch_bounds__(1024) void k_initial_residual(LduOp<double> A,
   const int lb = logical_block(A.xcdRemap);
   int r0, r1;
   block_rows(A.n, lb, r0, r1);
-  const double xbar = os->xbar;
   double v[3] = {0.0, 0.0, 0.0};
-  for_rows(A, r0, r1, [&](int i, auto row) {
-    const double bi = b[i], t = u[i] * xbar;
-    const double wA = row(psi[i], [&](int j) { return psi[j]; });
-    const double ri = bi - wA;
-    r[i] = ri;
-    v[0] += fabs(ri);
-    v[1] += fabs(wA - t) + fabs(bi - t);
-    v[2] += ri;
-  });
+  for_rows(A, r0, r1, InitialResidualRow{psi, b, u, r, os->xbar, v});
   block_partials<3>(v, partials, pstride, lb);
 }
 
@@ -1316,23 +1447,7 @@ __global__ __launch_bounds__(1024) void k_prepare_inner(LduOp<double> A,
   int r0, r1;
   block_rows(A.n, lb, r0, r1);
   double v[2] = {0.0, -1.0e308};
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-    const double d = A.diag[i];
-    double ds = d;
-    if (deflate) {
-      ds = d - u[i];
-      if (ds * d <= kVSmall * fabs(ds)) v[0] += 1.0;
-    }
-    diagT[i] = (T)ds;
-    rDT[i] = (T)(1.0 / ds);
-    if (wantBeta) {
-      double so = 0.0;
-      for (int f = A.ownStart[i]; f < A.ownStart[i + 1]; ++f) so += fabs(upperF[f]);
-      for (int k = A.losortStart[i]; k < A.losortStart[i + 1]; ++k)
-        so += fabs(upperF[losort[k]]);
-      v[1] = fmax(v[1], 1.0 + so / fabs(ds));
-    }
-  }
+  prepare_rows<T>(A, r0, r1, upperF, losort, u, deflate, wantBeta, diagT, rDT, v);
   block_partials<2, true>(v, partials, pstride, lb);
 }
 
@@ -1364,7 +1479,7 @@ __global__ __launch_bounds__(1024) void k_delta_mean(int n, int xcdRemap,
   int r0, r1;
   block_rows(n, lb, r0, r1);
   double v[1] = {0.0};
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) v[0] += (double)delta[i];
+  delta_mean_rows<T>(r0, r1, delta, v);
   block_partials<1>(v, partials, pstride, lb);
 }
 
@@ -1380,18 +1495,7 @@ __global__ __launch_bounds__(1024) void k_refine_residual(
   block_rows(A.n, lb, r0, r1);
   const double dm = useMean ? os->dmean : 0.0;
   double v[3] = {0.0, 0.0, 0.0};
-  auto pf = [&](int j) { return psiOld[j] + ((double)delta[j] - dm); };
-  for_rows(A, r0, r1, [&](int i, auto row) {
-    const double bi = b[i], ui = u[i];
-    const double pi = pf(i);
-    const double wA = row(pi, pf);
-    psiNew[i] = pi;
-    const double ri = bi - wA;
-    r[i] = ri;
-    v[0] += ui * ri;
-    v[1] += fabs(ri);
-    v[2] += ri;
-  });
+  for_rows(A, r0, r1, RefineRow<T>{psiOld, delta, psiNew, b, u, r, dm, v});
   block_partials<3>(v, partials, pstride, lb);
 }
 
@@ -1403,15 +1507,8 @@ __global__ __launch_bounds__(1024) void k_const_shift(int n, int xcdRemap, doubl
   const int lb = logical_block(xcdRemap);
   int r0, r1;
   block_rows(n, lb, r0, r1);
-  const double c = os->c;
   double v[2] = {0.0, 0.0};
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-    psi[i] += c;
-    const double ri = r[i] - c * u[i];
-    r[i] = ri;
-    v[0] += fabs(ri);
-    v[1] += ri;
-  }
+  const_shift_rows(r0, r1, psi, r, u, os->c, v);
   block_partials<2>(v, partials, pstride, lb);
 }
 
