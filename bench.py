@@ -8,7 +8,10 @@ One step = one complete solve from psi = 0 to the FP64 L1 residual 1e-6
 (refined: FP32 Chebyshev-Jacobi PCG inner iterations, constant-mode deflation,
 FP64 refinement on the GPU).  Inputs are resident in HBM before timing starts.
 The line also carries small meshes (10k-cell Poisson and convection-diffusion,
-24^3) with the inner loop as one persistent launch against the launched loop.
+24^3) with the inner loop as one persistent launch against the launched loop,
+and the 2D Poisson at the reference benchmark's cavity sizes (400 to 640,000
+cells) beside the reference's published per-call times and the same-host CPU
+PCG(DIC).
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload poisson128]
 """
@@ -115,6 +118,20 @@ def _relmax(a, b):
 CPU_RUNS = 3  # each CPU time is the median of this many runs
 
 
+_NATIVE = []
+
+
+def _native_oracle(oracle):
+    """The CPU solvers built once on this host with -O3 -march=native (None when
+    that build fails; the prebuilt -march=x86-64-v2 build then runs)."""
+    import tempfile
+
+    if not _NATIVE:
+        oracle.lib()
+        _NATIVE.append(oracle.build_native(tempfile.mkdtemp(prefix="hipfoam_cpu_")))
+    return _NATIVE[0]
+
+
 def cpu_baseline(m, solver, sample_iters):
     """CPU solves on this host, in FP64, built here with -O3 -march=native:
 
@@ -132,12 +149,9 @@ def cpu_baseline(m, solver, sample_iters):
     timed and scaled to the iteration count of the full solve (committed
     fixture written by the oracle); no solution, no floor.
     Returns (baseline dict, x or None, floor dict or None)."""
-    import tempfile
-
     import oracle
 
-    oracle.lib()
-    native = oracle.build_native(tempfile.mkdtemp(prefix="hipfoam_cpu_"))
+    native = _native_oracle(oracle)
     prev = oracle.swap_library(native) if native is not None else None
     build = ("gcc -O3 -march=native -fopenmp, built on this host" if native is not None else
              "the prebuilt oracle (-O3 -march=x86-64-v2): the native build failed")
@@ -410,6 +424,109 @@ def small_meshes(H, mg, reps=20):
     return out
 
 
+# The reference's published pressure-solve table (BENCHMARK.md:59-69):
+# icoFoam 2D lid-driven cavity, fvMatrix::solve.p total / calls over 20 time
+# steps (40 pressure solves, warm-started from the previous field), OpenFOAM
+# v2506 on an Apple M3 Pro.  (side, cells, CPU PCG+DIC ms, metalPCG ms).
+REFERENCE_CAVITY_P = (
+    (20, 400, 0.066, 1.109),
+    (50, 2500, 2.082, 5.838),
+    (70, 4900, 6.434, 8.331),
+    (100, 10000, 20.52, 11.63),
+    (200, 40000, 173.8, 34.40),
+    (400, 160000, 1500.8, 249.7),
+    (800, 640000, 10508.3, 1787.8),
+)
+
+
+def reference_size_meshes(mg, sides):
+    """2D pressure Poisson side x side x 1 (unit faces, cell 0 pinned, zero-mean
+    U[-1,1] right-hand side) at the reference benchmark's mesh sizes."""
+    return [mg.poisson(n, n, 1, seed=1000, lognormal=False, name=f"poisson2d_{n}x{n}")
+            for n in sides]
+
+
+def reference_sizes(H, mg, sides, reps=20):
+    """Refined hipPCG at the reference benchmark's cavity sizes, each solve from
+    psi = 0 to the FP64 L1 residual 1e-6 (a cold start: at least the work of
+    the benchmark's warm-started calls).  Two times per mesh, medians of
+    `reps` solves (5 from 160,000 cells): device arrays (the solve alone) and
+    one plugin call on host arrays (hipfoam_set_topology's content check,
+    staging, copies both ways and the solve), set beside the reference's
+    published per-call times."""
+    ref = {r[0]: r for r in REFERENCE_CAVITY_P}
+    out = {}
+    ctx = H.Context(0)
+    for m, side in zip(reference_size_meshes(mg, sides), sides):
+        ctx.set_topology(m)
+        D = H.DeviceArray
+        d, u = D.from_host(ctx, m.diag), D.from_host(ctx, m.upper)
+        b = D.from_host(ctx, m.source)
+        psi = D.from_host(ctx, np.zeros(m.n))
+        zero = D.from_host(ctx, np.zeros(m.n))
+        ctl = H.default_controls(maxRestarts=8, maxIter=20000)
+        k = reps if m.n < 160000 else 5
+        ts = []
+        for _ in range(k + 2):
+            psi.copy_from(zero)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            perf = ctx.solve_device(H.PCG, d, u, None, psi, b, ctl)
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        rsum, _ = numpy_residual(m, psi.download())
+        th = []
+        hp = np.zeros(m.n)
+        for _ in range(k + 2):
+            hp[:] = 0.0
+            t0 = time.perf_counter()
+            ctx.set_topology(m)
+            ph = ctx.solve(H.PCG, m.diag, m.upper, None, hp, m.source, ctl)
+            th.append(time.perf_counter() - t0)
+        dev = 1e3 * float(np.median(ts[2:]))
+        host = 1e3 * float(np.median(th[2:]))
+        e = {"cells": m.n, "solve_ms": round(dev, 4), "host_arrays_ms": round(host, 4),
+             "iterations": perf.nIterations, "restarts": perf.restarts,
+             "converged": bool(perf.converged and ph.converged and rsum / perf.normFactor < 1e-6)}
+        if side in ref:
+            _, _, cpu_ms, gpu_ms = ref[side]
+            e.update({"reference_cpu_dic_ms": cpu_ms, "reference_metalpcg_ms": gpu_ms,
+                      "vs_reference_metalpcg": round(gpu_ms / host, 1)})
+        out[f"{side}x{side}"] = e
+        log(f"reference size {side}x{side}: {dev:.3f} ms device arrays, {host:.3f} ms host "
+            f"arrays ({perf.nIterations} it.); reference metalPCG "
+            f"{ref[side][3] if side in ref else '-'} ms")
+        for a in (d, u, b, psi, zero):
+            a.free()
+    ctx.close()
+    return out
+
+
+def cpu_reference_sizes(mg, sides):
+    """The CPU baseline at the reference benchmark's sizes: OpenFOAM PCG(DIC),
+    FP64, 1 core (median of CPU_RUNS runs; one run from 640,000 cells)."""
+    import oracle
+
+    native = _native_oracle(oracle)
+    prev = oracle.swap_library(native) if native is not None else None
+    try:
+        out = {}
+        for m, side in zip(reference_size_meshes(mg, sides), sides):
+            runs = []
+            for _ in range(CPU_RUNS if m.n < 640000 else 1):
+                t0 = time.perf_counter()
+                x, perf = oracle.pcg(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=20000)
+                runs.append(time.perf_counter() - t0)
+            out[f"{side}x{side}"] = {"cpu_dic_ms": round(1e3 * float(np.median(runs)), 4),
+                                     "cpu_iterations": perf.nIterations,
+                                     "cpu_converged": bool(perf.converged),
+                                     "cpu_runs": len(runs)}
+        return out
+    finally:
+        if prev is not None:
+            oracle.swap_library(prev)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -433,7 +550,14 @@ def main():
                     help="SpMV launches timed after a MALL-flushing copy")
     ap.add_argument("--extras", type=int, default=1,
                     help="poisson128: also measure the past-MALL 256^3 mesh, the LDU face "
-                         "schedule and the 96^3 BiCGStab kernels")
+                         "schedule, the 96^3 BiCGStab kernels, renumbered meshes, the small "
+                         "meshes and the reference's cavity sizes (--ref-sizes)")
+    ap.add_argument("--small-meshes", type=int, default=1,
+                    help="poisson128 / poisson2d100 with --extras: the small-mesh section "
+                         "(persistent against launched loops)")
+    ap.add_argument("--ref-sizes", default="20,50,70,100,200,400,800",
+                    help="poisson128 / poisson2d100 with --extras: 2D Poisson sides of the "
+                         "reference benchmark's cavity table to solve (empty: none)")
     ap.add_argument("--adjoint-precision", choices=["fp32", "fp64"], default="fp32",
                     help="adjoint128: transpose solve refined (FP32 inner) or plain FP64")
     ap.add_argument("--fd-faces", type=int, default=0,
@@ -618,8 +742,14 @@ def main():
 
     # ---- small meshes: the persistent inner loop against the launched one
     small = None
-    if args.extras and args.workload in ("poisson128", "poisson2d100"):
+    if args.extras and args.small_meshes and args.workload in ("poisson128", "poisson2d100"):
         small = small_meshes(H, mg)
+
+    # ---- the reference benchmark's cavity sizes (2D Poisson, 400 to 640k cells)
+    sides = [int(v) for v in args.ref_sizes.split(",") if v.strip()]
+    sizes = None
+    if args.extras and args.workload in ("poisson128", "poisson2d100") and sides:
+        sizes = reference_sizes(H, mg, sides)
 
     # ---- more rooflines: a mesh past the 256 MB MALL, the LDU face schedule
     # priced on the LDU arrays, the BiCGStab kernels of the 96^3 workload
@@ -655,6 +785,14 @@ def main():
     if not args.no_cpu:
         log("cpu baseline ...")
         cpu, x_cpu, floor = cpu_baseline(m, solver, args.cpu_iters)
+        if sizes is not None:
+            cpu["reference_sizes"] = cpu_reference_sizes(mg, sides)
+            for tag, c in cpu["reference_sizes"].items():
+                sizes[tag]["cpu_dic_ms"] = c["cpu_dic_ms"]
+                sizes[tag]["speedup_vs_cpu"] = round(
+                    c["cpu_dic_ms"] / sizes[tag]["host_arrays_ms"], 2)
+            log("reference sizes, CPU PCG(DIC) 1 core: " +
+                ", ".join(f"{t} {c['cpu_dic_ms']} ms" for t, c in cpu["reference_sizes"].items()))
         log(f"cpu baseline: {cpu['value']} ms ({cpu['sample']}; {cpu['build']})")
         if cpu.get("all_cores"):
             log(f"cpu all cores: {cpu['all_cores']}")
@@ -701,6 +839,7 @@ def main():
         "spmv": None if spmv is None else dict(spmv, mall_flushed=cold, autotuned=tuned),
         "rooflines_more": extras,
         "small_meshes": small,
+        "reference_sizes": sizes,
         "solve": {
             "iterations": perf.nIterations,
             "restarts": perf.restarts,

@@ -397,6 +397,15 @@ typedef struct
                              never.  The
                              same rows, partials and fold order as the
                              launched loop: identical results */
+    int32_t localLoop;    /* Chebyshev PCG inner loop in one workgroup of
+                             512 threads with every vector in LDS, on a
+                             mesh of at most 2,048 rows (FP32 inner loop)
+                             or 1,536 (plain FP64) whose tiles are all
+                             row-slot tiles, on the per-tile launch shape:
+                             -1 = auto (default), 0 = never.  Taken before
+                             persistent where both apply; the same rows,
+                             partials and fold order as the launched loop:
+                             identical results */
 } hipfoam_schedule;
 
 int32_t hipfoam_get_schedule(const hipfoam_context* ctx, hipfoam_schedule* s);
@@ -431,6 +440,9 @@ int32_t hipfoam_set_schedule(hipfoam_context* ctx, const hipfoam_schedule* s);
  * abandoned).  Any pointer may be NULL.  No reference counterpart. */
 int32_t hipfoam_persistent_stats(const hipfoam_context* ctx, int64_t* launches,
                                  int64_t* abandoned, int64_t* lastAbort);
+/* Inner loops this context ran in one workgroup (schedule field localLoop).
+ * No reference counterpart. */
+int32_t hipfoam_local_stats(const hipfoam_context* ctx, int64_t* launches);
 /* Times the candidate SpMV schedules on the staged operator and keeps the
  * fastest; *best (may be NULL) receives it. */
 int32_t hipfoam_autotune(hipfoam_context* ctx, hipfoam_schedule* best);

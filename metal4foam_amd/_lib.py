@@ -61,7 +61,8 @@ class Schedule(C.Structure):
     _fields_ = [("spmvSchedule", C.c_int32), ("blockSize", C.c_int32),
                 ("xcdRemap", C.c_int32), ("graphChunk", C.c_int32),
                 ("gridLimit", C.c_int32), ("rowForm", C.c_int32),
-                ("persistent", C.c_int32)]
+                ("persistent", C.c_int32),
+                ("localLoop", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -138,6 +139,7 @@ SIGNATURES = {
     "hipfoam_slot_tiles": (C.c_int32, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                        C.POINTER(C.c_int64)]),
     "hipfoam_set_schedule": (C.c_int32, [_P, C.POINTER(Schedule)]),
+    "hipfoam_local_stats": (C.c_int32, [_P, C.POINTER(C.c_int64)]),
     "hipfoam_persistent_stats": (C.c_int32, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                              C.POINTER(C.c_int64)]),
     "hipfoam_autotune": (C.c_int32, [_P, C.POINTER(Schedule)]),

@@ -27,6 +27,7 @@
 #include "../../include/hipfoam.h"
 #include "hipfoam_batched.hpp"
 #include "hipfoam_device.hpp"
+#include "hipfoam_local.hpp"
 #include "hipfoam_persist.hpp"
 #include "hipfoam_slots.hpp"
 #include "hipfoam_stage.hpp"
@@ -197,10 +198,11 @@ struct hipfoam_context {
   int64_t persistLaunches = 0;   // persistent inner loops launched
   int64_t persistAbandoned = 0;  // ... of which abandoned (the launched loop ran instead)
   int64_t persistLastAbort = 0;  // abort word of the last abandoned launch (XcdSync)
+  int64_t localLaunches = 0;     // inner loops run in one workgroup (k_pcg_local)
   hipEvent_t evPoll[2] = {nullptr, nullptr};
   hipEvent_t evT0 = nullptr, evT1 = nullptr;
 
-  hipfoam_schedule sched{2, 0, 2, -1, 0, 0, -1};
+  hipfoam_schedule sched{2, 0, 2, -1, 0, 0, -1, -1};
   std::map<GraphKey, hipGraphExec_t> graphs;
   KernelTimer timer;
   // host helper threads (created on first use): staged copies, addressing hash
@@ -820,6 +822,45 @@ inline bool persistent_ok(const hipfoam_context* c, Kind kind, int order) {
   return ntiles <= (c->sched.persistent > 0 ? kAutoSmallTiles : autoTiles);
 }
 
+// Whether the inner loop runs in one workgroup (k_pcg_local): Chebyshev PCG
+// on the per-tile launch shape, whose rows, partials and fold order it
+// reproduces, on a mesh of at most local_max_rows<T>() rows whose tiles are
+// all row-slot tiles (schedule.localLoop -1, the default); never under kernel
+// timing.
+template <typename T>
+inline bool local_ok(const hipfoam_context* c, Kind kind, int order, const LduOp<T>& A) {
+  if (c->sched.localLoop == 0 || c->timer.on || kind != KIND_PCG || order < 2) return false;
+  if (!A.slots || !c->slotOK || c->slotFaceRows != 0 || c->n > local_max_rows<T>()) return false;
+  const int ntiles = (c->n + kSlotTile - 1) / kSlotTile;
+  return row_block(c) == kSlotTile && row_grid(c) == ntiles && merge_stages(c);
+}
+
+// The whole iteration loop in one workgroup; vectors and the state must be
+// initialised.  Ends with the state out of RUNNING (maxIter + 4 iterations
+// at most).
+template <typename T>
+int local_loop(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order, int maxIter,
+               double* gpuSeconds) {
+  Work<T>& w = work<T>(c);
+  const int loops = std::max(maxIter, 1) + 4;
+  const PcgVectors<T> V{w.r.p,     w.r1.p,        w.p0.p, w.p1.p, w.wA.p, w.Ap.p,
+                        w.delta.p, w.deltaBest.p, w.z0.p, w.z1.p, w.w.p};
+  c->localLaunches += 1;
+  CK(hipEventRecord(c->evT0, c->stream));
+  hipLaunchKernelGGL((k_pcg_local<T>), dim3(1), dim3(kLocalThreads), 0, c->stream, A, rD, V,
+                     c->kst.p, c->kst2.p, c->partials.p, c->partials2.p, c->pstride, order,
+                     loops);
+  CKL();
+  CK(hipEventRecord(c->evT1, c->stream));
+  CK(hipEventSynchronize(c->evT1));
+  float ms = 0.f;
+  CK(hipEventElapsedTime(&ms, c->evT0, c->evT1));
+  if (gpuSeconds) *gpuSeconds += 1e-3 * ms;
+  CK(hipMemcpy(c->hk, c->kst.p, sizeof(KState), hipMemcpyDeviceToHost));
+  if (c->hk->status != ST_RUNNING) return HIPFOAM_OK;
+  return fail(c, HIPFOAM_ERR_STATE, "inner loop did not terminate within maxIter iterations");
+}
+
 // One persistent launch of the whole iteration loop; vectors and the state
 // must be initialised (and the BiCGStab start enqueued).  *abandoned: the
 // participants were not on one XCD or a barrier wait timed out, so the launch
@@ -879,6 +920,7 @@ int persistent_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* r
 template <typename T, typename Reinit>
 int iterate(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
             int maxIter, double* gpuSeconds, Reinit reinit) {
+  if (local_ok<T>(c, kind, order, A)) return local_loop<T>(c, A, rD, order, maxIter, gpuSeconds);
   if (persistent_ok<T>(c, kind, order)) {
     if (kind == KIND_BICG) {
       int rc = enqueue_bicg_start<T>(c, A);
@@ -2329,6 +2371,8 @@ int32_t hipfoam_set_schedule(hipfoam_context* c, const hipfoam_schedule* s) {
   if (s->rowForm < 0 || s->rowForm > 1) return fail(c, HIPFOAM_ERR_ARG, "rowForm must be 0 or 1");
   if (s->persistent < -1 || s->persistent > 1)
     return fail(c, HIPFOAM_ERR_ARG, "persistent must be -1 (auto), 0 (off) or 1 (on)");
+  if (s->localLoop < -1 || s->localLoop > 0)
+    return fail(c, HIPFOAM_ERR_ARG, "localLoop must be -1 (auto) or 0 (off)");
   if (c->good) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
@@ -2346,6 +2390,12 @@ int32_t hipfoam_persistent_stats(const hipfoam_context* c, int64_t* launches,
   if (launches) *launches = c->persistLaunches;
   if (abandoned) *abandoned = c->persistAbandoned;
   if (lastAbort) *lastAbort = c->persistLastAbort;
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_local_stats(const hipfoam_context* c, int64_t* launches) {
+  if (!c || !launches) return HIPFOAM_ERR_ARG;
+  *launches = c->localLaunches;
   return HIPFOAM_OK;
 }
 

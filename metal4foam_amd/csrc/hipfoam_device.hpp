@@ -430,7 +430,12 @@ struct RowSlots {
   // are skipped, so the terms and their order are those of the face loops
   template <typename XF>
   __device__ __forceinline__ T apply(const T* diag, int i, T xi, XF xf) const {
-    T s = diag[i] * xi;
+    return apply_d(diag[i], xi, xf);
+  }
+  // the same with the row's diagonal coefficient di already at hand
+  template <typename XF>
+  __device__ __forceinline__ T apply_d(T di, T xi, XF xf) const {
+    T s = di * xi;
     T xq[2 * kShortRow];
 #pragma unroll
     for (int q = 0; q < 2 * kShortRow; ++q) xq[q] = xf(col[q]);

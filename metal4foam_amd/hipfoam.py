@@ -344,6 +344,13 @@ class Context:
                                                       C.byref(w)), "persistent_stats")
         return a.value, b.value, w.value
 
+    def local_launches(self) -> int:
+        """Inner loops run in one workgroup (schedule field localLoop) since
+        the context was created."""
+        a = C.c_int64()
+        self._check(self.lib.hipfoam_local_stats(self.h, C.byref(a)), "local_stats")
+        return a.value
+
     def set_kernel_timing(self, enable, kclass=0):
         self._check(self.lib.hipfoam_set_kernel_timing(self.h, int(enable), kclass),
                     "set_kernel_timing")

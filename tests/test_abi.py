@@ -74,6 +74,13 @@ def test_schedule_validation_without_device():
         bad = _lib.Schedule(0, 256, 1, 8, 0, 0, p)
         assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(bad)) == _lib.ERR_ARG
     assert s.persistent == -1
+    for lp in (-2, 1):
+        bad = _lib.Schedule(0, 256, 1, 8, 0, 0, -1, lp)
+        assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(bad)) == _lib.ERR_ARG
+    assert s.localLoop == -1
+    assert ctx.local_launches() == 0
+    assert ctx.lib.hipfoam_local_stats(ctx.h, None) == _lib.ERR_ARG
+    assert ctx.lib.hipfoam_local_stats(None, C.byref(C.c_int64())) == _lib.ERR_ARG
     ok = _lib.Schedule(1, 512, 0, 16, 1024)
     assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(ok)) == _lib.OK
     assert ctx.get_schedule().as_dict() == ok.as_dict()

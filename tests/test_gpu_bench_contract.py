@@ -2,8 +2,8 @@
 the required keys and types, a time-like value with higher_is_better false,
 a roofline whose fraction is achieved / peak and below 1 and whose achieved
 rate is its bytes per launch over its average launch time, a CPU baseline
-timed on one core with its sample stated, and --gpus above 1 refused with a
-message."""
+timed on one core with its sample stated, the small-mesh and reference-size
+sections, and --gpus above 1 refused with a message."""
 
 import json
 import pathlib
@@ -57,14 +57,43 @@ def test_bench_small_meshes():
     loop (never abandoned) in the same iterations as the launched loop."""
     r, out = _bench("--workload", "poisson2d100", "--steps", "1", "--warmup", "1",
                     "--fp64-steps", "0", "--host-steps", "0", "--spmv-reps", "2",
-                    "--spmv-tune", "0", "--cold-reps", "0", "--no-cpu", "--extras", "1")
+                    "--spmv-tune", "0", "--cold-reps", "0", "--no-cpu", "--extras", "1",
+                    "--ref-sizes", "")
     assert r.returncode == 0, r.stderr[-3000:]
+    assert out["reference_sizes"] is None
     small = out["small_meshes"]
     assert set(small) == {"poisson2d100", "convdiff2d100", "poisson24"}
     for tag, e in small.items():
         assert e["converged"] and e["same_iterations"], tag
         assert e["persistent_launches"] >= 1 and e["abandoned"] == 0, tag
         assert e["solve_ms"] > 0 and e["launched_loop_ms"] > 0, tag
+
+
+def test_bench_reference_sizes():
+    """The reference benchmark's cavity sizes: each 2D Poisson converged, its
+    published M3 Pro per-call times carried unchanged, the same-host CPU
+    PCG(DIC) time taken from the CPU leg, and the ratios their quotients."""
+    published = {"20x20": (0.066, 1.109), "50x50": (2.082, 5.838),
+                 "100x100": (20.52, 11.63)}
+    r, out = _bench("--workload", "poisson2d100", "--steps", "1", "--warmup", "0",
+                    "--fp64-steps", "0", "--host-steps", "0", "--spmv-reps", "2",
+                    "--spmv-tune", "0", "--cold-reps", "0", "--extras", "1",
+                    "--ref-sizes", "20,50,100")
+    assert r.returncode == 0, r.stderr[-3000:]
+    sizes = out["reference_sizes"]
+    assert set(sizes) == set(published)
+    cpu = out["cpu_baseline"]["reference_sizes"]
+    for tag, (cpu_ref, gpu_ref) in published.items():
+        e = sizes[tag]
+        side = int(tag.split("x")[0])
+        assert e["cells"] == side * side and e["converged"], tag
+        assert e["solve_ms"] > 0 and e["host_arrays_ms"] > 0, tag
+        assert (e["reference_cpu_dic_ms"], e["reference_metalpcg_ms"]) == (cpu_ref, gpu_ref)
+        assert e["vs_reference_metalpcg"] == pytest.approx(gpu_ref / e["host_arrays_ms"],
+                                                           abs=0.051)
+        assert cpu[tag]["cpu_converged"] and e["cpu_dic_ms"] == cpu[tag]["cpu_dic_ms"] > 0
+        assert e["speedup_vs_cpu"] == pytest.approx(e["cpu_dic_ms"] / e["host_arrays_ms"],
+                                                    abs=0.0051)
 
 
 def test_bench_refuses_several_gpus():

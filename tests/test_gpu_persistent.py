@@ -21,10 +21,12 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(autouse=True)
 def _auto_schedule(ctx):
+    # the one-workgroup loop (localLoop) would take the meshes of at most
+    # 3,072 rows: off here, so every case runs the one-XCD loop
     ctx.set_schedule(spmvSchedule=2, blockSize=0, xcdRemap=2, graphChunk=-1, gridLimit=0,
-                     rowForm=0, persistent=-1)
+                     rowForm=0, persistent=-1, localLoop=0)
     yield
-    ctx.set_schedule(persistent=-1)
+    ctx.set_schedule(persistent=-1, localLoop=-1)
 
 
 def polish_case():
