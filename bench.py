@@ -439,6 +439,92 @@ REFERENCE_CAVITY_P = (
 )
 
 
+# The reference's momentum table (BENCHMARK.md:112-119): icoFoam cavity 200x200
+# (40,000 cells), fvMatrix::solve.U total / calls, Ux and Uy per call, Apple
+# M3 Pro: CPU PBiCGStab with the diagonal preconditioner, and metalPBiCGStab.
+REFERENCE_CAVITY_U = {"cells": 40000, "cpu_pbicgstab_diagonal_ms": 11.16,
+                      "metalpbicgstab_ms": 13.75}
+
+
+def momentum_matrix(mg):
+    """Stand-in for the cavity's 200x200 momentum matrix: upwind
+    convection-diffusion (umax 2) with a transient diagonal term (shift 1) that
+    makes PBiCGStab(diagonal) converge in 15 iterations, as the benchmark's
+    momentum matrices converge in about 15; two N(0,1) right-hand sides, the
+    Ux and Uy components of one call."""
+    return mg.convection_diffusion(200, 200, 1, seed=1005, nrhs=2, shift=1.0, umax=2.0,
+                                   name="momentum2d_200x200")
+
+
+def reference_momentum(H, mg, reps=20):
+    """One momentum call of the reference benchmark as a plugin makes it: two
+    refined hipPBiCGStab solves (Ux, Uy) of the same matrix on host arrays,
+    each with hipfoam_set_topology's content check, from psi = 0 to 1e-6;
+    median of `reps` calls."""
+    m = momentum_matrix(mg)
+    comps = [np.ascontiguousarray(m.source[:, k]) for k in range(2)]
+    ctx = H.Context(0)
+    ctl = H.default_controls(maxRestarts=8, maxIter=20000)
+    ts = []
+    for _ in range(reps + 2):
+        psis = [np.zeros(m.n), np.zeros(m.n)]
+        t0 = time.perf_counter()
+        perfs = []
+        for k in range(2):
+            ctx.set_topology(m)
+            perfs.append(ctx.solve(H.PBICGSTAB, m.diag, m.upper, m.lower, psis[k], comps[k],
+                                   ctl))
+        ts.append(time.perf_counter() - t0)
+    conv = True
+    for k in range(2):
+        mk = m
+        mk.source = comps[k]
+        rsum, _ = numpy_residual(mk, psis[k])
+        conv = conv and bool(perfs[k].converged and rsum / perfs[k].normFactor < 1e-6)
+    ms = 1e3 * float(np.median(ts[2:]))
+    ref = REFERENCE_CAVITY_U
+    out = {"what": "2D upwind convection-diffusion 200x200x1 with a transient diagonal term, "
+                   "Ux and Uy per call, refined hipPBiCGStab on host arrays (the benchmark's "
+                   "cavity momentum call)",
+           "cells": m.n, "call_ms": round(ms, 4),
+           "iterations": [p.nIterations for p in perfs],
+           "restarts": [p.restarts for p in perfs], "converged": conv,
+           "reference_cpu_pbicgstab_diagonal_ms": ref["cpu_pbicgstab_diagonal_ms"],
+           "reference_metalpbicgstab_ms": ref["metalpbicgstab_ms"],
+           "vs_reference_metalpbicgstab": round(ref["metalpbicgstab_ms"] / ms, 1)}
+    log(f"reference momentum call: {ms:.3f} ms (iterations {out['iterations']}); reference "
+        f"metalPBiCGStab {ref['metalpbicgstab_ms']} ms")
+    ctx.close()
+    return out
+
+
+def cpu_reference_momentum(mg):
+    """The CPU side of the momentum call: OpenFOAM PBiCGStab with the diagonal
+    preconditioner (the reference's CPU solver for it), FP64, 1 core, Ux and Uy
+    (median of CPU_RUNS calls)."""
+    import oracle
+
+    native = _native_oracle(oracle)
+    prev = oracle.swap_library(native) if native is not None else None
+    try:
+        m = momentum_matrix(mg)
+        comps = [np.ascontiguousarray(m.source[:, k]) for k in range(2)]
+        runs, its = [], []
+        for _ in range(CPU_RUNS):
+            t0 = time.perf_counter()
+            its = []
+            for k in range(2):
+                x, perf = oracle.pbicgstab(m, precon=oracle.PRE_DIAGONAL, tol=1e-6,
+                                           maxIter=20000, source=comps[k])
+                its.append(perf.nIterations)
+            runs.append(time.perf_counter() - t0)
+        return {"cpu_pbicgstab_diagonal_ms": round(1e3 * float(np.median(runs)), 4),
+                "cpu_iterations": its, "cpu_runs": len(runs)}
+    finally:
+        if prev is not None:
+            oracle.swap_library(prev)
+
+
 def reference_size_meshes(mg, sides):
     """2D pressure Poisson side x side x 1 (unit faces, cell 0 pinned, zero-mean
     U[-1,1] right-hand side) at the reference benchmark's mesh sizes."""
@@ -558,6 +644,9 @@ def main():
     ap.add_argument("--ref-sizes", default="20,50,70,100,200,400,800",
                     help="poisson128 / poisson2d100 with --extras: 2D Poisson sides of the "
                          "reference benchmark's cavity table to solve (empty: none)")
+    ap.add_argument("--ref-momentum", type=int, default=1,
+                    help="poisson128 / poisson2d100 with --extras: the reference benchmark's "
+                         "momentum call (two BiCGStab components on 40,000 cells)")
     ap.add_argument("--adjoint-precision", choices=["fp32", "fp64"], default="fp32",
                     help="adjoint128: transpose solve refined (FP32 inner) or plain FP64")
     ap.add_argument("--fd-faces", type=int, default=0,
@@ -748,8 +837,11 @@ def main():
     # ---- the reference benchmark's cavity sizes (2D Poisson, 400 to 640k cells)
     sides = [int(v) for v in args.ref_sizes.split(",") if v.strip()]
     sizes = None
+    momentum = None
     if args.extras and args.workload in ("poisson128", "poisson2d100") and sides:
         sizes = reference_sizes(H, mg, sides)
+    if args.extras and args.workload in ("poisson128", "poisson2d100") and args.ref_momentum:
+        momentum = reference_momentum(H, mg)
 
     # ---- more rooflines: a mesh past the 256 MB MALL, the LDU face schedule
     # priced on the LDU arrays, the BiCGStab kernels of the 96^3 workload
@@ -793,6 +885,14 @@ def main():
                     c["cpu_dic_ms"] / sizes[tag]["host_arrays_ms"], 2)
             log("reference sizes, CPU PCG(DIC) 1 core: " +
                 ", ".join(f"{t} {c['cpu_dic_ms']} ms" for t, c in cpu["reference_sizes"].items()))
+        if momentum is not None:
+            cm = cpu_reference_momentum(mg)
+            cpu["reference_momentum"] = cm
+            momentum["cpu_pbicgstab_diagonal_ms"] = cm["cpu_pbicgstab_diagonal_ms"]
+            momentum["speedup_vs_cpu"] = round(cm["cpu_pbicgstab_diagonal_ms"] /
+                                               momentum["call_ms"], 2)
+            log(f"reference momentum, CPU PBiCGStab(diagonal) 1 core: "
+                f"{cm['cpu_pbicgstab_diagonal_ms']} ms (iterations {cm['cpu_iterations']})")
         log(f"cpu baseline: {cpu['value']} ms ({cpu['sample']}; {cpu['build']})")
         if cpu.get("all_cores"):
             log(f"cpu all cores: {cpu['all_cores']}")
@@ -840,6 +940,7 @@ def main():
         "rooflines_more": extras,
         "small_meshes": small,
         "reference_sizes": sizes,
+        "reference_momentum": momentum,
         "solve": {
             "iterations": perf.nIterations,
             "restarts": perf.restarts,

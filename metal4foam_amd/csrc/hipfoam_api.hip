@@ -454,6 +454,12 @@ LduOp<double> op64(hipfoam_context* c) {
 }
 
 
+KStateWords state_words(const KState& s) {
+  KStateWords w;
+  std::memcpy(&w, &s, sizeof(KState));
+  return w;
+}
+
 template <typename T>
 Work<T>& work(hipfoam_context* c);
 template <>
@@ -835,12 +841,11 @@ inline bool local_ok(const hipfoam_context* c, Kind kind, int order, const LduOp
   return row_block(c) == kSlotTile && row_grid(c) == ntiles && merge_stages(c);
 }
 
-// The whole iteration loop in one workgroup; vectors and the state must be
-// initialised.  Ends with the state out of RUNNING (maxIter + 4 iterations
-// at most).
+// The whole iteration loop in one workgroup, enqueued between the timing
+// events evT0 and evT1; vectors and the state must be initialised.  The
+// state leaves RUNNING within maxIter + 4 iterations.
 template <typename T>
-int local_loop(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order, int maxIter,
-               double* gpuSeconds) {
+int local_enqueue(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order, int maxIter) {
   Work<T>& w = work<T>(c);
   const int loops = std::max(maxIter, 1) + 4;
   const PcgVectors<T> V{w.r.p,     w.r1.p,        w.p0.p, w.p1.p, w.wA.p, w.Ap.p,
@@ -852,13 +857,27 @@ int local_loop(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order, in
                      loops);
   CKL();
   CK(hipEventRecord(c->evT1, c->stream));
-  CK(hipEventSynchronize(c->evT1));
+  return HIPFOAM_OK;
+}
+
+// After evT1 has completed and the state has been copied to c->hk: the GPU
+// time of the loop, and the check that it ended.
+int local_finish(hipfoam_context* c, double* gpuSeconds) {
   float ms = 0.f;
   CK(hipEventElapsedTime(&ms, c->evT0, c->evT1));
   if (gpuSeconds) *gpuSeconds += 1e-3 * ms;
-  CK(hipMemcpy(c->hk, c->kst.p, sizeof(KState), hipMemcpyDeviceToHost));
   if (c->hk->status != ST_RUNNING) return HIPFOAM_OK;
   return fail(c, HIPFOAM_ERR_STATE, "inner loop did not terminate within maxIter iterations");
+}
+
+template <typename T>
+int local_loop(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order, int maxIter,
+               double* gpuSeconds) {
+  int rc = local_enqueue<T>(c, A, rD, order, maxIter);
+  if (rc) return rc;
+  CK(hipEventSynchronize(c->evT1));
+  CK(hipMemcpy(c->hk, c->kst.p, sizeof(KState), hipMemcpyDeviceToHost));
+  return local_finish(c, gpuSeconds);
 }
 
 // One persistent launch of the whole iteration loop; vectors and the state
@@ -939,13 +958,26 @@ int iterate(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int o
   return run_loop<T>(c, kind, A, rD, order, maxIter, gpuSeconds);
 }
 
+// An inner solve left running on the stream (inner_solve with a pending
+// record): its loop, the best-iterate restore and the copy of its final
+// state are enqueued; inner_finish reads its statistics once the caller has
+// synchronised the stream.
+struct InnerPending {
+  bool on = false;
+  double tw0 = 0.0;
+};
+
 // Inner correction solve in reference-backend semantics
 // (metalFoamBackend::pcg / ::bicgstab): r64 -> FP32 residual (minus *mean,
 // times proj when given, when mean is given), delta0 = 0, device loop,
-// best-iterate restore.
+// best-iterate restore.  With pending (not NULL), a loop that runs in one
+// workgroup is only enqueued, together with the restore (on the device) and
+// the state copy: the caller queues its next steps behind it and calls
+// inner_finish after its next synchronisation.
 int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* mean,
                 const double* proj, double target, int maxIter, int ips, int order,
-                double beta, double ratio, hipfoam_stats* stats) {
+                double beta, double ratio, hipfoam_stats* stats,
+                InnerPending* pending = nullptr) {
   const double tw0 = now_seconds();
   hipfoam_stats st{};
   if (!c->good || c->n == 0) {
@@ -972,11 +1004,9 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
     const dim3 g1(blocks_for(n, 256)), b1(256);
     hipLaunchKernelGGL((k_inner_init<float>), g1, b1, 0, c->stream, n, r64, mean, proj, w.r.p,
                        kind == KIND_BICG ? w.rA0.p : nullptr, w.delta.p, w.p0.p, w.p1.p,
-                       kind == KIND_BICG ? w.AyA.p : nullptr);
+                       kind == KIND_BICG ? w.AyA.p : nullptr, state_words(s), c->kst.p,
+                       c->kst2.p);
     CKL();
-    *c->hk = s;
-    CK(hipMemcpyAsync(c->kst.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
-    CK(hipMemcpyAsync(c->kst2.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
     return HIPFOAM_OK;
   };
   int rc0 = init();
@@ -984,11 +1014,23 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
   double gpu = 0;
   if (s.status == ST_RUNNING) {
     const LduOp<float> A = op32(c);
-    int rc = iterate<float>(c, kind, A, c->rD32.p, kind == KIND_PCG ? order : 1, maxIter, &gpu,
-                            init);
+    const int ord = kind == KIND_PCG ? order : 1;
+    if (pending && local_ok<float>(c, kind, ord, A)) {
+      int rc = local_enqueue<float>(c, A, c->rD32.p, ord, maxIter);
+      if (rc) return rc;
+      hipLaunchKernelGGL((k_restore_best<float>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                         c->stream, n, c->kst.p, w.delta.p, w.deltaBest.p);
+      CKL();
+      CK(hipMemcpyAsync(c->hk, c->kst.p, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+      pending->on = true;
+      pending->tw0 = tw0;
+      return HIPFOAM_OK;
+    }
+    int rc = iterate<float>(c, kind, A, c->rD32.p, ord, maxIter, &gpu, init);
     if (rc) return rc;
   } else {
     CK(hipStreamSynchronize(c->stream));
+    *c->hk = s;
   }
   const KState& h = *c->hk;
   st.nIterations = h.it;
@@ -1004,6 +1046,28 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
   }
   st.tGpuSeconds = gpu;
   st.tWallSeconds = now_seconds() - tw0;
+  if (stats) *stats = st;
+  return HIPFOAM_OK;
+}
+
+// Statistics of a pending inner solve (its loop ran from RUNNING; the state
+// is in c->hk and the restore has run), once the stream has been
+// synchronised: the same as inner_solve reports.
+int inner_finish(hipfoam_context* c, InnerPending& pending, hipfoam_stats* stats) {
+  hipfoam_stats st{};
+  double gpu = 0;
+  pending.on = false;
+  int rc = local_finish(c, &gpu);
+  if (rc) return rc;
+  const KState& h = *c->hk;
+  st.nIterations = h.it;
+  st.finalResSum = h.res;
+  st.converged = h.status == ST_CONVERGED;
+  st.singular = h.status == ST_SINGULAR;
+  st.stagnated = h.status == ST_STAGNATED;
+  if (h.bestRes < st.finalResSum) st.finalResSum = h.bestRes;
+  st.tGpuSeconds = gpu;
+  st.tWallSeconds = now_seconds() - pending.tw0;
   if (stats) *stats = st;
   return HIPFOAM_OK;
 }
@@ -1081,7 +1145,7 @@ int prepare_inner(hipfoam_context* c, int deflate, int wantBeta) {
   hipLaunchKernelGGL((k_prepare_inner<float>), dim3(row_grid(c)), dim3(bs), 0, c->stream,
                      A, c->upper64.p, c->losort.p, c->u64.p, deflate, wantBeta, c->diag32.p,
                      c->rD32.p, c->ost.p, c->partials.p, c->pstride);
-  launch_fin<2, true>(c, row_grid(c), false, 0, FinPrepare{c->ost.p, deflate, wantBeta});
+  launch_fin<2, true>(c, row_grid(c), false, 0, FinPrepare{c->ost.p, deflate, wantBeta, c->n});
   CKL();
   c->have32 = true;
   return HIPFOAM_OK;
@@ -1093,8 +1157,10 @@ int read_ostate(hipfoam_context* c) {
   return HIPFOAM_OK;
 }
 
-// initial FP64 residual + normFactor (metalFoamSolver.C:101-129): psi in psiCur.
-int initial_residual(hipfoam_context* c, const double* psiCur, const double* src) {
+// initial FP64 residual + normFactor (metalFoamSolver.C:101-129): psi in psiCur;
+// read: wait for it and copy the outer state to the host.
+int initial_residual(hipfoam_context* c, const double* psiCur, const double* src,
+                     bool read = true) {
   const int bs = row_block(c);
   LduOp<double> A = op64(c);
   const dim3 g(row_grid(c)), bl(bs);
@@ -1105,7 +1171,7 @@ int initial_residual(hipfoam_context* c, const double* psiCur, const double* src
                      c->r64.p, c->ost.p, c->partials.p, c->pstride);
   launch_fin<3>(c, g.x, false, 0, FinInitialResidual{c->ost.p, c->n});
   CKL();
-  return read_ostate(c);
+  return read ? read_ostate(c) : HIPFOAM_OK;
 }
 
 bool of_converged_h(double res, double res0, double tol, double relTol) {
@@ -1148,12 +1214,39 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
   rc = stage_matrix(c, diag, upper, lower, matKind);
   if (rc) return rc;
   const double tStage = now_seconds() - t0;
-  XF(c->psi0.p, psi, sizeof(double) * n, kind);
-  XF(c->b.p, source, sizeof(double) * n, kind);
-  double* psiCur = c->psi0.p;
+  // device arrays: refine in the caller's psi and read its source in place
+  // (psi ping-pongs with psi1; the last copy back is needed only when the
+  // solution ends in psi1); host arrays, or psi overlapping source: copies
+  const bool inPlace =
+      kind == hipMemcpyDeviceToDevice && (psi + n <= source || source + n <= psi);
+  double* psiCur = inPlace ? psi : c->psi0.p;
   double* psiNxt = c->psi1.p;
-  rc = initial_residual(c, psiCur, c->b.p);
+  const double* src = inPlace ? source : c->b.p;
+  if (!inPlace) {
+    XF(c->psi0.p, psi, sizeof(double) * n, kind);
+    XF(c->b.p, source, sizeof(double) * n, kind);
+  }
+  // deflation (metalFoamSolver.C:170-190) is decided on the device from the
+  // row sums (deflate 2 and 3: automatic; 3 also projects the inner
+  // right-hand side along u instead of the constant, no reference
+  // counterpart), so the inner operator is prepared before the first read
+  // and one round trip returns the initial residual and the preparation;
+  // a solve converged at start discards the preparation
+  const bool pinnedProj = ctl->deflate == 3;
+  const int mode = useBiCG || ctl->deflate == 0 ? 0 : (ctl->deflate == 1 ? 1 : kDeflateAuto);
+  const int chebOrder = useBiCG ? 1 : ctl->chebyshevOrder;
+  const bool orderOK = chebOrder >= 1 && chebOrder <= kMaxCheb + 1;
+  const bool wantBeta = chebOrder > 1;
+  rc = initial_residual(c, psiCur, src, !orderOK);
   if (rc) return rc;
+  const double tc0 = now_seconds();
+  if (orderOK) {
+    // metalFoamSolver.C:192-249: diag0 = diag - u, sign-flip bail, setCoeffs, Gershgorin
+    rc = prepare_inner(c, mode, wantBeta);
+    if (rc) return rc;
+    rc = read_ostate(c);
+    if (rc) return rc;
+  }
   const OState& o = *c->ho;
   const double normFactor = o.normFactor;
   pf.normFactor = normFactor;
@@ -1161,27 +1254,8 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
   pf.finalResidual = o.initRes;
   if (ctl->minIter > 0 ||
       !of_converged_h(pf.finalResidual, pf.initialResidual, ctl->tolerance, ctl->relTol)) {
-    // deflation (metalFoamSolver.C:170-190)
-    // deflate 3: auto detection, inner right-hand side projected along u
-    // instead of the constant (no reference counterpart)
-    const bool pinnedProj = ctl->deflate == 3;
-    bool deflate = false;
-    if (!useBiCG && ctl->deflate != 0) {
-      if (ctl->deflate == 1)
-        deflate = true;
-      else
-        deflate = (10.0 * o.nPinned <= (double)n);
-    }
-    const int chebOrder = useBiCG ? 1 : ctl->chebyshevOrder;
-    if (chebOrder > kMaxCheb + 1 || chebOrder < 1)
-      return fail(c, HIPFOAM_ERR_ARG, "chebyshevOrder must be in [1, 33]");
-    const bool wantBeta = chebOrder > 1;
-    const double tc0 = now_seconds();
-    // metalFoamSolver.C:192-249: diag0 = diag - u, sign-flip bail, setCoeffs, Gershgorin
-    rc = prepare_inner(c, deflate ? 1 : 0, wantBeta);
-    if (rc) return rc;
-    rc = read_ostate(c);
-    if (rc) return rc;
+    if (!orderOK) return fail(c, HIPFOAM_ERR_ARG, "chebyshevOrder must be in [1, 33]");
+    bool deflate = o.deflate != 0;
     if (deflate && o.bail > 0.0) {
       deflate = false;
       rc = prepare_inner(c, 0, wantBeta);
@@ -1204,16 +1278,15 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
     const dim3 g(row_grid(c)), bl(bs);
     for (outer = 0; outer <= ctl->maxRestarts; ++outer) {
       hipfoam_stats st{};
-      // zero-mean projection of the inner RHS (metalFoamSolver.C:278-287)
+      // zero-mean projection of the inner RHS (metalFoamSolver.C:278-287);
+      // an inner solve in one workgroup stays pending, so the refinement
+      // below queues behind it and one round trip ends the restart
+      InnerPending pending;
       rc = inner_solve(c, useBiCG ? KIND_BICG : KIND_PCG, c->r64.p,
                        deflate ? (pinnedProj ? &c->ost.p->rproj : &c->ost.p->rmean) : nullptr,
                        deflate && pinnedProj ? c->u64.p : nullptr, target, ctl->maxIter - totalIter,
-                       ctl->iterationsPerSync, chebOrder, chebBeta, chebRatio, &st);
+                       ctl->iterationsPerSync, chebOrder, chebBeta, chebRatio, &st, &pending);
       if (rc) return rc;
-      totalIter += st.nIterations;
-      pf.tGpu += st.tGpuSeconds;
-      pf.tInner += st.tWallSeconds;
-      singular = st.singular;
       // psi += delta - mean(delta); r = b - A psi (metalFoamSolver.C:318-340)
       LduOp<double> A = op64(c);
       if (deflate) {
@@ -1222,7 +1295,7 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
         launch_fin<1>(c, g.x, false, 0, FinDeltaMean{c->ost.p, n});
       }
       hipLaunchKernelGGL((k_refine_residual<float>), g, bl, 0, c->stream, A, psiCur,
-                         c->w32.delta.p, psiNxt, c->b.p, c->u64.p, c->r64.p, deflate ? 1 : 0,
+                         c->w32.delta.p, psiNxt, src, c->u64.p, c->r64.p, deflate ? 1 : 0,
                          c->ost.p, c->partials.p, c->pstride);
       launch_fin<3>(c, g.x, false, 0, FinRefine{c->ost.p, n});
       std::swap(psiCur, psiNxt);
@@ -1235,6 +1308,14 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
       CKL();
       rc = read_ostate(c);
       if (rc) return rc;
+      if (pending.on) {
+        rc = inner_finish(c, pending, &st);
+        if (rc) return rc;
+      }
+      totalIter += st.nIterations;
+      pf.tGpu += st.tGpuSeconds;
+      pf.tInner += st.tWallSeconds;
+      singular = st.singular;
       pf.finalResidual = o.finalRes;
       pf.nIterations = totalIter;
       // hybrid polish (metalFoamSolver.C:371-391): exact operator, beta kept
@@ -1267,7 +1348,7 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
       of_converged_h(pf.finalResidual, pf.initialResidual, ctl->tolerance, ctl->relTol);
   const hipMemcpyKind back =
       kind == hipMemcpyHostToDevice ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-  XF(psi, psiCur, sizeof(double) * n, back);
+  if (psiCur != psi) XF(psi, psiCur, sizeof(double) * n, back);
   if (residual) XF(residual, c->r64.p, sizeof(double) * n, back);
   CK(hipStreamSynchronize(c->stream));
   pf.tTotal = now_seconds() - t0;
@@ -1338,12 +1419,9 @@ int fp64_solve(hipfoam_context* c, int solver, const double* diag, const double*
       hipLaunchKernelGGL((k_inner_init<double>), g1, b1, 0, c->stream, n, c->r64.p,
                          (const double*)nullptr, (const double*)nullptr, w.r.p,
                          useBiCG ? w.rA0.p : nullptr, (double*)nullptr, w.p0.p, w.p1.p,
-                         useBiCG ? w.AyA.p : nullptr);
+                         useBiCG ? w.AyA.p : nullptr, state_words(s), c->kst.p, c->kst2.p);
       hipLaunchKernelGGL((k_copy<double>), g1, b1, 0, c->stream, n, c->psi0.p, w.delta.p);
       CKL();
-      *c->hk = s;
-      CK(hipMemcpyAsync(c->kst.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
-      CK(hipMemcpyAsync(c->kst2.p, c->hk, sizeof(KState), hipMemcpyHostToDevice, c->stream));
       return HIPFOAM_OK;
     };
     rc = init();

@@ -68,6 +68,13 @@ struct alignas(128) KState {
   int32_t pend, alphaDue;
 };
 
+// A Krylov state as a kernel argument: its words, without KState's 128-byte
+// alignment.
+struct KStateWords {
+  uint32_t w[sizeof(KState) / sizeof(uint32_t)];
+};
+static_assert(sizeof(KState) % sizeof(uint32_t) == 0, "KState words");
+
 // Outer (FP64 refinement) scalars.
 struct alignas(128) OState {
   double sumPsi, xbar, nPinned, uu;  // rowsum pass
@@ -1271,14 +1278,24 @@ __device__ __forceinline__ void inner_init_row(int i, const double* __restrict__
 // rA = T(r64 - m v) with m = *mean and v = proj (or 1 without proj): the
 // zero-mean projection of the deflated inner right-hand side, or with
 // proj = u = A 1 the projection along the pinned rows; delta = p0 = p1 =
-// AyA = 0; rA0 = rA (BiCGStab).
+// AyA = 0; rA0 = rA (BiCGStab).  Workgroup 0 also writes the solve's start
+// state s into *kst and *kst2 (the merged stages read the first iteration's
+// state from kst2).
 template <typename T>
 __global__ __launch_bounds__(1024) void k_inner_init(int n, const double* __restrict__ r64,
                                                      const double* mean,
                                                      const double* __restrict__ proj,
                                                      T* __restrict__ rA, T* __restrict__ rA0,
                                                      T* __restrict__ delta, T* __restrict__ p0,
-                                                     T* __restrict__ p1, T* __restrict__ AyA) {
+                                                     T* __restrict__ p1, T* __restrict__ AyA,
+                                                     KStateWords s, KState* kst, KState* kst2) {
+  if (blockIdx.x == 0) {
+    constexpr int W = (int)(sizeof(KStateWords) / sizeof(uint32_t));
+    for (int k = threadIdx.x; k < W; k += blockDim.x) {
+      reinterpret_cast<uint32_t*>(kst)[k] = s.w[k];
+      reinterpret_cast<uint32_t*>(kst2)[k] = s.w[k];
+    }
+  }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   inner_init_row<T>(i, r64, mean != nullptr, mean ? *mean : 0.0, proj, rA, rA0, delta, p0, p1,
@@ -1356,6 +1373,16 @@ struct RefineRow {
     v[2] += ri;
   }
 };
+
+// Deflation mode of k_prepare_inner / FinPrepare: 0 off, 1 on, or
+// kDeflateAuto: on when at most a tenth of the rows are pinned
+// (metalFoamSolver.C:170-190), from the row-sum pass's count os->nPinned,
+// which the stream has computed before (FinRowsum).
+constexpr int kDeflateAuto = 2;
+
+__device__ __forceinline__ int deflate_on(int mode, const OState* os, int n) {
+  return mode == kDeflateAuto ? (10.0 * os->nPinned <= (double)n ? 1 : 0) : mode;
+}
 
 // Rows [r0, r1), stride blockDim, of k_prepare_inner: diag* = deflate ? diag
 // - u : diag, diagT = T(diag*), rDT = T(1 / diag*); v[0] += sign flips,
@@ -1452,7 +1479,8 @@ __global__ __launch_bounds__(1024) void k_prepare_inner(LduOp<double> A,
   int r0, r1;
   block_rows(A.n, lb, r0, r1);
   double v[2] = {0.0, -1.0e308};
-  prepare_rows<T>(A, r0, r1, upperF, losort, u, deflate, wantBeta, diagT, rDT, v);
+  prepare_rows<T>(A, r0, r1, upperF, losort, u, deflate_on(deflate, os, A.n), wantBeta, diagT,
+                  rDT, v);
   block_partials<2, true>(v, partials, pstride, lb);
 }
 
@@ -1585,6 +1613,17 @@ template <typename T>
 __global__ __launch_bounds__(1024) void k_copy(int n, const T* __restrict__ s, T* __restrict__ d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = s[i];
+}
+
+// The best-iterate restore of an inner solve (metalFoamBackend.C:718-727) on
+// the device: delta = deltaBest when the best checked residual is below the
+// final one in *st.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_restore_best(int n, const KState* st,
+                                                       T* __restrict__ delta,
+                                                       const T* __restrict__ deltaBest) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && st->bestRes < st->res) delta[i] = deltaBest[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1854,11 +1893,13 @@ struct FinInitialResidual {
 
 struct FinPrepare {
   OState* os;
-  int deflate, wantBeta;
+  int deflate, wantBeta, n;  // deflate: the mode k_prepare_inner was given
   __device__ void operator()(const double* t) const {
+    const int on = deflate_on(deflate, os, n);
+    os->deflate = on;
     os->bail = t[0];
     if (wantBeta) {
-      if (deflate)
+      if (on)
         os->betaDefl = t[1];
       else
         os->betaExact = t[1];

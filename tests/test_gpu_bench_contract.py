@@ -58,9 +58,9 @@ def test_bench_small_meshes():
     r, out = _bench("--workload", "poisson2d100", "--steps", "1", "--warmup", "1",
                     "--fp64-steps", "0", "--host-steps", "0", "--spmv-reps", "2",
                     "--spmv-tune", "0", "--cold-reps", "0", "--no-cpu", "--extras", "1",
-                    "--ref-sizes", "")
+                    "--ref-sizes", "", "--ref-momentum", "0")
     assert r.returncode == 0, r.stderr[-3000:]
-    assert out["reference_sizes"] is None
+    assert out["reference_sizes"] is None and out["reference_momentum"] is None
     small = out["small_meshes"]
     assert set(small) == {"poisson2d100", "convdiff2d100", "poisson24"}
     for tag, e in small.items():
@@ -70,9 +70,10 @@ def test_bench_small_meshes():
 
 
 def test_bench_reference_sizes():
-    """The reference benchmark's cavity sizes: each 2D Poisson converged, its
-    published M3 Pro per-call times carried unchanged, the same-host CPU
-    PCG(DIC) time taken from the CPU leg, and the ratios their quotients."""
+    """The reference benchmark's cavity sizes and its momentum call: each
+    solve converged, the published M3 Pro per-call times carried unchanged,
+    the same-host CPU times taken from the CPU leg, and the ratios their
+    quotients."""
     published = {"20x20": (0.066, 1.109), "50x50": (2.082, 5.838),
                  "100x100": (20.52, 11.63)}
     r, out = _bench("--workload", "poisson2d100", "--steps", "1", "--warmup", "0",
@@ -94,6 +95,18 @@ def test_bench_reference_sizes():
         assert cpu[tag]["cpu_converged"] and e["cpu_dic_ms"] == cpu[tag]["cpu_dic_ms"] > 0
         assert e["speedup_vs_cpu"] == pytest.approx(e["cpu_dic_ms"] / e["host_arrays_ms"],
                                                     abs=0.0051)
+    # the momentum call: Ux and Uy on 40,000 cells, beside the published
+    # 11.16 ms (CPU PBiCGStab, diagonal) and 13.75 ms (metalPBiCGStab)
+    mo = out["reference_momentum"]
+    assert mo["cells"] == 40000 and mo["converged"] and len(mo["iterations"]) == 2
+    assert (mo["reference_cpu_pbicgstab_diagonal_ms"], mo["reference_metalpbicgstab_ms"]) == (
+        11.16, 13.75)
+    assert mo["vs_reference_metalpbicgstab"] == pytest.approx(13.75 / mo["call_ms"], abs=0.051)
+    cm = out["cpu_baseline"]["reference_momentum"]
+    assert all(0 < i < 40 for i in cm["cpu_iterations"])
+    assert mo["cpu_pbicgstab_diagonal_ms"] == cm["cpu_pbicgstab_diagonal_ms"] > 0
+    assert mo["speedup_vs_cpu"] == pytest.approx(mo["cpu_pbicgstab_diagonal_ms"] / mo["call_ms"],
+                                                 abs=0.0051)
 
 
 def test_bench_refuses_several_gpus():
