@@ -541,7 +541,7 @@ def reference_sizes(H, mg, sides, reps=20):
     staging, copies both ways and the solve), set beside the reference's
     published per-call times."""
     ref = {r[0]: r for r in REFERENCE_CAVITY_P}
-    out = {}
+    out, solutions = {}, {}
     ctx = H.Context(0)
     for m, side in zip(reference_size_meshes(mg, sides), sides):
         ctx.set_topology(m)
@@ -560,7 +560,8 @@ def reference_sizes(H, mg, sides, reps=20):
             perf = ctx.solve_device(H.PCG, d, u, None, psi, b, ctl)
             ctx.synchronize()
             ts.append(time.perf_counter() - t0)
-        rsum, _ = numpy_residual(m, psi.download())
+        x_dev = psi.download()
+        rsum, _ = numpy_residual(m, x_dev)
         th = []
         hp = np.zeros(m.n)
         for _ in range(k + 2):
@@ -579,18 +580,23 @@ def reference_sizes(H, mg, sides, reps=20):
             e.update({"reference_cpu_dic_ms": cpu_ms, "reference_metalpcg_ms": gpu_ms,
                       "vs_reference_metalpcg": round(gpu_ms / host, 1)})
         out[f"{side}x{side}"] = e
+        solutions[f"{side}x{side}"] = x_dev
         log(f"reference size {side}x{side}: {dev:.3f} ms device arrays, {host:.3f} ms host "
             f"arrays ({perf.nIterations} it.); reference metalPCG "
             f"{ref[side][3] if side in ref else '-'} ms")
         for a in (d, u, b, psi, zero):
             a.free()
     ctx.close()
-    return out
+    return out, solutions
 
 
-def cpu_reference_sizes(mg, sides):
+def cpu_reference_sizes(mg, sides, solutions=None):
     """The CPU baseline at the reference benchmark's sizes: OpenFOAM PCG(DIC),
-    FP64, 1 core (median of CPU_RUNS runs; one run from 640,000 cells)."""
+    FP64, 1 core (median of CPU_RUNS runs; one run from 640,000 cells).  With
+    the GPU solutions (name -> psi), each entry also states the GPU
+    solution's relMax against the CPU one, beside the CPU-vs-CPU floor:
+    PCG(diagonal) against PCG(DIC), both to 1e-6, as the reference states its
+    accuracy (BENCHMARK.md:23-40)."""
     import oracle
 
     native = _native_oracle(oracle)
@@ -603,10 +609,17 @@ def cpu_reference_sizes(mg, sides):
                 t0 = time.perf_counter()
                 x, perf = oracle.pcg(m, precon=oracle.PRE_DIC, tol=1e-6, maxIter=20000)
                 runs.append(time.perf_counter() - t0)
-            out[f"{side}x{side}"] = {"cpu_dic_ms": round(1e3 * float(np.median(runs)), 4),
-                                     "cpu_iterations": perf.nIterations,
-                                     "cpu_converged": bool(perf.converged),
-                                     "cpu_runs": len(runs)}
+            e = {"cpu_dic_ms": round(1e3 * float(np.median(runs)), 4),
+                 "cpu_iterations": perf.nIterations, "cpu_converged": bool(perf.converged),
+                 "cpu_runs": len(runs)}
+            tag = f"{side}x{side}"
+            if solutions is not None and tag in solutions:
+                e["gpu_vs_cpu_relmax"] = _relmax(solutions[tag], x)
+                if m.n <= 160000:
+                    xd, pd = oracle.pcg(m, precon=oracle.PRE_DIAGONAL, tol=1e-6, maxIter=20000)
+                    if pd.converged:
+                        e["cpu_vs_cpu_floor_relmax"] = _relmax(xd, x)
+            out[tag] = e
         return out
     finally:
         if prev is not None:
@@ -839,7 +852,7 @@ def main():
     sizes = None
     momentum = None
     if args.extras and args.workload in ("poisson128", "poisson2d100") and sides:
-        sizes = reference_sizes(H, mg, sides)
+        sizes, size_solutions = reference_sizes(H, mg, sides)
     if args.extras and args.workload in ("poisson128", "poisson2d100") and args.ref_momentum:
         momentum = reference_momentum(H, mg)
 
@@ -878,9 +891,12 @@ def main():
         log("cpu baseline ...")
         cpu, x_cpu, floor = cpu_baseline(m, solver, args.cpu_iters)
         if sizes is not None:
-            cpu["reference_sizes"] = cpu_reference_sizes(mg, sides)
+            cpu["reference_sizes"] = cpu_reference_sizes(mg, sides, size_solutions)
             for tag, c in cpu["reference_sizes"].items():
                 sizes[tag]["cpu_dic_ms"] = c["cpu_dic_ms"]
+                for k in ("gpu_vs_cpu_relmax", "cpu_vs_cpu_floor_relmax"):
+                    if k in c:
+                        sizes[tag][k] = c[k]
                 sizes[tag]["speedup_vs_cpu"] = round(
                     c["cpu_dic_ms"] / sizes[tag]["host_arrays_ms"], 2)
             log("reference sizes, CPU PCG(DIC) 1 core: " +

@@ -436,13 +436,21 @@ int32_t hipfoam_set_schedule(hipfoam_context* ctx, const hipfoam_schedule* s);
  * persistent), how many of those were abandoned part-way and run again on
  * the launched loop, and why the last one was: 0x10000 | workgroup << 4 |
  * XCD id when that workgroup found a participant on another XCD (the id),
- * 0x20000 | barrier index for a barrier wait that timed out (0: none
- * abandoned).  Any pointer may be NULL.  No reference counterpart. */
+ * 0x20000 | barrier index for a barrier wait that timed out, 0x40000 for
+ * one reported by hipfoam_simulate_abandon (0: none abandoned).  Any
+ * pointer may be NULL.  No reference counterpart. */
 int32_t hipfoam_persistent_stats(const hipfoam_context* ctx, int64_t* launches,
                                  int64_t* abandoned, int64_t* lastAbort);
 /* Inner loops this context ran in one workgroup (schedule field localLoop).
  * No reference counterpart. */
 int32_t hipfoam_local_stats(const hipfoam_context* ctx, int64_t* launches);
+/* For tests of the recovery from an abandoned persistent launch: the next
+ * count persistent launches of this context run to their end and are then
+ * reported as abandoned -- their Krylov state is set back to running and
+ * their abort word to 0x40000, as a launch that gave up leaves them -- so the
+ * driver runs the inner solve again (and whatever it had queued behind the
+ * launch does not run).  count 0 cancels.  No reference counterpart. */
+int32_t hipfoam_simulate_abandon(hipfoam_context* ctx, int32_t count);
 /* Times the candidate SpMV schedules on the staged operator and keeps the
  * fastest; *best (may be NULL) receives it. */
 int32_t hipfoam_autotune(hipfoam_context* ctx, hipfoam_schedule* best);

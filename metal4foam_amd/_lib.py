@@ -140,6 +140,7 @@ SIGNATURES = {
                                        C.POINTER(C.c_int64)]),
     "hipfoam_set_schedule": (C.c_int32, [_P, C.POINTER(Schedule)]),
     "hipfoam_local_stats": (C.c_int32, [_P, C.POINTER(C.c_int64)]),
+    "hipfoam_simulate_abandon": (C.c_int32, [_P, C.c_int32]),
     "hipfoam_persistent_stats": (C.c_int32, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                              C.POINTER(C.c_int64)]),
     "hipfoam_autotune": (C.c_int32, [_P, C.POINTER(Schedule)]),

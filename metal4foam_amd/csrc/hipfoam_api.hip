@@ -199,6 +199,7 @@ struct hipfoam_context {
   int64_t persistAbandoned = 0;  // ... of which abandoned (the launched loop ran instead)
   int64_t persistLastAbort = 0;  // abort word of the last abandoned launch (XcdSync)
   int64_t localLaunches = 0;     // inner loops run in one workgroup (k_pcg_local)
+  int simAbandon = 0;            // persistent launches still to report as abandoned
   hipEvent_t evPoll[2] = {nullptr, nullptr};
   hipEvent_t evT0 = nullptr, evT1 = nullptr;
 
@@ -321,6 +322,14 @@ void launch_fin(hipfoam_context* c, int nblk, bool checkStatus, int runMask, F f
 // kernel just enqueued (nblk workgroups) and applies the finaliser f.  With
 // checkStatus, it runs only when the Krylov status bit is set in runMask.
 constexpr int kRunning = 1 << ST_RUNNING;
+
+// The same for a refinement stage: skipped while *guard (NULL: always runs)
+// is RUNNING, posting no status.
+template <int NV, typename F>
+void launch_fin_guarded(hipfoam_context* c, int nblk, const int32_t* guard, F f) {
+  hipLaunchKernelGGL((k_finalize<NV, false, F>), dim3(1), dim3(256), 0, c->stream,
+                     c->partials.p, c->pstride, nblk, guard, ~kRunning, (int32_t*)nullptr, f);
+}
 constexpr int kRunningOrEarly = (1 << ST_RUNNING) | (1 << ST_EARLY);
 
 template <int NV, bool LASTMAX = false, typename F>
@@ -880,14 +889,12 @@ int local_loop(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order, in
   return local_finish(c, gpuSeconds);
 }
 
-// One persistent launch of the whole iteration loop; vectors and the state
-// must be initialised (and the BiCGStab start enqueued).  *abandoned: the
-// participants were not on one XCD or a barrier wait timed out, so the launch
-// gave up part-way; the barrier words are reset and the caller starts the
-// solve again on the launched loop.
+// One persistent launch of the whole iteration loop, enqueued between the
+// timing events evT0 and evT1; vectors and the state must be initialised
+// (and the BiCGStab start enqueued).
 template <typename T>
-int persistent_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
-                    int maxIter, double* gpuSeconds, bool* abandoned) {
+int persistent_enqueue(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
+                       int maxIter) {
   Work<T>& w = work<T>(c);
   const int ntiles = (c->n + kSlotTile - 1) / kSlotTile;
   const int K = std::min(ntiles, kPersistMaxGroups);
@@ -909,17 +916,28 @@ int persistent_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* r
     hipLaunchKernelGGL((k_bicg_persistent<T>), grid, block, 0, c->stream, A, rD, V, c->kst.p,
                        c->kst2.p, c->partials.p, c->partials2.p, c->pstride, K, loops, S);
   }
+  if (c->simAbandon > 0) {
+    c->simAbandon -= 1;
+    hipLaunchKernelGGL(k_simulate_abandon, dim3(1), dim3(64), 0, c->stream, c->kst.p, S.abortw);
+  }
   CKL();
   CK(hipEventRecord(c->evT1, c->stream));
-  CK(hipEventSynchronize(c->evT1));
+  return HIPFOAM_OK;
+}
+
+// After evT1 has completed and the state has been copied to c->hk: the GPU
+// time of the launch, and whether it ended the loop.  A state left running
+// means the launch was abandoned (*abandoned: the barrier words are reset
+// and the caller starts the solve again on the launched loop) or ran out of
+// iterations (an error).
+int persistent_finish(hipfoam_context* c, double* gpuSeconds, bool* abandoned) {
   float ms = 0.f;
   CK(hipEventElapsedTime(&ms, c->evT0, c->evT1));
   if (gpuSeconds) *gpuSeconds += 1e-3 * ms;
-  CK(hipMemcpy(c->hk, c->kst.p, sizeof(KState), hipMemcpyDeviceToHost));
   if (c->hk->status != ST_RUNNING) return HIPFOAM_OK;
-  // the state was left running: abandoned, or out of iterations
+  uint32_t* xs = c->xsync.p;
   int32_t* ab = &c->hstatus[2];
-  CK(hipMemcpy(ab, S.abortw, sizeof(int32_t), hipMemcpyDeviceToHost));
+  CK(hipMemcpy(ab, xs + 2 * kPersistMaxGroups, sizeof(int32_t), hipMemcpyDeviceToHost));
   CK(hipMemsetAsync(xs, 0, sizeof(uint32_t) * kXcdSyncWords, c->stream));
   CK(hipStreamSynchronize(c->stream));
   if (*ab != 0) {
@@ -929,6 +947,17 @@ int persistent_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* r
     return HIPFOAM_OK;
   }
   return fail(c, HIPFOAM_ERR_STATE, "inner loop did not terminate within maxIter iterations");
+}
+
+// The persistent launch, waited for.
+template <typename T>
+int persistent_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
+                    int maxIter, double* gpuSeconds, bool* abandoned) {
+  int rc = persistent_enqueue<T>(c, kind, A, rD, order, maxIter);
+  if (rc) return rc;
+  CK(hipEventSynchronize(c->evT1));
+  CK(hipMemcpy(c->hk, c->kst.p, sizeof(KState), hipMemcpyDeviceToHost));
+  return persistent_finish(c, gpuSeconds, abandoned);
 }
 
 // The iteration loop of an initialised solve (BiCGStab: its start, then the
@@ -959,20 +988,23 @@ int iterate(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int o
 }
 
 // An inner solve left running on the stream (inner_solve with a pending
-// record): its loop, the best-iterate restore and the copy of its final
-// state are enqueued; inner_finish reads its statistics once the caller has
-// synchronised the stream.
+// record): its loop (one workgroup, or persistent on one XCD), the
+// best-iterate restore and the copy of its final state are enqueued;
+// inner_finish reads its statistics once the caller has synchronised the
+// stream.
 struct InnerPending {
   bool on = false;
+  bool persist = false;  // the loop is a persistent launch, which may be abandoned
   double tw0 = 0.0;
 };
 
 // Inner correction solve in reference-backend semantics
 // (metalFoamBackend::pcg / ::bicgstab): r64 -> FP32 residual (minus *mean,
 // times proj when given, when mean is given), delta0 = 0, device loop,
-// best-iterate restore.  With pending (not NULL), a loop that runs in one
-// workgroup is only enqueued, together with the restore (on the device) and
-// the state copy: the caller queues its next steps behind it and calls
+// best-iterate restore.  With pending (not NULL), a loop that runs as one
+// launch (one workgroup, or persistent on one XCD) is only enqueued,
+// together with the restore (on the device) and the state copy: the caller
+// queues its next steps behind it, guarded by the state's status, and calls
 // inner_finish after its next synchronisation.
 int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* mean,
                 const double* proj, double target, int maxIter, int ips, int order,
@@ -1015,14 +1047,23 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
   if (s.status == ST_RUNNING) {
     const LduOp<float> A = op32(c);
     const int ord = kind == KIND_PCG ? order : 1;
-    if (pending && local_ok<float>(c, kind, ord, A)) {
-      int rc = local_enqueue<float>(c, A, c->rD32.p, ord, maxIter);
+    const bool local = pending && local_ok<float>(c, kind, ord, A);
+    const bool persist = pending && !local && persistent_ok<float>(c, kind, ord);
+    if (local || persist) {
+      int rc = HIPFOAM_OK;
+      if (local) {
+        rc = local_enqueue<float>(c, A, c->rD32.p, ord, maxIter);
+      } else {
+        if (kind == KIND_BICG) rc = enqueue_bicg_start<float>(c, A);
+        if (!rc) rc = persistent_enqueue<float>(c, kind, A, c->rD32.p, ord, maxIter);
+      }
       if (rc) return rc;
       hipLaunchKernelGGL((k_restore_best<float>), dim3(blocks_for(n, 256)), dim3(256), 0,
                          c->stream, n, c->kst.p, w.delta.p, w.deltaBest.p);
       CKL();
       CK(hipMemcpyAsync(c->hk, c->kst.p, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
       pending->on = true;
+      pending->persist = persist;
       pending->tw0 = tw0;
       return HIPFOAM_OK;
     }
@@ -1052,13 +1093,16 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
 
 // Statistics of a pending inner solve (its loop ran from RUNNING; the state
 // is in c->hk and the restore has run), once the stream has been
-// synchronised: the same as inner_solve reports.
-int inner_finish(hipfoam_context* c, InnerPending& pending, hipfoam_stats* stats) {
+// synchronised: the same as inner_solve reports.  *redo: the persistent
+// launch was abandoned, so nothing queued behind it ran (its guards saw the
+// state still RUNNING); the caller solves again with inner_solve.
+int inner_finish(hipfoam_context* c, InnerPending& pending, hipfoam_stats* stats, bool* redo) {
   hipfoam_stats st{};
   double gpu = 0;
   pending.on = false;
-  int rc = local_finish(c, &gpu);
-  if (rc) return rc;
+  *redo = false;
+  int rc = pending.persist ? persistent_finish(c, &gpu, redo) : local_finish(c, &gpu);
+  if (rc || *redo) return rc;
   const KState& h = *c->hk;
   st.nIterations = h.it;
   st.finalResSum = h.res;
@@ -1279,39 +1323,56 @@ int refined_solve(hipfoam_context* c, int solver, const double* diag, const doub
     for (outer = 0; outer <= ctl->maxRestarts; ++outer) {
       hipfoam_stats st{};
       // zero-mean projection of the inner RHS (metalFoamSolver.C:278-287);
-      // an inner solve in one workgroup stays pending, so the refinement
-      // below queues behind it and one round trip ends the restart
+      // an inner solve that runs as one launch stays pending, so the
+      // refinement below queues behind it (guarded by its status) and one
+      // round trip ends the restart
+      const double* mean =
+          deflate ? (pinnedProj ? &c->ost.p->rproj : &c->ost.p->rmean) : nullptr;
+      const double* proj = deflate && pinnedProj ? c->u64.p : nullptr;
+      const int innerMax = ctl->maxIter - totalIter;
+      // psi += delta - mean(delta); r = b - A psi (metalFoamSolver.C:318-340),
+      // the new psi in psiNxt; the constant-mode correction
+      // (metalFoamSolver.C:342-361)
+      auto refine = [&](const int32_t* guard) -> int {
+        LduOp<double> A = op64(c);
+        if (deflate) {
+          hipLaunchKernelGGL((k_delta_mean<float>), g, bl, 0, c->stream, n, A.xcdRemap,
+                             c->w32.delta.p, c->ost.p, c->partials.p, c->pstride, guard);
+          launch_fin_guarded<1>(c, g.x, guard, FinDeltaMean{c->ost.p, n});
+        }
+        hipLaunchKernelGGL((k_refine_residual<float>), g, bl, 0, c->stream, A, psiCur,
+                           c->w32.delta.p, psiNxt, src, c->u64.p, c->r64.p, deflate ? 1 : 0,
+                           c->ost.p, c->partials.p, c->pstride, guard);
+        launch_fin_guarded<3>(c, g.x, guard, FinRefine{c->ost.p, n});
+        if (deflate && uu > kVSmall) {
+          hipLaunchKernelGGL(k_const_shift, g, bl, 0, c->stream, n, A.xcdRemap, psiNxt,
+                             c->r64.p, c->u64.p, c->ost.p, c->partials.p, c->pstride, guard);
+          launch_fin_guarded<2>(c, g.x, guard, FinShift{c->ost.p, n});
+        }
+        CKL();
+        return read_ostate(c);
+      };
       InnerPending pending;
-      rc = inner_solve(c, useBiCG ? KIND_BICG : KIND_PCG, c->r64.p,
-                       deflate ? (pinnedProj ? &c->ost.p->rproj : &c->ost.p->rmean) : nullptr,
-                       deflate && pinnedProj ? c->u64.p : nullptr, target, ctl->maxIter - totalIter,
-                       ctl->iterationsPerSync, chebOrder, chebBeta, chebRatio, &st, &pending);
+      rc = inner_solve(c, useBiCG ? KIND_BICG : KIND_PCG, c->r64.p, mean, proj, target,
+                       innerMax, ctl->iterationsPerSync, chebOrder, chebBeta, chebRatio, &st,
+                       &pending);
       if (rc) return rc;
-      // psi += delta - mean(delta); r = b - A psi (metalFoamSolver.C:318-340)
-      LduOp<double> A = op64(c);
-      if (deflate) {
-        hipLaunchKernelGGL((k_delta_mean<float>), g, bl, 0, c->stream, n, A.xcdRemap,
-                           c->w32.delta.p, c->ost.p, c->partials.p, c->pstride);
-        launch_fin<1>(c, g.x, false, 0, FinDeltaMean{c->ost.p, n});
-      }
-      hipLaunchKernelGGL((k_refine_residual<float>), g, bl, 0, c->stream, A, psiCur,
-                         c->w32.delta.p, psiNxt, src, c->u64.p, c->r64.p, deflate ? 1 : 0,
-                         c->ost.p, c->partials.p, c->pstride);
-      launch_fin<3>(c, g.x, false, 0, FinRefine{c->ost.p, n});
-      std::swap(psiCur, psiNxt);
-      // constant-mode correction (metalFoamSolver.C:342-361)
-      if (deflate && uu > kVSmall) {
-        hipLaunchKernelGGL(k_const_shift, g, bl, 0, c->stream, n, A.xcdRemap, psiCur, c->r64.p,
-                           c->u64.p, c->ost.p, c->partials.p, c->pstride);
-        launch_fin<2>(c, g.x, false, 0, FinShift{c->ost.p, n});
-      }
-      CKL();
-      rc = read_ostate(c);
+      rc = refine(pending.on ? &c->kst.p->status : nullptr);
       if (rc) return rc;
       if (pending.on) {
-        rc = inner_finish(c, pending, &st);
+        bool redo = false;
+        rc = inner_finish(c, pending, &st, &redo);
         if (rc) return rc;
+        if (redo) {
+          rc = inner_solve(c, useBiCG ? KIND_BICG : KIND_PCG, c->r64.p, mean, proj, target,
+                           innerMax, ctl->iterationsPerSync, chebOrder, chebBeta, chebRatio,
+                           &st);
+          if (rc) return rc;
+          rc = refine(nullptr);
+          if (rc) return rc;
+        }
       }
+      std::swap(psiCur, psiNxt);
       totalIter += st.nIterations;
       pf.tGpu += st.tGpuSeconds;
       pf.tInner += st.tWallSeconds;
@@ -2468,6 +2529,12 @@ int32_t hipfoam_persistent_stats(const hipfoam_context* c, int64_t* launches,
   if (launches) *launches = c->persistLaunches;
   if (abandoned) *abandoned = c->persistAbandoned;
   if (lastAbort) *lastAbort = c->persistLastAbort;
+  return HIPFOAM_OK;
+}
+
+int32_t hipfoam_simulate_abandon(hipfoam_context* c, int32_t count) {
+  if (!c || count < 0) return HIPFOAM_ERR_ARG;
+  c->simAbandon = count;
   return HIPFOAM_OK;
 }
 

@@ -1503,11 +1503,19 @@ __global__ __launch_bounds__(1024) void k_stage_faces(int nf, const int32_t* __r
   if (lower32) lower32[k] = (float)lo[k];
 }
 
+// The refinement kernels below (k_delta_mean, k_refine_residual,
+// k_const_shift) return at entry when *guard -- the Krylov status of the
+// inner solve they follow, NULL for none -- is still RUNNING: queued behind an
+// inner loop that gave up part-way, they write nothing, and the host runs the
+// inner solve and them again.
+
 // sum(double(delta)) -> dmean
 template <typename T>
 __global__ __launch_bounds__(1024) void k_delta_mean(int n, int xcdRemap,
                                                      const T* __restrict__ delta, OState* os,
-                                                     double* partials, int pstride) {
+                                                     double* partials, int pstride,
+                                                     const int32_t* guard) {
+  if (guard && *guard == ST_RUNNING) return;
   const int lb = logical_block(xcdRemap);
   int r0, r1;
   block_rows(n, lb, r0, r1);
@@ -1522,7 +1530,9 @@ template <typename T>
 __global__ __launch_bounds__(1024) void k_refine_residual(
     LduOp<double> A, const double* __restrict__ psiOld, const T* __restrict__ delta,
     double* __restrict__ psiNew, const double* __restrict__ b, const double* __restrict__ u,
-    double* __restrict__ r, int useMean, OState* os, double* partials, int pstride) {
+    double* __restrict__ r, int useMean, OState* os, double* partials, int pstride,
+    const int32_t* guard) {
+  if (guard && *guard == ST_RUNNING) return;
   const int lb = logical_block(A.xcdRemap);
   int r0, r1;
   block_rows(A.n, lb, r0, r1);
@@ -1536,7 +1546,9 @@ __global__ __launch_bounds__(1024) void k_refine_residual(
 __global__ __launch_bounds__(1024) void k_const_shift(int n, int xcdRemap, double* __restrict__ psi,
                                                       double* __restrict__ r,
                                                       const double* __restrict__ u, OState* os,
-                                                      double* partials, int pstride) {
+                                                      double* partials, int pstride,
+                                                      const int32_t* guard) {
+  if (guard && *guard == ST_RUNNING) return;
   const int lb = logical_block(xcdRemap);
   int r0, r1;
   block_rows(n, lb, r0, r1);
@@ -1616,14 +1628,14 @@ __global__ __launch_bounds__(1024) void k_copy(int n, const T* __restrict__ s, T
 }
 
 // The best-iterate restore of an inner solve (metalFoamBackend.C:718-727) on
-// the device: delta = deltaBest when the best checked residual is below the
-// final one in *st.
+// the device: delta = deltaBest when the loop ended (status not RUNNING) and
+// its best checked residual is below the final one in *st.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_restore_best(int n, const KState* st,
                                                        T* __restrict__ delta,
                                                        const T* __restrict__ deltaBest) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && st->bestRes < st->res) delta[i] = deltaBest[i];
+  if (i < n && st->status != ST_RUNNING && st->bestRes < st->res) delta[i] = deltaBest[i];
 }
 
 // ---------------------------------------------------------------------------

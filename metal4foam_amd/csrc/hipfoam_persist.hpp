@@ -155,6 +155,18 @@ __device__ __forceinline__ void persistent_end(const XcdSync& S, int wg, const K
     __hip_atomic_store(S.epoch, tgt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// What an abandoned launch leaves behind, written after a launch that ran to
+// its end (hipfoam_simulate_abandon, for tests of the recovery): the state
+// back at RUNNING and the abort word set to kAbortSimulated.
+constexpr uint32_t kAbortSimulated = 0x40000u;
+
+__global__ void k_simulate_abandon(KState* kst, uint32_t* abortw) {
+  if (threadIdx.x == 0) {
+    kst->status = ST_RUNNING;
+    *abortw = kAbortSimulated;
+  }
+}
+
 // The vectors of one inner PCG solve (the context's Work<T>).
 template <typename T>
 struct PcgVectors {

@@ -344,6 +344,11 @@ class Context:
                                                       C.byref(w)), "persistent_stats")
         return a.value, b.value, w.value
 
+    def simulate_abandon(self, count: int = 1):
+        """Report the next `count` persistent launches as abandoned after they
+        ran (hipfoam_simulate_abandon): exercises the driver's recovery."""
+        self._check(self.lib.hipfoam_simulate_abandon(self.h, int(count)), "simulate_abandon")
+
     def local_launches(self) -> int:
         """Inner loops run in one workgroup (schedule field localLoop) since
         the context was created."""

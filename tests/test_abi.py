@@ -81,6 +81,9 @@ def test_schedule_validation_without_device():
     assert ctx.local_launches() == 0
     assert ctx.lib.hipfoam_local_stats(ctx.h, None) == _lib.ERR_ARG
     assert ctx.lib.hipfoam_local_stats(None, C.byref(C.c_int64())) == _lib.ERR_ARG
+    assert ctx.lib.hipfoam_simulate_abandon(None, 1) == _lib.ERR_ARG
+    assert ctx.lib.hipfoam_simulate_abandon(ctx.h, -1) == _lib.ERR_ARG
+    assert ctx.lib.hipfoam_simulate_abandon(ctx.h, 0) == _lib.OK
     ok = _lib.Schedule(1, 512, 0, 16, 1024)
     assert ctx.lib.hipfoam_set_schedule(ctx.h, C.byref(ok)) == _lib.OK
     assert ctx.get_schedule().as_dict() == ok.as_dict()

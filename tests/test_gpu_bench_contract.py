@@ -93,6 +93,8 @@ def test_bench_reference_sizes():
         assert e["vs_reference_metalpcg"] == pytest.approx(gpu_ref / e["host_arrays_ms"],
                                                            abs=0.051)
         assert cpu[tag]["cpu_converged"] and e["cpu_dic_ms"] == cpu[tag]["cpu_dic_ms"] > 0
+        # the reference's accuracy measure beside its CPU-vs-CPU floor
+        assert 0 < e["gpu_vs_cpu_relmax"] < 1e-4 and 0 <= e["cpu_vs_cpu_floor_relmax"] < 1e-4
         assert e["speedup_vs_cpu"] == pytest.approx(e["cpu_dic_ms"] / e["host_arrays_ms"],
                                                     abs=0.0051)
     # the momentum call: Ux and Uy on 40,000 cells, beside the published

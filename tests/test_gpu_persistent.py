@@ -255,3 +255,44 @@ def test_persistent_against_oracle_c0(ctx, orc):
     assert perf.converged and true_res < 1e-6
     assert abs(perf.nIterations - ref.nIterations) <= max(4, ref.nIterations // 10)
     assert np.abs(psi - ref_psi).max() <= 1e-5 * np.abs(ref_psi).max()
+
+
+RECOVERY = [("pcg", 1, False), ("pcg", 2, False), ("pcg", 3, False), ("bicgstab", 1, False),
+            ("bicgstab", 2, False), ("pcg", 1, True), ("bicgstab", 1, True)]
+
+
+@pytest.mark.parametrize("kind,count,fp64", RECOVERY,
+                         ids=[f"{k}-{c}{'-fp64' if f else ''}" for k, c, f in RECOVERY])
+def test_persistent_abandon_recovery(ctx, kind, count, fp64):
+    """Persistent launches reported abandoned (hipfoam_simulate_abandon: each
+    runs to its end, then its state is set back to running and its abort word
+    set, as a launch that gave up leaves them).  The refinement queued behind
+    the launch writes nothing, the driver solves again -- persistent, then on
+    the launched loop when that launch is abandoned too (count 2), and again
+    at the next restart (count 3) -- and every solve is bit-identical to the
+    undisturbed one."""
+    m = mg.config0() if kind == "pcg" else mg.convection_diffusion(100, 100, 1, seed=43)
+    solver = H.PCG if kind == "pcg" else H.PBICGSTAB
+    ctl = dict(maxRestarts=6) if not fp64 else dict(maxIter=2000)
+    print(f"[persistent] recovery {kind} x{count} {'fp64' if fp64 else 'refined'}", flush=True)
+    ctx.set_schedule(persistent=-1)
+    ctx.set_topology(m)
+    out = []
+    try:
+        for sim in (0, count):
+            ctx.simulate_abandon(sim)
+            before = ctx.persistent_abort()
+            psi = np.zeros(m.n)
+            perf, res = ctx.solve(solver, m.diag, m.upper, m.lower, psi, m.source,
+                                  H.default_controls(**ctl), residual=True, fp64=fp64)
+            after = ctx.persistent_abort()
+            out.append((psi, res, perf, after[1] - before[1], after[2]))
+    finally:
+        ctx.simulate_abandon(0)
+    (psi0, res0, perf0, ab0, _), (psi1, res1, perf1, ab1, code) = out
+    assert ab0 == 0
+    assert ab1 == count and code == 0x40000, (ab1, hex(code))
+    _same_perf(perf0, perf1)
+    np.testing.assert_array_equal(psi1, psi0)
+    np.testing.assert_array_equal(res1, res0)
+    assert perf1.converged
