@@ -397,11 +397,12 @@ typedef struct
                              never.  The
                              same rows, partials and fold order as the
                              launched loop: identical results */
-    int32_t localLoop;    /* Chebyshev PCG inner loop in one workgroup of
-                             512 threads with every vector in LDS, on a
-                             mesh of at most 2,048 rows (FP32 inner loop)
-                             or 1,536 (plain FP64) whose tiles are all
-                             row-slot tiles, on the per-tile launch shape:
+    int32_t localLoop;    /* inner loop in one workgroup of 512 threads
+                             with every vector in LDS, on a mesh whose
+                             tiles are all row-slot tiles, on the per-tile
+                             launch shape: Chebyshev PCG up to 2,048 rows
+                             (FP32 inner loop) or 1,536 (plain FP64),
+                             BiCGStab up to 1,280 rows:
                              -1 = auto (default), 0 = never.  Taken before
                              persistent where both apply; the same rows,
                              partials and fold order as the launched loop:

@@ -837,33 +837,49 @@ inline bool persistent_ok(const hipfoam_context* c, Kind kind, int order) {
   return ntiles <= (c->sched.persistent > 0 ? kAutoSmallTiles : autoTiles);
 }
 
-// Whether the inner loop runs in one workgroup (k_pcg_local): Chebyshev PCG
-// on the per-tile launch shape, whose rows, partials and fold order it
-// reproduces, on a mesh of at most local_max_rows<T>() rows whose tiles are
+// Whether the inner loop runs in one workgroup (k_pcg_local, k_bicg_local):
+// Chebyshev PCG or BiCGStab on the per-tile launch shape, whose rows,
+// partials and fold order it reproduces, on a mesh of at most
+// local_max_rows<T>() (PCG) or local_bicg_max_rows<T>() rows whose tiles are
 // all row-slot tiles (schedule.localLoop -1, the default); never under kernel
 // timing.
 template <typename T>
 inline bool local_ok(const hipfoam_context* c, Kind kind, int order, const LduOp<T>& A) {
-  if (c->sched.localLoop == 0 || c->timer.on || kind != KIND_PCG || order < 2) return false;
-  if (!A.slots || !c->slotOK || c->slotFaceRows != 0 || c->n > local_max_rows<T>()) return false;
+  if (c->sched.localLoop == 0 || c->timer.on || (kind == KIND_PCG && order < 2)) return false;
+  const int maxRows = kind == KIND_PCG ? local_max_rows<T>() : local_bicg_max_rows<T>();
+  if (!A.slots || !c->slotOK || c->slotFaceRows != 0 || c->n > maxRows) return false;
   const int ntiles = (c->n + kSlotTile - 1) / kSlotTile;
   return row_block(c) == kSlotTile && row_grid(c) == ntiles && merge_stages(c);
 }
 
-// The whole iteration loop in one workgroup, enqueued between the timing
-// events evT0 and evT1; vectors and the state must be initialised.  The
-// state leaves RUNNING within maxIter + 4 iterations.
+// The whole iteration loop in one workgroup (BiCGStab: its start, then the
+// loop), enqueued between the timing events evT0 and evT1; vectors and the
+// state must be initialised.  The state leaves RUNNING within maxIter + 4
+// iterations.
 template <typename T>
-int local_enqueue(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order, int maxIter) {
+int local_enqueue(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
+                  int maxIter) {
   Work<T>& w = work<T>(c);
   const int loops = std::max(maxIter, 1) + 4;
-  const PcgVectors<T> V{w.r.p,     w.r1.p,        w.p0.p, w.p1.p, w.wA.p, w.Ap.p,
-                        w.delta.p, w.deltaBest.p, w.z0.p, w.z1.p, w.w.p};
+  if (kind == KIND_BICG) {
+    int rc = enqueue_bicg_start<T>(c, A);
+    if (rc) return rc;
+  }
   c->localLaunches += 1;
   CK(hipEventRecord(c->evT0, c->stream));
-  hipLaunchKernelGGL((k_pcg_local<T>), dim3(1), dim3(kLocalThreads), 0, c->stream, A, rD, V,
-                     c->kst.p, c->kst2.p, c->partials.p, c->partials2.p, c->pstride, order,
-                     loops);
+  if (kind == KIND_PCG) {
+    const PcgVectors<T> V{w.r.p,     w.r1.p,        w.p0.p, w.p1.p, w.wA.p, w.Ap.p,
+                          w.delta.p, w.deltaBest.p, w.z0.p, w.z1.p, w.w.p};
+    hipLaunchKernelGGL((k_pcg_local<T>), dim3(1), dim3(kLocalThreads), 0, c->stream, A, rD, V,
+                       c->kst.p, c->kst2.p, c->partials.p, c->partials2.p, c->pstride, order,
+                       loops);
+  } else {
+    const BicgVectors<T> V{w.r.p,  w.p0.p, w.p1.p, w.AyA.p, w.AyA1.p,  w.yA.p,
+                           w.rA0.p, w.sA.p, w.zA.p, w.tA.p,  w.delta.p, w.deltaBest.p};
+    hipLaunchKernelGGL((k_bicg_local<T>), dim3(1), dim3(kLocalThreads), 0, c->stream, A, rD,
+                       V, c->kst.p, c->kst2.p, c->partials.p, c->partials2.p, c->pstride,
+                       loops);
+  }
   CKL();
   CK(hipEventRecord(c->evT1, c->stream));
   return HIPFOAM_OK;
@@ -880,9 +896,9 @@ int local_finish(hipfoam_context* c, double* gpuSeconds) {
 }
 
 template <typename T>
-int local_loop(hipfoam_context* c, const LduOp<T>& A, const T* rD, int order, int maxIter,
-               double* gpuSeconds) {
-  int rc = local_enqueue<T>(c, A, rD, order, maxIter);
+int local_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
+               int maxIter, double* gpuSeconds) {
+  int rc = local_enqueue<T>(c, kind, A, rD, order, maxIter);
   if (rc) return rc;
   CK(hipEventSynchronize(c->evT1));
   CK(hipMemcpy(c->hk, c->kst.p, sizeof(KState), hipMemcpyDeviceToHost));
@@ -968,7 +984,8 @@ int persistent_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* r
 template <typename T, typename Reinit>
 int iterate(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
             int maxIter, double* gpuSeconds, Reinit reinit) {
-  if (local_ok<T>(c, kind, order, A)) return local_loop<T>(c, A, rD, order, maxIter, gpuSeconds);
+  if (local_ok<T>(c, kind, order, A))
+    return local_loop<T>(c, kind, A, rD, order, maxIter, gpuSeconds);
   if (persistent_ok<T>(c, kind, order)) {
     if (kind == KIND_BICG) {
       int rc = enqueue_bicg_start<T>(c, A);
@@ -1052,7 +1069,7 @@ int inner_solve(hipfoam_context* c, Kind kind, const double* r64, const double* 
     if (local || persist) {
       int rc = HIPFOAM_OK;
       if (local) {
-        rc = local_enqueue<float>(c, A, c->rD32.p, ord, maxIter);
+        rc = local_enqueue<float>(c, kind, A, c->rD32.p, ord, maxIter);
       } else {
         if (kind == KIND_BICG) rc = enqueue_bicg_start<float>(c, A);
         if (!rc) rc = persistent_enqueue<float>(c, kind, A, c->rD32.p, ord, maxIter);

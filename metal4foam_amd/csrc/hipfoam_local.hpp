@@ -295,3 +295,170 @@ __global__ __launch_bounds__(kLocalThreads) void k_pcg_local(LduOp<T> A,
 }
 
 }  // namespace hipfoam
+
+namespace hipfoam {
+
+// Rows of the one-workgroup BiCGStab loop: 1,280 in either precision
+// (thirteen vectors, the twelve of BicgVectors and rD: 65 KiB FP32, 130 KiB
+// FP64).  A one-XCD BiCGStab iteration is only three phases (9.1-9.4 us at
+// 400 to 2,048 rows), so one CU stops paying earlier than for PCG: measured
+// against it (refined solve, 2D convection-diffusion, same box), 4.2 against
+// 9.1 us per iteration at 400 rows and 7.2 against 9.4 at 1,280; plain FP64
+// 4.5 against 9.8 and 8.0 against 10.1; a version holding 2,048 FP32 rows
+// took 11.2 against 9.3 us at 2,025 and 2,048 rows.
+constexpr int kLocalBicgVectors = 13;
+template <typename T>
+constexpr int local_bicg_max_rows() {
+  return 1280;
+}
+template <typename T>
+constexpr int local_bicg_passes() {
+  return (local_bicg_max_rows<T>() + kLocalThreads - 1) / kLocalThreads;
+}
+static_assert(kLocalBicgVectors * 1280 * sizeof(double) <= 144 * 1024,
+              "BiCGStab vectors within 144 KiB of LDS");
+
+// The BiCGStab loop of k_bicg_persistent after its start (k_bicg_rho and
+// FinBicgRho, launched before this kernel) in one workgroup of
+// kLocalThreads threads, for a mesh of at most local_bicg_max_rows<T>() rows
+// in row-slot tiles only: k_bicg_py_spmv, k_bicg_sz_spmv with FinBicgAlpha,
+// k_bicg_update_rho with FinBicgSOmega, and FinBicgUpdateRho at the start of
+// the next iteration; p and AyA ping-pong on the iteration parity.  Wave 0's
+// lanes mirror the partial buffers P (two values a tile) and P2 (three), so
+// every fold sees what k_bicg_persistent's folds see.  The vectors, the
+// partials and the state *kst are loaded at the start; the vectors and
+// partials are stored back at the end and the state into *kst and *kst2.
+template <typename T>
+__global__ __launch_bounds__(kLocalThreads) void k_bicg_local(LduOp<T> A,
+                                                              const T* __restrict__ rDg,
+                                                              BicgVectors<T> G, KState* kst,
+                                                              KState* kst2, double* P,
+                                                              double* P2, int pstride,
+                                                              int maxLoops) {
+  constexpr int CAP = local_bicg_max_rows<T>(), NP = local_bicg_passes<T>();
+  __shared__ T lv[kLocalBicgVectors * CAP];
+  __shared__ double lp[2 * kLocalMaxTiles];
+  __shared__ double lp2[3 * kLocalMaxTiles];
+  __shared__ KState ks;
+  const int n = A.n, tid = (int)threadIdx.x;
+  const int ntiles = (n + kSlotTile - 1) / kSlotTile;
+  const int passes = (n + kLocalThreads - 1) / kLocalThreads;
+  T* const gv[kLocalBicgVectors] = {G.r,  G.p0, G.p1, G.AyA,   G.AyA1,      G.yA,
+                                    G.rA0, G.sA, G.zA, G.tA, G.delta, G.deltaBest,
+                                    const_cast<T*>(rDg)};
+#pragma unroll
+  for (int k = 0; k < kLocalBicgVectors; ++k)
+    for (int i = tid; i < n; i += kLocalThreads) lv[k * CAP + i] = gv[k][i];
+  double tpP[2] = {0.0, 0.0}, tpP2[3] = {0.0, 0.0, 0.0};
+  if (tid < ntiles) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) tpP[j] = lp[j * kLocalMaxTiles + tid] = P[j * pstride + tid];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      tpP2[j] = lp2[j * kLocalMaxTiles + tid] = P2[j * pstride + tid];
+  }
+  if (tid < kStateWords)
+    reinterpret_cast<uint32_t*>(&ks)[tid] = reinterpret_cast<const uint32_t*>(kst)[tid];
+  LocalRows<T, NP> R;
+  R.load(A, passes);
+  __syncthreads();
+  const BicgVectors<T> V{lv,           lv + CAP,     lv + 2 * CAP, lv + 3 * CAP,
+                         lv + 4 * CAP, lv + 5 * CAP, lv + 6 * CAP, lv + 7 * CAP,
+                         lv + 8 * CAP, lv + 9 * CAP, lv + 10 * CAP, lv + 11 * CAP};
+  const T* rD = lv + 12 * CAP;
+  for (int it = 0; it < maxLoops; ++it) {
+    const bool odd = (it & 1) != 0;
+    const T* pold = odd ? V.p1 : V.p0;
+    T* pnew = odd ? V.p0 : V.p1;
+    const T* AyAold = odd ? V.AyA1 : V.AyA;
+    T* AyAnew = odd ? V.AyA : V.AyA1;
+    // the previous iteration's end and the next rho, beta
+    if (it > 0) {
+      if (tid < 64) {
+        double t[2];
+        local_fold<2>(tpP, t);
+        if (tid == 0 && (ks.status == ST_RUNNING || ks.status == ST_EARLY))
+          FinBicgUpdateRho<T>{&ks}(t);
+      }
+      __syncthreads();
+    }
+    if (ks.status != ST_RUNNING) break;
+    // k_bicg_py_spmv
+    {
+      const T beta = (T)ks.beta, omega = (T)ks.omega;
+      double tp[1] = {0.0};
+      local_phase<1>(R, n, passes, ntiles, tp, lp, [&](double* v) {
+        return BicgPyRow<T, false>{V.r, rD, pold, pnew, AyAold, AyAnew, V.yA, V.rA0, beta,
+                                   omega, v};
+      });
+      tpP[0] = tp[0];
+    }
+    // k_bicg_sz_spmv: alpha's second stage, then sA, zA, tA
+    {
+      if (tid < 64) {
+        double t[1];
+        const double tp[1] = {tpP[0]};
+        local_fold<1>(tp, t);
+        if (tid == 0 && ks.status == ST_RUNNING) FinBicgAlpha<T>{&ks}(t);
+      }
+      __syncthreads();
+      if (ks.status != ST_RUNNING) break;
+      const T alpha = (T)ks.alpha;
+      local_phase<3>(R, n, passes, ntiles, tpP2, lp2, [&](double* v) {
+        return BicgSzRow<T, false>{V.r, rD, AyAnew, V.sA, V.zA, V.tA, alpha, v};
+      });
+    }
+    // k_bicg_update_rho: the |sA| test and omega, then delta and rA
+    {
+      if (tid < 64) {
+        double t[3];
+        local_fold<3>(tpP2, t);
+        if (tid == 0 && ks.status == ST_RUNNING) FinBicgSOmega<T>{&ks}(t);
+      }
+      __syncthreads();
+      const int status = ks.status;
+      if (status != ST_RUNNING && status != ST_EARLY) break;
+      const T alpha = (T)ks.alpha;
+      if (status == ST_EARLY) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          const int r0 = p * kLocalThreads;
+          if (p < passes) bicg_early_rows<T, false>(r0, min(r0 + kLocalThreads, n), V.delta,
+                                                    V.yA, alpha);
+        }
+        __syncthreads();
+      } else {
+        const T omega = (T)ks.omega;
+        const bool copy = ks.copyBest != 0;
+        double v[NP][2];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          v[p][0] = v[p][1] = 0.0;
+          const int r0 = p * kLocalThreads;
+          if (p < passes)
+            bicg_update_rows<T, false>(r0, min(r0 + kLocalThreads, n), V.delta, V.deltaBest,
+                                       V.yA, V.zA, V.r, V.sA, V.tA, V.rA0, alpha, omega, copy,
+                                       v[p]);
+        }
+        local_tile_partials<2, NP>(v, tpP, lp, passes, ntiles);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kLocalBicgVectors - 1; ++k)
+    for (int i = tid; i < n; i += kLocalThreads) gv[k][i] = lv[k * CAP + i];
+  if (tid < ntiles) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) P[j * pstride + tid] = lp[j * kLocalMaxTiles + tid];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) P2[j * pstride + tid] = lp2[j * kLocalMaxTiles + tid];
+  }
+  if (tid < kStateWords) {
+    const uint32_t word = reinterpret_cast<const uint32_t*>(&ks)[tid];
+    reinterpret_cast<uint32_t*>(kst)[tid] = word;
+    reinterpret_cast<uint32_t*>(kst2)[tid] = word;
+  }
+}
+
+}  // namespace hipfoam

@@ -1,13 +1,15 @@
-"""The one-workgroup inner loop (k_pcg_local: the whole Chebyshev PCG loop of
-a mesh of at most 2,048 rows in one workgroup, its vectors in LDS) against
-the launched loop, through the C-ABI.
+"""The one-workgroup inner loops (k_pcg_local, k_bicg_local: the whole
+Chebyshev PCG or BiCGStab loop of a mesh of at most 2,048 rows (BiCGStab: 1,280) in
+one workgroup, its vectors in LDS) against the launched loop, through the
+C-ABI.
 
 It runs the same rows, tiles, partials and fold order as the launched
 kernels on the per-tile launch shape, so every solve must agree bit for bit:
 psi, residual, iteration counts, restarts and the refinement flags, for the
 refined solve (every branch of its driver), the plain-FP64 solve (at most
-1,536 rows) and the reference-backend inner entry (Chebyshev orders 2, 3
-and 8, iterationsPerSync > 1, maxIter exits).  The launch counter proves the
+1,536 rows for PCG, 1,280 for BiCGStab) and the reference-backend inner
+entries (Chebyshev orders 2, 3 and 8, iterationsPerSync > 1, maxIter exits,
+BiCGStab's |sA| exit).  The launch counter proves the
 one-workgroup kernel ran, and the limits keep it off where it does not
 apply."""
 
@@ -19,7 +21,8 @@ from metal4foam_amd import meshgen as mg
 
 pytestmark = pytest.mark.gpu
 
-MAX_ROWS = {False: 2048, True: 1536}  # FP32 inner loop, plain FP64
+MAX_ROWS = {False: 2048, True: 1536}  # PCG: FP32 inner loop, plain FP64
+MAX_ROWS_BICG = {False: 1280, True: 1280}
 
 
 @pytest.fixture(autouse=True)
@@ -176,9 +179,13 @@ def test_local_limits(ctx):
     assert _solve(ctx, at, True)[3] == 0
     ctx.set_schedule(blockSize=0)
     assert _solve(ctx, at, True, chebyshevOrder=1)[3] == 0
-    # BiCGStab keeps its own loops
-    cd = mg.convection_diffusion(12, 11, 10, seed=41)
-    assert _solve(ctx, cd, True, kind=H.PBICGSTAB)[3] == 0
+    # BiCGStab: 1,280 rows in either precision
+    cd_at = mg.convection_diffusion(32, 40, 1, seed=41)     # 1,280 rows
+    cd_over = mg.convection_diffusion(61, 21, 1, seed=41)   # 1,281 rows
+    assert _solve(ctx, cd_at, True, kind=H.PBICGSTAB)[3] >= 1
+    assert _solve(ctx, cd_at, True, fp64=True, kind=H.PBICGSTAB)[3] == 1
+    assert _solve(ctx, cd_over, True, kind=H.PBICGSTAB)[3] == 0
+    assert _solve(ctx, cd_over, True, fp64=True, kind=H.PBICGSTAB)[3] == 0
     ctx.set_kernel_timing(True)
     try:
         assert _solve(ctx, at, True)[3] == 0
@@ -205,3 +212,60 @@ def test_local_against_oracle_cavity(ctx, orc):
     assert perf.converged and true_res < 1e-6
     assert abs(perf.nIterations - ref.nIterations) <= max(4, ref.nIterations // 10)
     assert np.abs(psi - ref_psi).max() <= 1e-5 * np.abs(ref_psi).max()
+
+
+ASYM = {
+    "convdiff_20x20": lambda: mg.convection_diffusion(20, 20, 1, seed=41),
+    "convdiff_11x11x10": lambda: mg.convection_diffusion(11, 11, 10, seed=42),
+    "convdiff_30x30_peclet8": lambda: mg.convection_diffusion(30, 30, 1, seed=43, umax=8.0),
+    "convdiff_45x45_over_the_limit": lambda: mg.convection_diffusion(45, 45, 1, seed=44),
+    "momentum_20x20": lambda: mg.convection_diffusion(20, 20, 1, seed=1005, shift=1.0, umax=2.0),
+    "convdiff_32x40_max_rows": lambda: mg.convection_diffusion(32, 40, 1, seed=45),
+    "convdiff10_rcm_explicit_tiles": lambda: mg.rcm_renumbering(mg.random_renumbering(
+        mg.convection_diffusion(10, 10, 10, seed=46), seed=47)),
+}
+
+
+@pytest.mark.parametrize("name", list(ASYM))
+@pytest.mark.parametrize("fp64", [False, True], ids=["refined", "fp64"])
+def test_local_bicgstab_equals_launched(ctx, name, fp64):
+    """hipPBiCGStab on asymmetric convection-diffusion: the one-workgroup and
+    the launched loops bit for bit."""
+    m = ASYM[name]()
+    print(f"[local] {name}: {m.n} cells, bicgstab {'fp64' if fp64 else 'refined'}", flush=True)
+    ctx.set_topology(m)
+    assert ctx.slot_tiles()[2] == 0
+    ctl = dict(maxRestarts=6)
+    psi0, res0, perf0, used0 = _solve(ctx, m, False, fp64, kind=H.PBICGSTAB, **ctl)
+    psi1, res1, perf1, used1 = _solve(ctx, m, True, fp64, kind=H.PBICGSTAB, **ctl)
+    assert used0 == 0
+    if m.n <= MAX_ROWS_BICG[fp64]:
+        assert used1 >= 1
+    else:
+        assert used1 == 0
+    _same_perf(perf0, perf1)
+    np.testing.assert_array_equal(psi1, psi0)
+    np.testing.assert_array_equal(res1, res0)
+    assert perf1.converged, (perf1.finalResidual, perf1.nIterations)
+
+
+@pytest.mark.parametrize("max_iter", [300, 4])
+def test_local_bicgstab_inner_entry(ctx, max_iter):
+    """hipfoam_bicgstab (metalFoamBackend::bicgstab): stats and delta bit for
+    bit, through the |sA| exit (a loose target) and a maxIter exit."""
+    m = mg.convection_diffusion(11, 11, 10, seed=48)
+    ctx.set_topology(m)
+    ctx.set_coeffs(m.diag, m.upper, m.lower)
+    r = m.source.copy()
+    for target in (1e-6 * np.abs(r).sum(), 0.5 * np.abs(r).sum()):
+        out = []
+        for local in (0, -1):
+            ctx.set_schedule(persistent=0, localLoop=local)
+            before = ctx.local_launches()
+            st = ctx.bicgstab(r, target, max_iter)
+            out.append((st, ctx.delta(), ctx.local_launches() - before))
+        (s0, d0, u0), (s1, d1, u1) = out
+        assert (u0, u1) == (0, 1)
+        for k in ("nIterations", "finalResSum", "converged", "singular", "stagnated"):
+            assert getattr(s0, k) == getattr(s1, k), k
+        np.testing.assert_array_equal(d1, d0)
