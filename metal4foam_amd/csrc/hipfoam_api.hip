@@ -198,7 +198,7 @@ struct hipfoam_context {
   int64_t persistLaunches = 0;   // persistent inner loops launched
   int64_t persistAbandoned = 0;  // ... of which abandoned (the launched loop ran instead)
   int64_t persistLastAbort = 0;  // abort word of the last abandoned launch (XcdSync)
-  int64_t localLaunches = 0;     // inner loops run in one workgroup (k_pcg_local)
+  int64_t localLaunches = 0;     // inner loops run in one workgroup (k_pcg_local, k_bicg_local)
   int simAbandon = 0;            // persistent launches still to report as abandoned
   hipEvent_t evPoll[2] = {nullptr, nullptr};
   hipEvent_t evT0 = nullptr, evT1 = nullptr;
@@ -944,7 +944,7 @@ int persistent_enqueue(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T
 // After evT1 has completed and the state has been copied to c->hk: the GPU
 // time of the launch, and whether it ended the loop.  A state left running
 // means the launch was abandoned (*abandoned: the barrier words are reset
-// and the caller starts the solve again on the launched loop) or ran out of
+// and the caller solves again, from the restored start) or ran out of
 // iterations (an error).
 int persistent_finish(hipfoam_context* c, double* gpuSeconds, bool* abandoned) {
   float ms = 0.f;
@@ -977,10 +977,10 @@ int persistent_loop(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* r
 }
 
 // The iteration loop of an initialised solve (BiCGStab: its start, then the
-// loop): one persistent launch where persistent_ok allows, else launched
-// iterations (run_loop).  If the persistent launch was abandoned, reinit()
-// restores the vectors and the state the loop starts from, and the launched
-// loop runs.
+// loop): one launch in one workgroup where local_ok allows, else one
+// persistent launch where persistent_ok allows, else launched iterations
+// (run_loop).  If the persistent launch was abandoned, reinit() restores the
+// vectors and the state the loop starts from, and the launched loop runs.
 template <typename T, typename Reinit>
 int iterate(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD, int order,
             int maxIter, double* gpuSeconds, Reinit reinit) {
