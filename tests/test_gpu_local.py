@@ -53,6 +53,12 @@ MESHES = {
     "poisson12_rcm_explicit_tiles": lambda: mg.rcm_renumbering(mg.random_renumbering(
         mg.poisson(12, 12, 12, seed=15, lognormal=True), seed=16)),
     "polish": polish_case,
+    # pass and tile boundaries of the workgroup: 6 rows, one wave, a one-row
+    # last tile, a second pass of one row
+    "poisson_3x2x1": lambda: mg.poisson(3, 2, 1, seed=18, lognormal=True),
+    "chain_64": lambda: mg.poisson(64, 1, 1, seed=19, lognormal=False),
+    "chain_257": lambda: mg.poisson(257, 1, 1, seed=20, lognormal=False),
+    "chain_513": lambda: mg.poisson(513, 1, 1, seed=21, lognormal=False),
 }
 
 
@@ -223,6 +229,9 @@ ASYM = {
     "convdiff_32x40_max_rows": lambda: mg.convection_diffusion(32, 40, 1, seed=45),
     "convdiff10_rcm_explicit_tiles": lambda: mg.rcm_renumbering(mg.random_renumbering(
         mg.convection_diffusion(10, 10, 10, seed=46), seed=47)),
+    "convdiff_3x2x1": lambda: mg.convection_diffusion(3, 2, 1, seed=49),
+    "convdiff_257x2x1": lambda: mg.convection_diffusion(257, 2, 1, seed=50),
+    "convdiff_513x2x1": lambda: mg.convection_diffusion(513, 2, 1, seed=51),
 }
 
 
