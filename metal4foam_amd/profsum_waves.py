@@ -1,12 +1,14 @@
-"""Per-wavefront instruction and wait counts of the batched iteration kernels
-from rocprofv3 --pmc runs (SQ_*, TCP_* counters): how many vector-memory,
-VALU, scalar-ALU and scalar-memory instructions one wavefront issues, how
-long it lives, how much of that it waits, and the mean L2 read latency its
-vector loads see.  Each counter is averaged over the kernel's launches in the
-top quartile of its value (the launches where every segment still runs).
+"""Per-wavefront instruction and wait counts of iteration kernels from
+rocprofv3 --pmc runs (SQ_*, TCP_* counters): how many vector-memory, VALU,
+scalar-ALU and scalar-memory instructions one wavefront issues, how long it
+lives, how much of that it waits, and the mean L2 read latency its vector
+loads see.  Each counter is averaged over the kernel's launches in the top
+quartile of its value (the launches that do the full work: every segment
+still running, no launch stopped at entry).  By default the batched
+iteration kernels; --kernel and --title select others.
 
     python -m metal4foam_amd.profsum_waves --pmc A.csv --pmc B.csv \\
-        [--command CMD ...] [--note TEXT ...]
+        [--kernel NAME ...] [--title TEXT] [--command CMD ...] [--note TEXT ...]
 """
 
 import argparse
@@ -45,12 +47,16 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--pmc", action="append", required=True,
                     help="run_counter_collection.csv of one --pmc pass (repeatable)")
+    ap.add_argument("--kernel", action="append", default=[],
+                    help="kernel name as the trace shows it, without 'hipfoam::' and the "
+                         "argument list (repeatable; default: the batched kernels)")
+    ap.add_argument("--title", default="Batched iteration kernels, 64 systems x 5 RHS: "
+                    "per-wavefront instructions, wave life and L2 read latency (MI355X)")
     ap.add_argument("--command", action="append", default=[])
     ap.add_argument("--note", action="append", default=[])
     a = ap.parse_args()
     res = load(a.pmc)
-    print("# Batched iteration kernels, 64 systems x 5 RHS: per-wavefront instructions, "
-          "wave life and L2 read latency (MI355X)\n")
+    print(f"# {a.title}\n")
     for c in a.command:
         print(f"    {c}")
     print()
@@ -60,7 +66,7 @@ def main():
     print("| kernel | waves | VMEM reads | VMEM writes | VALU | SALU | SMEM | wave life "
           "(cycles) | waiting (cycles) | waiting share | L2 read latency |")
     print("|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|")
-    for k in KERNELS:
+    for k in a.kernel or KERNELS:
         r = res.get(k)
         if not r or not r.get("SQ_WAVES"):
             continue
