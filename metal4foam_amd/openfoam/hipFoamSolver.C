@@ -111,6 +111,30 @@ hipfoam_context* Foam::hipFoamSolver::context() const
             slot->ctx,
             check == "pointers" ? HIPFOAM_TOPO_POINTERS : HIPFOAM_TOPO_CONTENT
         );
+        // how the inner loop runs on small meshes (every choice gives the
+        // same results): "persistent" auto|on|off, the loop as one launch on
+        // one XCD; "localLoop" auto|off, in one workgroup.  Off suits a
+        // caller whose own kernels on other streams occupy the GPU.
+        const word persistent = hipDict_.getOrDefault<word>("persistent", "auto");
+        if (persistent != "auto" && persistent != "on" && persistent != "off")
+        {
+            FatalErrorInFunction
+                << "hip.persistent must be auto, on or off, not "
+                << persistent << exit(FatalError);
+        }
+        const word localLoop = hipDict_.getOrDefault<word>("localLoop", "auto");
+        if (localLoop != "auto" && localLoop != "off")
+        {
+            FatalErrorInFunction
+                << "hip.localLoop must be auto or off, not "
+                << localLoop << exit(FatalError);
+        }
+        hipfoam_schedule sched;
+        hipfoam_get_schedule(slot->ctx, &sched);
+        sched.persistent =
+            persistent == "auto" ? -1 : (persistent == "on" ? 1 : 0);
+        sched.localLoop = localLoop == "auto" ? -1 : 0;
+        hipfoam_set_schedule(slot->ctx, &sched);
     }
     return slot->ctx;
 }

@@ -153,6 +153,19 @@ def test_unknown_topology_check_is_fatal(driver, tmp_path):
     assert "hip.topologyCheck must be content or pointers, not sometimes" in fatal[0]
 
 
+@pytest.mark.parametrize("key,value,message", [
+    ("persistent", "sometimes", "hip.persistent must be auto, on or off, not sometimes"),
+    ("localLoop", "on", "hip.localLoop must be auto or off, not on"),
+])
+def test_unknown_loop_keys_are_fatal(driver, tmp_path, key, value, message):
+    """hip { persistent auto|on|off; localLoop auto|off; } only."""
+    m = small_poisson()
+    r, res, fatal, _ = run_driver(driver, tmp_path, m, np.zeros(m.n), "hipPCG",
+                                  hip={key: value})
+    assert r.returncode == 3 and not res
+    assert message in fatal[0]
+
+
 # ---------------------------------------------------------------------------
 # GPU: the plugin solve
 # ---------------------------------------------------------------------------
@@ -286,3 +299,23 @@ def test_plugin_topology_check_keyword(driver, tmp_path, ctx, check):
     psi1, _, _ = _direct_solve(ctx, m, H.PCG, psi0, **ctl)
     psi2, _, _ = _direct_solve(ctx, m, H.PCG, psi1, **ctl)
     assert np.array_equal(arrays[0], psi2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("persistent,local", [("auto", "auto"), ("off", "off"), ("on", "off")])
+def test_plugin_loop_keywords(driver, tmp_path, ctx, persistent, local):
+    """hip { persistent ...; localLoop ...; }: every choice of how the inner
+    loop runs gives the direct C-ABI solve's psi bit for bit."""
+    from metal4foam_amd import hipfoam as H
+
+    m = small_poisson()
+    psi0 = np.zeros(m.n)
+    r, res, _, arrays = run_driver(driver, tmp_path, m, psi0, "hipPCG", tolerance=1e-7,
+                                   rel_tol=0.0, maxIter=500,
+                                   hip=dict(persistent=persistent, localLoop=local,
+                                            maxRestarts=6))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert len(res) == 1 and res[0]["converged"]
+    psi1, _, _ = _direct_solve(ctx, m, H.PCG, psi0, tolerance=1e-7, relTol=0.0, maxIter=500,
+                               maxRestarts=6)
+    assert np.array_equal(arrays[0], psi1)
