@@ -400,7 +400,7 @@ typedef struct
     int32_t localLoop;    /* inner loop in one workgroup of 512 threads
                              with every vector in LDS, on a mesh whose
                              tiles are all row-slot tiles, on the per-tile
-                             launch shape: Chebyshev PCG up to 2,048 rows
+                             launch shape: Chebyshev PCG up to 2,816 rows
                              (FP32 inner loop) or 1,536 (plain FP64),
                              BiCGStab up to 1,280 rows:
                              -1 = auto (default), 0 = never.  Taken before

@@ -870,9 +870,18 @@ int local_enqueue(hipfoam_context* c, Kind kind, const LduOp<T>& A, const T* rD,
   if (kind == KIND_PCG) {
     const PcgVectors<T> V{w.r.p,     w.r1.p,        w.p0.p, w.p1.p, w.wA.p, w.Ap.p,
                           w.delta.p, w.deltaBest.p, w.z0.p, w.z1.p, w.w.p};
-    hipLaunchKernelGGL((k_pcg_local<T>), dim3(1), dim3(kLocalThreads), 0, c->stream, A, rD, V,
-                       c->kst.p, c->kst2.p, c->partials.p, c->partials2.p, c->pstride, order,
-                       loops);
+    const auto launch = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(1), dim3(kLocalThreads), 0, c->stream, A, rD, V, c->kst.p,
+                         c->kst2.p, c->partials.p, c->partials2.p, c->pstride, order, loops);
+    };
+    if constexpr (sizeof(T) == sizeof(float)) {
+      if (local_pcg_passes<T>(c->n) == 4)
+        launch(k_pcg_local<T, 4>);
+      else
+        launch(k_pcg_local<T, 6>);
+    } else {
+      launch(k_pcg_local<T, 3>);
+    }
   } else {
     const BicgVectors<T> V{w.r.p,  w.p0.p, w.p1.p, w.AyA.p, w.AyA1.p,  w.yA.p,
                            w.rA0.p, w.sA.p, w.zA.p, w.tA.p,  w.delta.p, w.deltaBest.p};

@@ -1,4 +1,4 @@
-// hipfoam_local.hpp -- the inner loop of a mesh of at most 2,048 rows in one
+// hipfoam_local.hpp -- the inner loop of a mesh of at most 2,816 rows in one
 // workgroup: every vector of the solve in LDS, every thread's matrix rows in
 // registers.
 //
@@ -6,8 +6,8 @@
 // phase on a flag barrier across workgroups and a round of L2 loads of the
 // rows and the vectors (about 1.8 us a phase).  Here one workgroup of 512
 // threads runs the whole inner loop on one CU:
-//  - the twelve vectors of the solve sit in LDS (at most 2,048 FP32 rows,
-//    96 KiB, or 1,536 FP64 rows, 144 KiB, of the CU's 160 KiB), so the
+//  - the twelve vectors of the solve sit in LDS (at most 3,072 FP32 rows,
+//    144 KiB, or 1,536 FP64 rows, 144 KiB, of the CU's 160 KiB), so the
 //    gathers of a phase are LDS loads;
 //  - each thread holds the rows it owns (row p * 512 + thread in pass p) in
 //    registers from the first phase to the last, so no phase reads the
@@ -25,7 +25,8 @@
 // (patterned or explicit, no face tiles) qualify.  Measured against the
 // one-XCD loop (2D Poisson, refined solve, same box): 5.7 against 15.8 us
 // per iteration at 400 rows, 9.7 against 15.8 at 1,536, 11.8 against 15.8 at
-// 2,048; plain FP64 4.9 against 16.9 at 400 and 9.7 against 17.3 at 1,536.
+// 2,048, 13.4 against 15.9 at 2,304 (six passes); plain FP64 4.9 against
+// 16.9 at 400 and 9.7 against 17.3 at 1,536.
 #pragma once
 
 #include "hipfoam_persist.hpp"
@@ -35,25 +36,29 @@ namespace hipfoam {
 constexpr int kLocalThreads = 512;
 constexpr int kLocalTilesPerPass = kLocalThreads / kSlotTile;
 constexpr int kLocalVectors = 12;  // the 11 of PcgVectors and rD
-constexpr int kLocalMaxTiles = 8;
+constexpr int kLocalMaxTiles = 12;
 
-// Rows one workgroup holds: 2,048 FP32 rows (96 KiB of vectors in LDS) or
-// 1,536 FP64 rows (144 KiB).
+// Rows of the one-workgroup PCG loop: 2,816 FP32 rows or 1,536 FP64 rows.
+// k_pcg_local<T, NP> holds NP passes of kLocalThreads rows (NP * 24 KiB of
+// FP32 vectors in LDS): FP32 meshes up to 2,048 rows run the four-pass
+// build, larger ones the six-pass build, which is 0.7-0.9 us per iteration
+// slower on the small meshes (more unrolled code, more registers) but beats
+// the one-XCD loop up to about 2,800 rows (2,500 rows: 14.2 against 15.8 us
+// per iteration; 3,072 rows: 15.7 against 15.5).
 template <typename T>
 constexpr int local_max_rows() {
-  return sizeof(T) == sizeof(float) ? 2048 : 1536;
+  return sizeof(T) == sizeof(float) ? 2816 : 1536;
 }
+// passes of the build that runs a mesh of n rows
 template <typename T>
-constexpr int local_passes() {
-  return (local_max_rows<T>() + kLocalThreads - 1) / kLocalThreads;
+inline int local_pcg_passes(int n) {
+  return sizeof(T) == sizeof(float) ? (n <= 4 * kLocalThreads ? 4 : 6) : 3;
 }
-static_assert(local_max_rows<float>() <= kLocalMaxTiles * kSlotTile &&
-                  local_max_rows<double>() <= kLocalMaxTiles * kSlotTile,
-              "tile partials of one workgroup");
+static_assert(6 * kLocalThreads <= kLocalMaxTiles * kSlotTile, "tile partials of one workgroup");
 static_assert(kSlotTile % 64 == 0 && kLocalThreads % kSlotTile == 0, "whole tiles per pass");
 static_assert(kLocalMaxTiles <= 64, "wave 0 holds one tile partial per lane");
-static_assert(kLocalVectors * 2048 * sizeof(float) <= 144 * 1024 &&
-                  kLocalVectors * 1536 * sizeof(double) <= 144 * 1024,
+static_assert(kLocalVectors * 6 * kLocalThreads * sizeof(float) <= 144 * 1024 &&
+                  kLocalVectors * 3 * kLocalThreads * sizeof(double) <= 144 * 1024,
               "vectors within 144 KiB of LDS");
 
 // This thread's row of each pass, in registers for the whole launch.
@@ -155,19 +160,19 @@ __device__ __forceinline__ void local_phase(const LocalRows<T, NP>& R, int n, in
 // The Chebyshev PCG loop (order >= 2) of k_pcg_persistent -- k_cheb_first
 // with FinAlpha merged, k_cheb_step for steps 2 .. order-1, k_pcg_p_spmv with
 // FinResBeta merged, r and p ping-ponging on the iteration parity -- in one
-// workgroup of kLocalThreads threads, for a mesh of at most
-// local_max_rows<T>() rows in row-slot tiles only.  The vectors G and rD, the
+// workgroup of kLocalThreads threads, for a mesh of at most NP * kLocalThreads
+// rows in row-slot tiles only.  The vectors G and rD, the
 // partials P and P2 and the state *kst2 are loaded at the start (vectors and
 // state into LDS, the partials also into wave 0's lanes); the vectors and
 // partials are stored back at the end and the state into *kst and *kst2.
-template <typename T>
+template <typename T, int NP>
 __global__ __launch_bounds__(kLocalThreads) void k_pcg_local(LduOp<T> A,
                                                              const T* __restrict__ rDg,
                                                              PcgVectors<T> G, KState* kst,
                                                              KState* kst2, double* P, double* P2,
                                                              int pstride, int order,
                                                              int maxLoops) {
-  constexpr int CAP = local_max_rows<T>(), NP = local_passes<T>();
+  constexpr int CAP = NP * kLocalThreads;
   __shared__ T lv[kLocalVectors * CAP];
   __shared__ double lp[2 * kLocalMaxTiles];
   __shared__ double lp2[kLocalMaxTiles];

@@ -1,6 +1,7 @@
 """The one-workgroup inner loops (k_pcg_local, k_bicg_local: the whole
-Chebyshev PCG or BiCGStab loop of a mesh of at most 2,048 rows (BiCGStab: 1,280) in
-one workgroup, its vectors in LDS) against the launched loop, through the
+Chebyshev PCG or BiCGStab loop of a mesh of at most 2,816 rows (BiCGStab: 1,280) in
+one workgroup, its vectors in LDS; PCG meshes of more than 2,048 rows run a
+six-pass build) against the launched loop, through the
 C-ABI.
 
 It runs the same rows, tiles, partials and fold order as the launched
@@ -21,7 +22,7 @@ from metal4foam_amd import meshgen as mg
 
 pytestmark = pytest.mark.gpu
 
-MAX_ROWS = {False: 2048, True: 1536}  # PCG: FP32 inner loop, plain FP64
+MAX_ROWS = {False: 2816, True: 1536}  # PCG: FP32 inner loop, plain FP64
 MAX_ROWS_BICG = {False: 1280, True: 1280}
 
 
@@ -44,10 +45,13 @@ def polish_case():
 MESHES = {
     "cavity_20x20": lambda: mg.poisson(20, 20, 1, seed=1000, lognormal=False),
     "cavity_45x45": lambda: mg.poisson(45, 45, 1, seed=1000, lognormal=False),
-    "cavity_50x50_over_the_limit": lambda: mg.poisson(50, 50, 1, seed=1000, lognormal=False),
+    "cavity_50x50_six_passes": lambda: mg.poisson(50, 50, 1, seed=1000, lognormal=False),
+    "cavity_54x54_over_the_limit": lambda: mg.poisson(54, 54, 1, seed=1000, lognormal=False),
     "poisson_7x9x5": lambda: mg.poisson(7, 9, 5, seed=12, lognormal=True),
     "poisson12_lognormal": lambda: mg.poisson(12, 12, 12, seed=11, lognormal=True),
-    "poisson_32x64_max_rows": lambda: mg.poisson(32, 64, 1, seed=17, lognormal=True),
+    "poisson_32x64_four_pass_max": lambda: mg.poisson(32, 64, 1, seed=17, lognormal=True),
+    "poisson_41x50_six_pass_min": lambda: mg.poisson(41, 50, 1, seed=22, lognormal=True),
+    "poisson_44x64_max_rows": lambda: mg.poisson(44, 64, 1, seed=17, lognormal=True),
     "poisson_11x10x9_unpinned_rhs": lambda: mg.poisson(11, 10, 9, seed=13, lognormal=True,
                                                        zero_mean_rhs=False),
     "poisson12_rcm_explicit_tiles": lambda: mg.rcm_renumbering(mg.random_renumbering(
@@ -159,12 +163,12 @@ def test_local_inner_entry(ctx, order, ips, max_iter):
 
 
 def test_local_limits(ctx):
-    """The one-workgroup loop takes at most 2,048 rows (FP32 inner loop) or
+    """The one-workgroup loop takes at most 2,816 rows (FP32 inner loop) or
     1,536 (plain FP64), only row-slot tiles, only the per-tile launch shape,
     only Chebyshev PCG, and never under kernel timing; elsewhere the one-XCD
     loop or the launched loop runs, with the same result."""
-    at = mg.poisson(32, 64, 1, seed=21, lognormal=True)     # 2,048 rows
-    over = mg.poisson(683, 3, 1, seed=21, lognormal=True)   # 2,049 rows
+    at = mg.poisson(44, 64, 1, seed=21, lognormal=True)     # 2,816 rows
+    over = mg.poisson(313, 9, 1, seed=21, lognormal=True)   # 2,817 rows
     at64 = mg.poisson(48, 32, 1, seed=21, lognormal=True)   # 1,536 rows
     over64 = mg.poisson(53, 29, 1, seed=21, lognormal=True)  # 1,537 rows
     assert _solve(ctx, at, True)[3] >= 1
