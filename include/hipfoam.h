@@ -402,7 +402,7 @@ typedef struct
                              tiles are all row-slot tiles, on the per-tile
                              launch shape: Chebyshev PCG up to 2,816 rows
                              (FP32 inner loop) or 1,536 (plain FP64),
-                             BiCGStab up to 1,280 rows:
+                             BiCGStab up to 1,536 (FP64 1,280) rows:
                              -1 = auto (default), 0 = never.  Taken before
                              persistent where both apply; the same rows,
                              partials and fold order as the launched loop:

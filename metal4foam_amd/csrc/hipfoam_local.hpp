@@ -303,9 +303,9 @@ __global__ __launch_bounds__(kLocalThreads) void k_pcg_local(LduOp<T> A,
 
 namespace hipfoam {
 
-// Rows of the one-workgroup BiCGStab loop: 1,280 in either precision
-// (thirteen vectors, the twelve of BicgVectors and rD: 65 KiB FP32, 130 KiB
-// FP64).  A one-XCD BiCGStab iteration is only three phases (9.1-9.4 us at
+// Rows of the one-workgroup BiCGStab loop: 1,536 FP32 rows and 1,280 FP64
+// rows, three passes of 512 (thirteen vectors, the twelve of BicgVectors and
+// rD: 78 KiB FP32, 130 KiB FP64).  A one-XCD BiCGStab iteration is only three phases (9.1-9.4 us at
 // 400 to 2,048 rows), so one CU stops paying earlier than for PCG: measured
 // against it (refined solve, 2D convection-diffusion, same box), 4.2 against
 // 9.1 us per iteration at 400 rows and 7.2 against 9.4 at 1,280; plain FP64
@@ -314,7 +314,7 @@ namespace hipfoam {
 constexpr int kLocalBicgVectors = 13;
 template <typename T>
 constexpr int local_bicg_max_rows() {
-  return 1280;
+  return sizeof(T) == sizeof(float) ? 1536 : 1280;
 }
 template <typename T>
 constexpr int local_bicg_passes() {

@@ -1,5 +1,5 @@
 """The one-workgroup inner loops (k_pcg_local, k_bicg_local: the whole
-Chebyshev PCG or BiCGStab loop of a mesh of at most 2,816 rows (BiCGStab: 1,280) in
+Chebyshev PCG or BiCGStab loop of a mesh of at most 2,816 rows (BiCGStab: 1,536) in
 one workgroup, its vectors in LDS; PCG meshes of more than 2,048 rows run a
 six-pass build) against the launched loop, through the
 C-ABI.
@@ -23,7 +23,7 @@ from metal4foam_amd import meshgen as mg
 pytestmark = pytest.mark.gpu
 
 MAX_ROWS = {False: 2816, True: 1536}  # PCG: FP32 inner loop, plain FP64
-MAX_ROWS_BICG = {False: 1280, True: 1280}
+MAX_ROWS_BICG = {False: 1536, True: 1280}
 
 
 @pytest.fixture(autouse=True)
@@ -189,13 +189,16 @@ def test_local_limits(ctx):
     assert _solve(ctx, at, True)[3] == 0
     ctx.set_schedule(blockSize=0)
     assert _solve(ctx, at, True, chebyshevOrder=1)[3] == 0
-    # BiCGStab: 1,280 rows in either precision
-    cd_at = mg.convection_diffusion(32, 40, 1, seed=41)     # 1,280 rows
-    cd_over = mg.convection_diffusion(61, 21, 1, seed=41)   # 1,281 rows
+    # BiCGStab: 1,536 rows (FP32 inner loop) or 1,280 (plain FP64)
+    cd_at = mg.convection_diffusion(32, 48, 1, seed=41)     # 1,536 rows
+    cd_over = mg.convection_diffusion(53, 29, 1, seed=41)   # 1,537 rows
+    cd_at64 = mg.convection_diffusion(32, 40, 1, seed=41)   # 1,280 rows
+    cd_over64 = mg.convection_diffusion(61, 21, 1, seed=41)  # 1,281 rows
     assert _solve(ctx, cd_at, True, kind=H.PBICGSTAB)[3] >= 1
-    assert _solve(ctx, cd_at, True, fp64=True, kind=H.PBICGSTAB)[3] == 1
     assert _solve(ctx, cd_over, True, kind=H.PBICGSTAB)[3] == 0
-    assert _solve(ctx, cd_over, True, fp64=True, kind=H.PBICGSTAB)[3] == 0
+    assert _solve(ctx, cd_at64, True, fp64=True, kind=H.PBICGSTAB)[3] == 1
+    assert _solve(ctx, cd_over64, True, fp64=True, kind=H.PBICGSTAB)[3] == 0
+    assert _solve(ctx, cd_over64, True, kind=H.PBICGSTAB)[3] >= 1
     ctx.set_kernel_timing(True)
     try:
         assert _solve(ctx, at, True)[3] == 0
@@ -230,7 +233,8 @@ ASYM = {
     "convdiff_30x30_peclet8": lambda: mg.convection_diffusion(30, 30, 1, seed=43, umax=8.0),
     "convdiff_45x45_over_the_limit": lambda: mg.convection_diffusion(45, 45, 1, seed=44),
     "momentum_20x20": lambda: mg.convection_diffusion(20, 20, 1, seed=1005, shift=1.0, umax=2.0),
-    "convdiff_32x40_max_rows": lambda: mg.convection_diffusion(32, 40, 1, seed=45),
+    "convdiff_32x40_fp64_max_rows": lambda: mg.convection_diffusion(32, 40, 1, seed=45),
+    "convdiff_32x48_max_rows": lambda: mg.convection_diffusion(32, 48, 1, seed=48),
     "convdiff10_rcm_explicit_tiles": lambda: mg.rcm_renumbering(mg.random_renumbering(
         mg.convection_diffusion(10, 10, 10, seed=46), seed=47)),
     "convdiff_3x2x1": lambda: mg.convection_diffusion(3, 2, 1, seed=49),
